@@ -1,0 +1,116 @@
+/*
+ * teuk_oracle.h -- CPU FP64 restatement of teuk-kokkos's linear per-stage
+ * Teukolsky operator and RK4 stepper.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it.  The product (teuk_kokkos_amd/) never links, loads or calls it.
+ *
+ * It is a nodal restatement that follows the reference's own structure
+ * (nodal (mode, field=13, radial, theta) state, per-mode analyze/diag/
+ * synthesize angular Laplacian, three-pass radial RHS, per-field band
+ * projection, four-stage classical RK4), so it is independent of the modal
+ * formulation used on the GPU.  Each function cites the reference file:line
+ * it follows.
+ *
+ * Parity pins: see tests/test_oracle_*.py (SBP identities, exactness and
+ * orders, dissipation energy identity and spectral radius, coefficient point
+ * oracle, Gauss-Legendre moments, harmonic closed forms, orthonormality,
+ * Galerkin residual, band invariance, RK4 temporal order, QNM values).
+ * Complex division uses the libstdc++ algorithm; the reference uses
+ * Kokkos::complex division, whose bitwise rounding is "parity unpinned".
+ */
+#ifndef TEUK_ORACLE_H
+#define TEUK_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Radial scheme ids: 0 = d4-2, 1 = d8-4, 2 = d10-5. */
+/* Reduction ids: 0 = free_damped, 1 = stage_constrained. */
+
+typedef struct OracleParams {
+  double mass, spin, length;      /* Kerr M, a, compactification L        */
+  int nr, ntheta;                 /* radial points, Gauss-Legendre nodes  */
+  int ell_max_first, ell_max_second;
+  int n_modes;                    /* stored signed modes (sorted)         */
+  int modes[64];
+  int n_parents;                  /* first-order modes                    */
+  int parents[64];
+  int n_targets;                  /* second-order modes                   */
+  int targets[64];
+  double reduction_damping;       /* gamma                                */
+  double dissipation;             /* epsilon                              */
+  int reduction;                  /* 0 free_damped, 1 stage_constrained   */
+  int scheme;                     /* 0 d4-2, 1 d8-4, 2 d10-5              */
+  int threads;                    /* OpenMP threads (<=0: runtime default)*/
+} OracleParams;
+
+typedef struct OraclePipeline OraclePipeline;
+
+/* Returns NULL on invalid parameters (message via oracle_last_error). */
+OraclePipeline* oracle_create(const OracleParams* params);
+void oracle_destroy(OraclePipeline* p);
+const char* oracle_last_error(void);
+
+/* State is interleaved complex128 (re, im) in (mode, field=13, radial,
+ * theta) order: 2 * M * 13 * nr * ntheta doubles. */
+long long oracle_value_count(const OraclePipeline* p);
+void oracle_get_state(const OraclePipeline* p, double* out);
+void oracle_set_state(OraclePipeline* p, const double* in);
+
+/* Gaussian initial data (pipeline_initial_data.hpp:211-340).  center and
+ * width are absolute compactified radii.  Seeds are already expanded (sharp
+ * partners included) and sorted by (m, ell).  Returns the Galerkin residual
+ * or a negative value on failure. */
+double oracle_init_gaussian(OraclePipeline* p, double center, double width,
+                            int compact_support, int n_seeds, const int* ells,
+                            const int* ms, const double* amp_re,
+                            const double* amp_im);
+
+/* One spatial RHS (spatial_pipeline.hpp:458-515, Disabled policy). */
+void oracle_evaluate_rhs(OraclePipeline* p, const double* stage, double* out);
+/* Classical RK4 steps of the owned state (device_rk4.hpp:84-128). */
+void oracle_step(OraclePipeline* p, double time, double dt);
+void oracle_advance(OraclePipeline* p, int steps, double time0, double dt);
+
+/* Modal endpoint coefficients of FirstPsi (waveform_output.hpp:40-160) for
+ * one stored mode index: out[2*n_ell] at scri (radial 0) and horizon
+ * (radial nr-1); returns n_ell or -1. */
+int oracle_endpoint_modal(const OraclePipeline* p, int mode_index,
+                          double* scri_out, double* horizon_out);
+
+/* ---- primitives used by the known-answer tests ---- */
+void oracle_gauss_legendre(int n, double* x, double* w);
+double oracle_swsh_theta(int ell, int m, int spin, double theta);
+/* S is ntheta x n_ell row-major, A is n_ell x ntheta row-major. */
+int oracle_transform_matrices(int spin, int m, int ell_max, int ntheta,
+                              double* S, double* A);
+/* out: C_T, K, H_R, Re G, Im G, Re q, Im q, Re psi, Im psi. */
+void oracle_teukolsky_coefficients(double mass, double spin, double length,
+                                   int spin_weight, int m, double radius,
+                                   double theta, double* out);
+double oracle_sbp_derivative_entry(int scheme, int n, int row, int column);
+double oracle_sbp_norm_weight(int scheme, int n, int index);
+int oracle_sbp_minimum_points(int scheme);
+/* complex lines (interleaved), length n */
+void oracle_sbp_derivative(int scheme, int n, double h, const double* in,
+                           double* out);
+void oracle_sbp_dissipation(int scheme, int n, double h, double strength,
+                            const double* in, double* out);
+double oracle_dissipation_spectral_radius_bound(int scheme);
+int oracle_dissipation_step_admissible(int scheme, double strength,
+                                       double step_over_spacing);
+/* maximum |coordinate velocity| over the grid (solver_driver.hpp:36-57) */
+double oracle_maximum_coordinate_speed(double mass, double spin,
+                                       double length, int nr, int ntheta);
+/* dense complex partial-pivot solve (dense_solve.hpp:20-72); returns the
+ * relative residual or -1 on a singular matrix. */
+double oracle_dense_solve(int n, const double* matrix, const double* rhs,
+                          double* solution);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

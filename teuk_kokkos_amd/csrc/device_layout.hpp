@@ -1,0 +1,62 @@
+// Device-side descriptors shared by the plan (host) and the stage kernels.
+//
+// Evolved state layout in HBM (per buffer y, acc, Ya, Yb):
+//   for every radial line (instance, active first-order mode m):
+//     [radial r = 0..nr-1][field f in {P, Q, Psi}][ell = ell_min..ell_max]
+//   complex128 values, one contiguous block per line at LineDesc::base.
+// A radial tile of a line is therefore one contiguous byte range, loaded by
+// the whole workgroup with coalesced 16-byte accesses.
+#pragma once
+
+#include "physics.hpp"
+
+namespace teuk {
+
+inline constexpr int kStageThreads = 256;
+inline constexpr int kMaxClosureRows = 11;
+inline constexpr int kMaxClosureWidth = 16;
+inline constexpr int kMaxDissOrder = 6;
+
+struct LineDesc {
+  long long base;   // cdouble offset of this line's block
+  int nr, n_ell, ntheta, ell_min;
+  int m, constrained, kerr, dissipate;
+  int mat_off;      // double offset of S (ntheta x n_ell) then A (n_ell x ntheta)
+  int node_off;     // double offset of sin(theta_j)[ntheta] then cos(theta_j)[ntheta]
+  double mass, spin, length, h, inv_h, gamma, eps, dt;
+};
+
+struct TileDesc {
+  int line;
+  int t0, t1;    // output rows
+  int lo, hi;    // loaded rows
+  int mlo, mhi;  // rows of the intermediate psi_t (and D psi when constrained)
+  int lds_bytes;
+};
+
+// One SBP scheme: closure rows of h*D, interior antisymmetric stencil,
+// boundary norm weights and the rows of A^T A for the dissipation.
+struct SchemeTable {
+  int W, C, B, order;
+  double left[kMaxClosureRows * kMaxClosureWidth];
+  double interior[6];
+  double norm[kMaxClosureRows];
+  double diss_left[kMaxDissOrder][2 * kMaxDissOrder + 1];
+  double diss_interior[2 * kMaxDissOrder + 1];
+};
+
+// Rows of the nodal psi_t chunk processed at once by a workgroup.
+__host__ __device__ inline int nodal_chunk_rows(int ntheta) {
+  const int r = 512 / ntheta;
+  return r < 1 ? 1 : r;
+}
+
+__host__ __device__ inline int tile_lds_bytes(int rows, int mrows, int n_ell, int ntheta,
+                                              int constrained, int kerr) {
+  long long b = 16LL * rows * 3 * n_ell + 16LL * mrows * n_ell;
+  if (constrained) b += 16LL * mrows * n_ell;
+  if (kerr) b += 16LL * ntheta * n_ell + 16LL * nodal_chunk_rows(ntheta) * ntheta;
+  return static_cast<int>(b);
+}
+
+}  // namespace teuk

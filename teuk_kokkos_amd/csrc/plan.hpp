@@ -1,0 +1,132 @@
+// Plan: the device-resident replacement of the reference's SpatialPipeline
+// (include/teuk/spatial_pipeline.hpp:144-620) for the linear first-order
+// path, batched over any number of independent instances.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "config.hpp"
+#include "device_layout.hpp"
+#include "host_math.hpp"
+
+namespace teuk {
+
+struct StageArgs;
+
+struct WaveformRecordC {
+  std::uint64_t step;
+  double time;
+  int boundary;  // 0 scri, 1 horizon
+  int order;     // 1 first (FirstPsi), 2 second (SecondPsi)
+  int m;
+  int ell;
+  double rescaled_re, rescaled_im;
+  double endpoint_re, endpoint_im;
+};
+
+struct SourceActivation {
+  int active = 0;
+  int consecutive_passes = 0;
+  double activation_time = -1.0;
+  double last_eligibility_time = -1.0;
+};
+
+class Plan {
+ public:
+  struct Instance {
+    RunParameters params;
+    ModeRegistry registry;
+    int M = 0, N = 0, T = 0;
+    double horizon = 0, h = 0, dt = 0;
+    double time = 0;
+    std::uint64_t steps_done = 0;
+    std::vector<int> line_of_mode;   // stored mode index -> line, -1 if not first order
+    std::vector<double> sin_theta, cos_theta;
+    // Entries of an imported nodal state that the linear RHS never changes
+    // (fields 3..12, inactive modes) and the off-band residual of the
+    // evolved fields; empty when the state was generated in band.
+    std::vector<cplx> passive;
+    std::vector<cplx> residual;
+    SourceActivation activation;
+    int scheme_group = 0, tile_first = 0, tile_count = 0;
+    explicit Instance(RunParameters p);
+  };
+
+  explicit Plan(const std::vector<RunParameters>& params);
+  ~Plan();
+  Plan(const Plan&) = delete;
+  Plan& operator=(const Plan&) = delete;
+
+  int instance_count() const { return static_cast<int>(inst_.size()); }
+  const Instance& instance(int i) const { return inst_.at(i); }
+  long long nodal_value_count(int i) const;
+  long long modal_value_count() const { return modal_count_; }
+  double grid_points_per_step() const;    // sum M * N * T
+  double modal_points_per_step() const;   // sum over lines N * n_ell
+  int tile_count() const { return static_cast<int>(tiles_.size()); }
+  int max_lds_bytes() const { return max_lds_; }
+  hipStream_t stream() const { return stream_; }
+
+  void initialize_gaussian();   // every instance from its configuration
+  double import_state(int i, const cplx* host, double relative_tolerance);
+  void export_state(int i, cplx* host);
+  void evaluate_rhs(int i, const cplx* d_stage, cplx* d_out, double t);
+  void step(double t, double dt);  // every instance, explicit dt
+  void advance(int steps);         // every instance, its own dt
+  std::vector<WaveformRecordC> sample_waveform(int i);
+  void sync();
+  void set_graph(bool enable);
+  void set_kernel_timing(bool enable);
+  void kernel_timing(double* total_ms, long long* launches);
+  void set_lds_budget(int bytes);   // re-tiles
+  void set_tile_rows(int rows);     // 0 = automatic
+  SourceActivation activation(int i) const { return inst_.at(i).activation; }
+  void set_activation(int i, const SourceActivation& a);
+
+ private:
+  void build_lines();
+  void build_tiles();
+  void upload_tables();
+  void launch_step(double dt_override);
+  void launch_stage(int stage, const cplx* yin, cplx* yout, double dt_override, int group_first,
+                    int tile_first, int tile_count);
+  void record_time_step(double dt_override);
+
+  std::vector<Instance> inst_;
+  std::vector<LineDesc> lines_;
+  std::vector<int> line_instance_, line_mode_;
+  std::vector<double> mats_, nodes_;
+  std::vector<TileDesc> tiles_;
+  struct Group {
+    int scheme, first, count, lds;
+  };
+  std::vector<Group> groups_;
+  long long modal_count_ = 0;
+  int max_lds_ = 0;
+  int lds_budget_ = 64 * 1024;
+  int tile_rows_ = 0;
+
+  hipStream_t stream_ = nullptr;
+  LineDesc* d_lines_ = nullptr;
+  TileDesc* d_tiles_ = nullptr;
+  double* d_mats_ = nullptr;
+  double* d_nodes_ = nullptr;
+  SchemeTable* d_tables_ = nullptr;
+  int* d_line_mode_ = nullptr;
+  cdouble *d_y_ = nullptr, *d_acc_ = nullptr, *d_ya_ = nullptr, *d_yb_ = nullptr,
+          *d_k_ = nullptr;
+
+  bool use_graph_ = false;
+  hipGraphExec_t graph_exec_ = nullptr;
+  bool timing_ = false;
+  std::vector<hipEvent_t> events_;
+  double timing_ms_ = 0.0;
+  long long timing_launches_ = 0;
+};
+
+}  // namespace teuk
