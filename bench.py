@@ -1,0 +1,144 @@
+#!/usr/bin/env python3
+"""Throughput of the fused Teukolsky RK4 stage kernels on one MI355X.
+
+Default workload: the batched sweep of 512 independent linear Kerr
+evolutions (spin U[0,0.99], m U{-4..4}, ellmax U{8,12,16,24}, nr
+U{257,513,1025,2049}, D10-5, CFL 0.5), all advancing together with one
+launch per RK stage.  A step is one classical RK4 step of every instance;
+the metric counts grid points as the reference does
+(include/teuk/solver_driver.hpp:323-341): steps * modes * nr * ntheta.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+METRIC = ("grid-point-updates/s, mixed vs FP64 on GPU, at ≤1e-9 waveform err; "
+          "% HBM peak")
+HBM_PEAK_GBS = 8000.0
+# Algorithmic HBM bytes per retained modal coefficient per RK4 step: three
+# complex128 fields, 16 state vectors (stage reads/writes of y, stage value
+# and accumulator); one step is four launches.
+BYTES_PER_MODAL_STEP = 3 * 16 * 16
+
+
+def workload(name: str):
+    from teuk_kokkos_amd import workloads as W
+    if name == "sweep512":
+        return W.sweep(512), "kerr_sweep_512: a~U[0,0.99], m~U{-4..4}, ellmax~U{8,12,16,24}, " \
+                             "nr~U{257,513,1025,2049}, d10-5, cfl 0.5"
+    if name == "case2":
+        return [W.case2()], "kerr_a0.7_ringdown: nr=1025 d10-5 ellmax=16 ntheta=25 cfl 0.5"
+    if name == "case1":
+        return [W.case1()], "schwarzschild: nr=257 d8-4 ellmax=10 ntheta=16 cfl 0.02"
+    if name == "case4":
+        return [W.case4()], "near_extremal_a0.999: nr=4097 d10-5 ellmax=32 ntheta=49 cfl 0.5"
+    raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_baseline(cases, target_seconds: float):
+    """Oracle (nodal FP64 C++/OpenMP restatement) on a bounded sample."""
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests"))
+    import helpers as H
+    threads = min(16, os.cpu_count() or 1)
+    sample = cases[:8]
+    pipes = [H.oracle_initialized(c, threads=threads) for c in sample]
+    dts = [c["final_time"] / c["steps"] for c in sample]
+    pts = sum(len(set(c["first_order_modes"]) | set(c["second_order_modes"])) * c["nr"] *
+              c["ntheta"] for c in sample)
+    t0 = time.perf_counter()
+    for p, dt in zip(pipes, dts):
+        p.advance(1, dt)
+    one = time.perf_counter() - t0
+    steps = max(1, int(target_seconds / max(one, 1e-6)))
+    t0 = time.perf_counter()
+    for p, dt in zip(pipes, dts):
+        p.advance(steps, dt)
+    sec = time.perf_counter() - t0
+    return {"value": pts * steps / sec, "unit": "grid-point-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"first {len(sample)} instances of the workload, {steps} RK4 steps each, "
+                      f"{sec:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="sweep512")
+    ap.add_argument("--graph", action="store_true", help="replay each step as a hipGraph")
+    ap.add_argument("--lds-budget", type=int, default=0)
+    ap.add_argument("--tile-rows", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: this build has no multi-GPU path (got --gpus "
+              f"{args.gpus})", file=sys.stderr)
+        return 2
+
+    import teuk_kokkos_amd as T
+    from teuk_kokkos_amd import workloads as W
+
+    cases, desc = workload(args.workload)
+    print(f"[bench] building plan: {len(cases)} instances", file=sys.stderr, flush=True)
+    plan = T.Plan(W.texts(cases))
+    if args.lds_budget:
+        plan.set_lds_budget(args.lds_budget)
+    if args.tile_rows:
+        plan.set_tile_rows(args.tile_rows)
+    t0 = time.perf_counter()
+    plan.initialize()
+    print(f"[bench] initial data {time.perf_counter() - t0:.1f} s; tiles {plan.tiles()}",
+          file=sys.stderr, flush=True)
+    plan.set_graph(args.graph)
+    plan.advance(args.warmup)
+    plan.sync()
+
+    plan.set_kernel_timing(not args.graph)
+    plan.sync()
+    t0 = time.perf_counter()
+    plan.advance(args.steps)
+    plan.sync()
+    seconds = time.perf_counter() - t0
+    kernel_ms, launches = plan.kernel_timing()
+    plan.set_kernel_timing(False)
+
+    grid = plan.grid_points
+    modal = plan.modal_points
+    value = grid * args.steps / seconds
+    roofline = None
+    if launches:
+        avg_launch_s = kernel_ms / launches * 1e-3
+        achieved = BYTES_PER_MODAL_STEP / 4.0 * modal / avg_launch_s / 1e9
+        roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                    "kernel": "teuk_stage_kernel (4 RK stages)",
+                    "avg_launch_ms": kernel_ms / launches}
+    out = {
+        "metric": METRIC, "value": value, "unit": "grid-point-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": seconds / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": desc, "instances": len(cases), "grid_points_per_step": grid,
+                   "modal_points_per_step": modal, "precision": "fp64",
+                   "graph": bool(args.graph)},
+        "modal_updates_per_s": modal * args.steps / seconds,
+        "roofline": roofline,
+        "cpu_baseline": None,
+    }
+    plan.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(cases, args.cpu_seconds)
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

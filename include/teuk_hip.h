@@ -1,0 +1,186 @@
+/*
+ * teuk_hip.h -- C ABI of the MI355X-native Teukolsky stage operator and
+ * RK4 stepper (libteuk_hip.so).
+ *
+ * This is the drop-in boundary for the reference's spatial-operator / RK4
+ * path.  The reference (HengruiZhu99/teuk-kokkos) exposes that path as the
+ * C++ class `teuk::SpatialPipeline` (include/teuk/spatial_pipeline.hpp:
+ * 144-620) driven by `teuk::run_solver` (include/teuk/solver_driver.hpp:
+ * 86-353) from a schema-v1 configuration (include/teuk/config.hpp:
+ * 194-713).  Each entry point below names the reference interface it
+ * replaces.
+ *
+ * Conventions
+ *  - Every call returns TEUK_OK (0) or an error code; TEUK_EINVAL mirrors the
+ *    reference's std::invalid_argument (contract / schema violations) and
+ *    TEUK_ERUNTIME its std::runtime_error (numerical, IO or device failures).
+ *    The message of the last failure is teuk_last_error(plan), or
+ *    teuk_config_last_error() for calls without a plan.  Failures happen
+ *    before any state mutation, as in the reference.
+ *  - State buffers use the reference interchange layout: interleaved
+ *    complex128 (re, im) in (mode, field = 13, radial, theta) order with
+ *    theta fastest (include/teuk/pipeline_storage.hpp:18-33,
+ *    include/teuk/io.hpp:96-99).
+ *  - A plan owns all device memory and one HIP stream; it allocates only in
+ *    teuk_plan_create* and teuk_plan_set_* tuning calls.  Calls that take
+ *    device pointers enqueue on the plan's stream and return without
+ *    synchronising; teuk_sync waits.  A plan is not thread-safe.
+ *  - A batch plan holds several independent instances (each its own
+ *    configuration) that advance together, one launch per RK stage.
+ */
+#ifndef TEUK_HIP_H
+#define TEUK_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  TEUK_OK = 0,
+  TEUK_EINVAL = 1,   /* std::invalid_argument in the reference */
+  TEUK_ERUNTIME = 2  /* std::runtime_error in the reference */
+};
+
+typedef struct teuk_plan teuk_plan;
+
+/* Endpoint waveform record (include/teuk/waveform_output.hpp:23-33). */
+typedef struct teuk_waveform_record {
+  uint64_t step;
+  double time;
+  int boundary; /* 0 = scri (R = 0), 1 = horizon (R = R_H) */
+  int order;    /* 1 = FirstPsi, 2 = SecondPsi */
+  int m;
+  int ell;
+  double rescaled_re, rescaled_im; /* modal F coefficient */
+  double endpoint_re, endpoint_im; /* L^2 F at scri, R_H F at the horizon */
+} teuk_waveform_record;
+
+/* Latched second-order source activation state
+ * (include/teuk/source_activation.hpp, spatial_pipeline.hpp:386-424). */
+typedef struct teuk_source_activation {
+  int active;
+  int consecutive_passes;
+  double activation_time;
+  double last_eligibility_time;
+} teuk_source_activation;
+
+/* ---------------------------------------------------------------------
+ * Configuration (no GPU needed)
+ * ------------------------------------------------------------------- */
+
+/* parse_run_configuration_text + resolved_configuration_text
+ * (config.hpp:686-831).  Writes the resolved text (NUL-terminated) into
+ * out[out_size]; *needed receives the required size including the NUL. */
+int teuk_config_resolve(const char* cfg_text, const char* const* overrides, int n_overrides,
+                        char* out, size_t out_size, size_t* needed);
+const char* teuk_config_last_error(void);
+
+/* ---------------------------------------------------------------------
+ * Plans: SpatialPipeline construction (spatial_pipeline.hpp:146-363) plus
+ * the run_solver set-up checks (solver_driver.hpp:86-136: plus2 gate, CFL).
+ * ------------------------------------------------------------------- */
+int teuk_plan_create(const char* cfg_text, const char* const* overrides, int n_overrides,
+                     teuk_plan** out);
+/* n_instances configuration texts; the overrides apply to every instance. */
+int teuk_plan_create_batch(const char* const* cfg_texts, int n_instances,
+                           const char* const* overrides, int n_overrides, teuk_plan** out);
+void teuk_plan_destroy(teuk_plan* plan);
+const char* teuk_last_error(const teuk_plan* plan);
+
+int teuk_plan_instance_count(const teuk_plan* plan);
+/* SpatialPipelineStorage extents (pipeline_storage.hpp:104-109): stored
+ * modes, fields (13), radial points, theta nodes. */
+int teuk_plan_shape(const teuk_plan* plan, int instance, int* modes, int* fields, int* nr,
+                    int* ntheta);
+/* Configured time step final_time / steps (solver_driver.hpp:117-118). */
+double teuk_plan_time_step(const teuk_plan* plan, int instance);
+/* Simulation time and completed steps of an instance. */
+int teuk_plan_progress(const teuk_plan* plan, int instance, double* time, uint64_t* steps);
+/* sum over instances of modes * nr * ntheta: the grid points one step
+ * updates (solver_driver.hpp:323-341); and the number of retained modal
+ * coefficients (first-order modes * nr * n_ell) the kernels evolve. */
+double teuk_plan_grid_points(const teuk_plan* plan);
+double teuk_plan_modal_points(const teuk_plan* plan);
+
+/* ---------------------------------------------------------------------
+ * State
+ * ------------------------------------------------------------------- */
+/* initialize_pipeline_state with Gaussian data for every instance
+ * (initial_data_factory.hpp:82-110, pipeline_initial_data.hpp:211-340). */
+int teuk_initialize(teuk_plan* plan);
+/* import_pipeline_state (pipeline_bands.hpp:211-232): host buffer of
+ * 2 * modes * 13 * nr * ntheta doubles, rejected with TEUK_ERUNTIME when
+ * its off-band relative content exceeds relative_tolerance (reference
+ * default 5e-11).  *offband receives the measured value (may be NULL). */
+int teuk_state_import(teuk_plan* plan, int instance, const double* host,
+                      double relative_tolerance, double* offband);
+/* Copy of storage().state() in the interchange layout. */
+int teuk_state_export(teuk_plan* plan, int instance, double* host);
+
+/* ---------------------------------------------------------------------
+ * Operator and stepper
+ * ------------------------------------------------------------------- */
+/* SpatialPipeline::evaluate_rhs_at_time (spatial_pipeline.hpp:458-515,
+ * Disabled source policy): d_stage and d_out are device buffers in the
+ * interchange layout; fields 3..12 and non-first-order modes of the output
+ * are written as zero, fields 0..2 are band-projected. */
+int teuk_evaluate_rhs(teuk_plan* plan, int instance, const void* d_stage, void* d_out,
+                      double time);
+/* SpatialPipeline::step (spatial_pipeline.hpp:590-620): one classical RK4
+ * step of every instance with the given time and step. */
+int teuk_step(teuk_plan* plan, double time, double dt);
+/* The driver's step loop (solver_driver.hpp:289-317): n_steps steps of every
+ * instance with its configured step, captured as a hipGraph when enabled. */
+int teuk_advance(teuk_plan* plan, int n_steps);
+int teuk_sync(teuk_plan* plan);
+
+/* EndpointWaveformSampler::sample (waveform_output.hpp:79-160): records in
+ * the reference order (first order scri, horizon; second order scri,
+ * horizon; modes ascending, ell ascending). */
+int teuk_sample_waveform(teuk_plan* plan, int instance, teuk_waveform_record* out,
+                         int capacity, int* count);
+
+int teuk_get_source_activation(const teuk_plan* plan, int instance,
+                               teuk_source_activation* state);
+/* restore_source_activation (spatial_pipeline.hpp:395-400). */
+int teuk_set_source_activation(teuk_plan* plan, int instance,
+                               const teuk_source_activation* state);
+
+/* ---------------------------------------------------------------------
+ * Execution tuning and measurement (no reference counterpart: they replace
+ * the reference's SpatialPipelineTiming, spatial_pipeline.hpp:27-38)
+ * ------------------------------------------------------------------- */
+int teuk_plan_set_graph(teuk_plan* plan, int enable);
+/* LDS budget per workgroup (bytes) and fixed tile rows (0 = automatic);
+ * both re-tile the plan. */
+int teuk_plan_set_lds_budget(teuk_plan* plan, int bytes);
+int teuk_plan_set_tile_rows(teuk_plan* plan, int rows);
+int teuk_plan_tiles(const teuk_plan* plan, int* tiles, int* max_lds_bytes);
+/* HIP-event timing of every stage-kernel launch on the plan's stream
+ * (eager mode only); reset by enabling. */
+int teuk_plan_set_kernel_timing(teuk_plan* plan, int enable);
+int teuk_plan_kernel_timing(const teuk_plan* plan, double* total_ms, long long* launches);
+
+/* ---------------------------------------------------------------------
+ * Host-side tables (no GPU needed; used by the CPU tests)
+ * ------------------------------------------------------------------- */
+/* scheme: 0 d4-2, 1 d8-4, 2 d10-5.  h*D(row, col) and H(i)/h of an n-point
+ * grid (sbp.hpp, sbp84.hpp, sbp105.hpp). */
+double teuk_sbp_derivative_entry(int scheme, int n, int row, int col);
+double teuk_sbp_norm_weight(int scheme, int n, int index);
+/* S (ntheta x n_ell) and A (n_ell x ntheta) of one fixed-(s, m) band
+ * (angular.hpp:329-355); returns n_ell or -1. */
+int teuk_band_matrices(int spin, int m, int ell_max, int ntheta, double* S, double* A);
+/* ModeRegistry (modes.hpp:32-213) of a configuration: sorted stored modes,
+ * sharp indices, and ordered pairs (m1, m2, target) target-major. */
+int teuk_config_modes(const char* cfg_text, int* modes, int* sharp, int capacity, int* count,
+                      int* pairs, int pair_capacity, int* pair_count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TEUK_HIP_H */

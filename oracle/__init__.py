@@ -41,7 +41,7 @@ class OracleParams(ctypes.Structure):
 def build() -> None:
     """Compile the oracle with its Makefile (g++, OpenMP)."""
     import subprocess
-    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    subprocess.run(["make", "-s", "-B", "-C", _HERE], check=True)
 
 
 def lib():

@@ -1,0 +1,187 @@
+"""HIP path vs CPU oracle on identical inputs (FP64 bar: <= 1e-12 relative
+on the state, north_star).  All calls go through the C ABI."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import helpers as H
+import teuk_kokkos_amd as T
+from teuk_kokkos_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1.0e-12  # relative, FP64 state parity
+
+CASES = {
+    "schw_d42": H.small_case(),
+    "schw_d42_diss": H.small_case(dissipation=0.005),
+    "kerr_d42": H.small_case(spin=0.7),
+    "kerr_d84_diss": H.small_case(spin=0.6, radial_discretization="d8-4", dissipation=0.004),
+    "kerr_d105": H.small_case(spin=0.7, nr=49, radial_discretization="d10-5", ellmax_first=6,
+                              ellmax_second=6, seeds=[(2, 2, 1e-3), (3, 2, 2e-4 - 5e-4j)]),
+    "kerr_d105_diss_constrained": H.small_case(
+        spin=0.9, nr=53, radial_discretization="d10-5", dissipation=0.003,
+        reduction_mode="stage_constrained", seeds=[(2, 2, 1e-3 + 2e-4j)]),
+    "kerr_m0_m3_sharp": H.small_case(
+        spin=0.5, nr=37, radial_discretization="d4-2", ellmax_first=5, ellmax_second=5,
+        first_order_modes=[-3, 0, 3], second_order_modes=[-3, 0, 3],
+        initial_data__add_sharp_partner=True, seeds=[(3, 3, 1e-3 - 1e-3j), (2, 0, 5e-4)]),
+    "targets_only_modes": H.small_case(
+        spin=0.3, ellmax_first=4, ellmax_second=6, first_order_modes=[-2, 2],
+        second_order_modes=[-4, 0, 4]),
+}
+
+
+def _plan(params):
+    print(f"[gpu] plan {params.get('radial_discretization')} nr={params['nr']} "
+          f"a={params.get('spin')}", flush=True)
+    return T.Plan(W.config_text(**dict(params)))
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_initial_state_matches_oracle(name):
+    params = CASES[name]
+    with _plan(params) as plan:
+        plan.initialize()
+        gpu = plan.export_state()
+    ref = H.oracle_initialized(params).get_state()
+    assert gpu.shape == ref.shape
+    assert H.rel_err(gpu, ref) <= TOL
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_steps_match_oracle(name):
+    params = CASES[name]
+    dt = 0.2 * W.cfl_time_step(params["mass"], params["spin"], params["compactification_length"],
+                               params["nr"], params["ntheta"], 1.0)
+    ref = H.oracle_initialized(params)
+    with _plan(params) as plan:
+        plan.initialize()
+        t = 0.0
+        for _ in range(10):
+            plan.step(t, dt)
+            t += dt
+        gpu = plan.export_state()
+    ref.advance(10, dt)
+    err = H.rel_err(gpu, ref.get_state())
+    print(f"[gpu] {name} 10 steps rel err {err:.3e}")
+    assert err <= TOL
+
+
+@pytest.mark.parametrize("name", ["kerr_d105", "kerr_d105_diss_constrained", "kerr_m0_m3_sharp"])
+def test_evaluate_rhs_matches_oracle(name):
+    import torch
+    params = CASES[name]
+    ref = H.oracle_initialized(params)
+    state = ref.get_state()
+    expected = ref.evaluate_rhs(state)
+    with _plan(params) as plan:
+        plan.initialize()
+        d_in = torch.from_numpy(state.copy()).to("cuda")
+        d_out = torch.full_like(d_in, complex(7.0, 7.0))
+        torch.cuda.synchronize()
+        plan.evaluate_rhs_device(d_in.data_ptr(), d_out.data_ptr())
+        plan.sync()
+        got = d_out.cpu().numpy()
+    assert np.all(got[:, 3:] == 0)
+    assert H.rel_err(got, expected) <= TOL
+
+
+def test_batch_equals_independent_runs():
+    cases = [CASES["kerr_d105"], CASES["kerr_m0_m3_sharp"], CASES["schw_d42_diss"],
+             CASES["kerr_d84_diss"]]
+    with T.Plan([W.config_text(**dict(c)) for c in cases]) as plan:
+        plan.initialize()
+        plan.advance(7)
+        got = [plan.export_state(i) for i in range(len(cases))]
+        steps = [plan.progress(i)[1] for i in range(len(cases))]
+    assert steps == [7] * len(cases)
+    for c, g in zip(cases, got):
+        ref = H.oracle_initialized(c)
+        ref.advance(7, c["final_time"] / c["steps"])
+        assert H.rel_err(g, ref.get_state()) <= TOL
+
+
+def test_graph_replay_equals_eager():
+    params = CASES["kerr_d105"]
+    out = []
+    for graph in (False, True):
+        with _plan(params) as plan:
+            plan.initialize()
+            plan.set_graph(graph)
+            plan.advance(3)
+            plan.sync()
+            plan.advance(4)
+            out.append(plan.export_state())
+    assert np.array_equal(out[0], out[1])
+
+
+def test_tiling_does_not_change_results():
+    params = CASES["kerr_d105_diss_constrained"]
+    out = []
+    for rows in (0, 1, 7, 16, 53):
+        with _plan(params) as plan:
+            plan.set_tile_rows(rows)
+            plan.initialize()
+            plan.advance(3)
+            out.append(plan.export_state())
+    for o in out[1:]:
+        assert H.rel_err(o, out[0]) <= 1e-14
+
+
+def test_import_export_roundtrip_and_band_rejection():
+    params = CASES["kerr_d105"]
+    ref = H.oracle_initialized(params)
+    ref.advance(3, 1e-3)
+    state = ref.get_state()
+    with _plan(params) as plan:
+        off = plan.import_state(state)
+        assert off < 5e-11
+        back = plan.export_state()
+        assert H.rel_err(back, state) <= 1e-14
+        bad = state.copy()
+        bad[1, 2, 10, :] += 1e-3 * np.cos(np.arange(bad.shape[-1]) * 2.1)
+        with pytest.raises(T.TeukError):
+            plan.import_state(bad)
+        # failure leaves the state untouched
+        assert H.rel_err(plan.export_state(), state) <= 1e-14
+
+
+def test_waveform_records_follow_reference_order():
+    params = CASES["targets_only_modes"]
+    ref = H.oracle_initialized(params)
+    with _plan(params) as plan:
+        plan.initialize()
+        recs = plan.sample_waveform()
+    keys = [(r.order, r.boundary, r.m, r.ell) for r in recs]
+    want = []
+    for order, modes, emax in ((1, [-2, 2], 4), (2, [-4, 0, 4], 6)):
+        for b in (0, 1):
+            for m in modes:
+                want += [(order, b, m, ell) for ell in range(max(2, abs(m)), emax + 1)]
+    assert keys == want
+    scri, hor = ref.endpoint_modal(ref.setup.modes.index(2))
+    first = [r for r in recs if r.order == 1 and r.m == 2]
+    got_scri = np.array([complex(r.rescaled_re, r.rescaled_im) for r in first if r.boundary == 0])
+    got_hor = np.array([complex(r.rescaled_re, r.rescaled_im) for r in first if r.boundary == 1])
+    assert np.abs(got_scri - scri).max() <= TOL * max(np.abs(scri).max(), 1e-300) + 1e-300
+    assert np.abs(got_hor - hor).max() <= TOL * max(np.abs(hor).max(), 1e-30) + 1e-300
+    L2 = params["compactification_length"] ** 2
+    for r in recs:
+        if r.boundary == 0:
+            assert r.endpoint_re == L2 * r.rescaled_re
+
+
+def test_invalid_inputs_fail_closed():
+    with pytest.raises(T.TeukInvalidArgument):
+        T.Plan(W.config_text(**dict(H.small_case(nr=7))))
+    p = H.small_case()
+    p["final_time"] = 10.0  # dt far above the CFL limit
+    with pytest.raises(T.TeukInvalidArgument):
+        T.Plan(W.config_text(**p))
+    with _plan(H.small_case()) as plan:
+        with pytest.raises(T.TeukInvalidArgument):
+            plan.step(0.0, -1.0)
+        with pytest.raises(T.TeukInvalidArgument):
+            plan.step(float("nan"), 1e-3)

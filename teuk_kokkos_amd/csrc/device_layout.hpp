@@ -59,4 +59,28 @@ __host__ __device__ inline int tile_lds_bytes(int rows, int mrows, int n_ell, in
   return static_cast<int>(b);
 }
 
+// Kernel arguments of one stage launch.
+struct StageArgs {
+  const TileDesc* tiles;
+  const LineDesc* lines;
+  const double* mats;
+  const double* nodes;
+  const SchemeTable* table;
+  const cdouble* yin;   // RHS input (stage value)
+  const cdouble* y;     // state at the start of the step
+  cdouble* acc;         // y + dt/6 k1 + dt/3 k2 + dt/3 k3
+  cdouble* yout;        // next stage value, or y after stage 4
+  cdouble* kout;        // RHS output (stage 0 only)
+  double dt_override;   // > 0: use instead of LineDesc::dt
+};
+
+// Nodal (mode, field=13, r, theta) <-> modal line conversion of one instance.
+struct NodalArgs {
+  const LineDesc* lines;  // first line of the instance
+  const int* line_mode;   // stored-mode index of each line
+  const double* mats;
+  int n_lines;
+  int nr, ntheta, modes;
+};
+
 }  // namespace teuk

@@ -1,0 +1,790 @@
+// Plan: device memory, line/tile tables, initial data, import/export,
+// RK4 sequencing, hipGraph capture and waveform sampling.
+#include "plan.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <thread>
+#include <tuple>
+
+namespace teuk {
+
+hipError_t launch_stage(int scheme, int stage, const StageArgs& a, int n_tiles, int lds,
+                        hipStream_t s);
+hipError_t set_stage_lds_limit(int bytes);
+hipError_t launch_nodal_to_modal(const NodalArgs& a, const cdouble* nodal, cdouble* modal,
+                                 hipStream_t s);
+hipError_t launch_modal_to_nodal(const NodalArgs& a, const cdouble* modal, cdouble* nodal,
+                                 hipStream_t s);
+hipError_t launch_fill_zero(cdouble* p, long long count, hipStream_t s);
+
+namespace {
+
+void check(hipError_t e, const char* what) {
+  if (e != hipSuccess)
+    throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class T>
+T* device_alloc(size_t count) {
+  if (count == 0) count = 1;
+  void* p = nullptr;
+  check(hipMalloc(&p, count * sizeof(T)), "hipMalloc");
+  return static_cast<T*>(p);
+}
+
+template <class F>
+void parallel_for(int n, F&& f) {
+  const int hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  const int nt = std::min(hw, std::max(1, n));
+  if (nt <= 1) {
+    for (int i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::vector<std::thread> pool;
+  std::vector<std::exception_ptr> errors(nt);
+  for (int t = 0; t < nt; ++t)
+    pool.emplace_back([&, t] {
+      try {
+        for (int i = t; i < n; i += nt) f(i);
+      } catch (...) {
+        errors[t] = std::current_exception();
+      }
+    });
+  for (auto& th : pool) th.join();
+  for (auto& e : errors)
+    if (e) std::rethrow_exception(e);
+}
+
+SchemeTable make_table(RadialScheme s) {
+  const SbpScheme& sc = sbp_scheme(s);
+  SchemeTable t{};
+  t.W = sc.closure_rows;
+  t.C = sc.closure_width;
+  t.B = sc.half_width;
+  t.order = sc.diss_order;
+  for (int r = 0; r < sc.closure_rows; ++r)
+    for (int c = 0; c < sc.closure_width; ++c)
+      t.left[r * kMaxClosureWidth + c] = sc.left[r * sc.closure_width + c];
+  for (size_t k = 0; k < sc.interior.size(); ++k) t.interior[k] = sc.interior[k];
+  for (size_t k = 0; k < sc.norm.size(); ++k) t.norm[k] = sc.norm[k];
+  for (int d = 0; d < sc.diss_order; ++d)
+    for (size_t c = 0; c < sc.diss_left[d].size(); ++c) t.diss_left[d][c] = sc.diss_left[d][c];
+  for (size_t k = 0; k < sc.diss_interior.size(); ++k) t.diss_interior[k] = sc.diss_interior[k];
+  return t;
+}
+
+// Expanded Gaussian seeds with sharp partners, sorted by (m, ell)
+// (initial_data_factory.hpp:45-74).
+std::vector<SeedMode> expanded_seeds(const RunParameters& p) {
+  std::vector<SeedMode> out;
+  for (const auto& s : p.seeds) {
+    out.push_back(s);
+    if (p.add_sharp_partner && s.m != 0) out.push_back({s.ell, -s.m, std::conj(s.amplitude)});
+  }
+  std::sort(out.begin(), out.end(), [](const SeedMode& a, const SeedMode& b) {
+    return a.m != b.m ? a.m < b.m : a.ell < b.ell;
+  });
+  for (size_t i = 1; i < out.size(); ++i)
+    if (out[i].m == out[i - 1].m && out[i].ell == out[i - 1].ell)
+      throw std::invalid_argument("expanded Gaussian modes contain duplicate (ell,m) seeds");
+  return out;
+}
+
+}  // namespace
+
+Plan::Instance::Instance(RunParameters p)
+    : params(std::move(p)), registry(make_run_registry(params)) {
+  M = registry.size();
+  N = params.nr;
+  T = params.ntheta;
+  horizon = params.horizon_radius();
+  h = (horizon - 0.0) / static_cast<double>(N - 1);
+  dt = params.time_step();
+  std::vector<double> x, w;
+  gauss_legendre(T, x, w);
+  sin_theta.resize(T);
+  cos_theta.resize(T);
+  for (int j = 0; j < T; ++j) {
+    const double th = std::acos(x[j]);
+    sin_theta[j] = std::sin(th);
+    cos_theta[j] = std::cos(th);
+  }
+  if (params.source_start_time == 0.0 && params.second_order_enabled &&
+      params.source_mode == SourceMode::Unrestricted) {
+    activation.active = 1;
+    activation.activation_time = 0.0;
+    activation.consecutive_passes = params.required_consecutive_passes;
+  }
+}
+
+Plan::Plan(const std::vector<RunParameters>& params) {
+  if (params.empty()) throw std::invalid_argument("a plan needs at least one instance");
+  for (const auto& p : params) {
+    validate_parameters(p);
+    if (p.plus2_enabled)
+      throw std::runtime_error("plus2 mode is not production integrated; use plus2.enabled=false "
+                               "and plus2.mode=disabled");
+    if (p.second_order_enabled)
+      throw std::invalid_argument(
+          "second_order.enabled=true: the quadratic-source path is not part of this build");
+    const RunParameters& q = p;
+    ModeRegistry reg = make_run_registry(q);
+    if (!reg.closed_under_sharp())
+      throw std::invalid_argument("spatial pipeline modes must be closed under m -> -m");
+    inst_.emplace_back(p);
+  }
+  // CFL admissibility of the configured step (solver_driver.hpp:36-57,
+  // 117-125).
+  for (const auto& in : inst_) {
+    double vmax = 0.0;
+    for (int r = 0; r < in.N; ++r)
+      for (int j = 0; j < in.T; ++j) {
+        const RadialCoefficients rc = radial_coefficients(
+            in.params.mass, in.params.spin, in.params.compactification_length, -2, 0, in.h * r);
+        const double ct = time_coefficient(rc, in.params.spin, in.sin_theta[j]);
+        const double root = std::sqrt(rc.adv * rc.adv + ct * rc.principal);
+        vmax = std::max(vmax, std::max(std::abs((rc.adv - root) / ct),
+                                       std::abs((rc.adv + root) / ct)));
+      }
+    if (in.dt > in.params.cfl * in.h / vmax)
+      throw std::invalid_argument("steps are too small for the requested radial CFL limit");
+  }
+  check(hipStreamCreate(&stream_), "hipStreamCreate");
+  build_lines();
+  build_tiles();
+  upload_tables();
+  d_y_ = device_alloc<cdouble>(modal_count_);
+  d_acc_ = device_alloc<cdouble>(modal_count_);
+  d_ya_ = device_alloc<cdouble>(modal_count_);
+  d_yb_ = device_alloc<cdouble>(modal_count_);
+  d_k_ = device_alloc<cdouble>(modal_count_);
+  for (cdouble* p : {d_y_, d_acc_, d_ya_, d_yb_, d_k_})
+    check(launch_fill_zero(p, modal_count_, stream_), "zero state");
+  check(hipStreamSynchronize(stream_), "plan setup");
+}
+
+Plan::~Plan() {
+  if (stream_) hipStreamSynchronize(stream_);
+  if (graph_exec_) hipGraphExecDestroy(graph_exec_);
+  for (hipEvent_t e : events_) hipEventDestroy(e);
+  for (void* p : {static_cast<void*>(d_lines_), static_cast<void*>(d_tiles_),
+                  static_cast<void*>(d_mats_), static_cast<void*>(d_nodes_),
+                  static_cast<void*>(d_tables_), static_cast<void*>(d_line_mode_),
+                  static_cast<void*>(d_y_), static_cast<void*>(d_acc_), static_cast<void*>(d_ya_),
+                  static_cast<void*>(d_yb_), static_cast<void*>(d_k_)})
+    if (p) hipFree(p);
+  if (stream_) hipStreamDestroy(stream_);
+}
+
+void Plan::build_lines() {
+  std::map<std::tuple<int, int, int>, int> mat_index;  // (m, E, ntheta) -> offset
+  long long base = 0;
+  for (int i = 0; i < instance_count(); ++i) {
+    Instance& in = inst_[i];
+    const RunParameters& p = in.params;
+    const int node_off = static_cast<int>(nodes_.size());
+    nodes_.insert(nodes_.end(), in.sin_theta.begin(), in.sin_theta.end());
+    nodes_.insert(nodes_.end(), in.cos_theta.begin(), in.cos_theta.end());
+    in.line_of_mode.assign(in.M, -1);
+    for (int mi = 0; mi < in.M; ++mi) {
+      const int m = in.registry.modes()[mi];
+      if (!in.registry.is_parent(m)) continue;
+      const auto key = std::make_tuple(m, p.ell_max_first, in.T);
+      int off;
+      auto it = mat_index.find(key);
+      if (it == mat_index.end()) {
+        const BandTransform bt = make_band_transform(-2, m, p.ell_max_first, in.T);
+        off = static_cast<int>(mats_.size());
+        mats_.insert(mats_.end(), bt.S.begin(), bt.S.end());
+        mats_.insert(mats_.end(), bt.A.begin(), bt.A.end());
+        mat_index[key] = off;
+      } else {
+        off = it->second;
+      }
+      LineDesc L{};
+      L.base = base;
+      L.nr = in.N;
+      L.n_ell = p.ell_max_first - minimum_ell(-2, m) + 1;
+      L.ntheta = in.T;
+      L.ell_min = minimum_ell(-2, m);
+      L.m = m;
+      L.constrained = p.reduction == Reduction::StageConstrained ? 1 : 0;
+      L.kerr = p.spin != 0.0 ? 1 : 0;
+      L.dissipate = p.dissipation > 0.0 ? 1 : 0;
+      L.mat_off = off;
+      L.node_off = node_off;
+      L.mass = p.mass;
+      L.spin = p.spin;
+      L.length = p.compactification_length;
+      L.h = in.h;
+      L.inv_h = 1.0 / in.h;
+      L.gamma = p.reduction_damping;
+      L.eps = p.dissipation;
+      L.dt = in.dt;
+      in.line_of_mode[mi] = static_cast<int>(lines_.size());
+      lines_.push_back(L);
+      line_instance_.push_back(i);
+      line_mode_.push_back(mi);
+      base += static_cast<long long>(in.N) * 3 * L.n_ell;
+    }
+  }
+  modal_count_ = base;
+}
+
+void Plan::build_tiles() {
+  tiles_.clear();
+  groups_.clear();
+  max_lds_ = 0;
+  long long total_rows = 0;
+  for (const auto& L : lines_) total_rows += L.nr;
+  // Enough workgroups to fill 256 CUs several times over.
+  int parallel_rows = static_cast<int>(std::max<long long>(16, total_rows / 1024));
+  for (int scheme = 0; scheme < 3; ++scheme) {
+    const RadialScheme rs = static_cast<RadialScheme>(scheme);
+    Group g{scheme, static_cast<int>(tiles_.size()), 0, 0};
+    for (int i = 0; i < instance_count(); ++i) {
+      Instance& in = inst_[i];
+      if (static_cast<int>(in.params.scheme) != scheme) continue;
+      in.scheme_group = static_cast<int>(groups_.size());
+      in.tile_first = static_cast<int>(tiles_.size());
+      for (int mi = 0; mi < in.M; ++mi) {
+        const int li = in.line_of_mode[mi];
+        if (li < 0) continue;
+        const LineDesc& L = lines_[li];
+        const int n = L.nr;
+        auto ranges = [&](int t0, int t1, TileDesc& t) {
+          t.t0 = t0;
+          t.t1 = t1;
+          t.mlo = derivative_columns(rs, n, t0).first;
+          t.mhi = derivative_columns(rs, n, t1 - 1).second;
+          int lo = t.mlo, hi = t.mhi;
+          if (L.dissipate) {
+            lo = std::min(lo, dissipation_columns(rs, n, t0).first);
+            hi = std::max(hi, dissipation_columns(rs, n, t1 - 1).second);
+          }
+          if (L.constrained) {
+            lo = std::min(lo, derivative_columns(rs, n, t.mlo).first);
+            hi = std::max(hi, derivative_columns(rs, n, t.mhi - 1).second);
+          }
+          t.lo = lo;
+          t.hi = hi;
+          t.lds_bytes = tile_lds_bytes(hi - lo, t.mhi - t.mlo, L.n_ell, L.ntheta, L.constrained,
+                                       L.kerr);
+        };
+        // Largest row count within the LDS budget, capped for parallelism.
+        int rows = tile_rows_ > 0 ? tile_rows_ : std::min(256, parallel_rows);
+        rows = std::max(1, std::min(rows, n));
+        while (rows > 1) {
+          TileDesc probe{};
+          const int a = n / 2 - rows / 2;
+          ranges(a, a + rows, probe);
+          if (probe.lds_bytes <= lds_budget_) break;
+          rows = rows > 16 ? rows - 8 : rows - 1;
+        }
+        const int nt = (n + rows - 1) / rows;
+        const int basec = n / nt, rem = n % nt;
+        int t0 = 0;
+        for (int k = 0; k < nt; ++k) {
+          const int len = basec + (k < rem ? 1 : 0);
+          TileDesc t{};
+          t.line = li;
+          ranges(t0, t0 + len, t);
+          max_lds_ = std::max(max_lds_, t.lds_bytes);
+          g.lds = std::max(g.lds, t.lds_bytes);
+          tiles_.push_back(t);
+          t0 += len;
+        }
+      }
+      in.tile_count = static_cast<int>(tiles_.size()) - in.tile_first;
+    }
+    g.count = static_cast<int>(tiles_.size()) - g.first;
+    groups_.push_back(g);
+  }
+}
+
+void Plan::upload_tables() {
+  if (d_lines_) hipFree(d_lines_);
+  if (d_tiles_) hipFree(d_tiles_);
+  d_lines_ = device_alloc<LineDesc>(lines_.size());
+  d_tiles_ = device_alloc<TileDesc>(tiles_.size());
+  check(hipMemcpy(d_lines_, lines_.data(), lines_.size() * sizeof(LineDesc), hipMemcpyHostToDevice),
+        "upload lines");
+  check(hipMemcpy(d_tiles_, tiles_.data(), tiles_.size() * sizeof(TileDesc), hipMemcpyHostToDevice),
+        "upload tiles");
+  if (!d_mats_) {
+    d_mats_ = device_alloc<double>(mats_.size());
+    check(hipMemcpy(d_mats_, mats_.data(), mats_.size() * sizeof(double), hipMemcpyHostToDevice),
+          "upload matrices");
+    d_nodes_ = device_alloc<double>(nodes_.size());
+    check(hipMemcpy(d_nodes_, nodes_.data(), nodes_.size() * sizeof(double),
+                    hipMemcpyHostToDevice),
+          "upload nodes");
+    SchemeTable tabs[3] = {make_table(RadialScheme::D42), make_table(RadialScheme::D84),
+                           make_table(RadialScheme::D105)};
+    d_tables_ = device_alloc<SchemeTable>(3);
+    check(hipMemcpy(d_tables_, tabs, sizeof(tabs), hipMemcpyHostToDevice), "upload tables");
+    d_line_mode_ = device_alloc<int>(line_mode_.size());
+    check(hipMemcpy(d_line_mode_, line_mode_.data(), line_mode_.size() * sizeof(int),
+                    hipMemcpyHostToDevice),
+          "upload line modes");
+  }
+  if (max_lds_ > 64 * 1024) check(set_stage_lds_limit(max_lds_), "raise LDS limit");
+}
+
+long long Plan::nodal_value_count(int i) const {
+  const Instance& in = inst_.at(i);
+  return static_cast<long long>(in.M) * 13 * in.N * in.T;
+}
+
+double Plan::grid_points_per_step() const {
+  double s = 0.0;
+  for (const auto& in : inst_) s += static_cast<double>(in.M) * in.N * in.T;
+  return s;
+}
+
+double Plan::modal_points_per_step() const {
+  double s = 0.0;
+  for (const auto& L : lines_) s += static_cast<double>(L.nr) * L.n_ell;
+  return s;
+}
+
+// Gaussian pulse in Psi, Q = D Psi with the selected scheme, Galerkin P,
+// then analysis into the retained band (pipeline_initial_data.hpp:211-340).
+void Plan::initialize_gaussian() {
+  std::vector<cdouble> host(static_cast<size_t>(modal_count_));
+  for (auto& in : inst_) {
+    if (in.params.initial_type != InitialDataKind::Gaussian)
+      throw std::invalid_argument("checkpoint initial data are not supported by this build");
+  }
+  std::vector<std::pair<int, int>> work;  // (instance, mode)
+  for (int i = 0; i < instance_count(); ++i)
+    for (int mi = 0; mi < inst_[i].M; ++mi)
+      if (inst_[i].line_of_mode[mi] >= 0) work.push_back({i, mi});
+  std::vector<double> residual(work.size(), 0.0);
+  parallel_for(static_cast<int>(work.size()), [&](int w) {
+    const Instance& in = inst_[work[w].first];
+    const int mi = work[w].second;
+    const RunParameters& p = in.params;
+    const int m = in.registry.modes()[mi];
+    const LineDesc& L = lines_[in.line_of_mode[mi]];
+    const int N = in.N, T = in.T, nl = L.n_ell;
+    const double center = p.center_fraction * in.horizon;
+    const double width = p.width_fraction * in.horizon;
+    if (!(width > 0.0) || center < 0.0 || center > in.horizon)
+      throw std::invalid_argument("Gaussian center must be on-grid and width must be positive");
+    const auto seeds = expanded_seeds(p);
+    for (const auto& s : seeds) {
+      if (!in.registry.is_parent(s.m))
+        throw std::invalid_argument("Gaussian seed m is not a first-order mode");
+      if (s.ell < std::max(2, std::abs(s.m)) || s.ell > p.ell_max_first)
+        throw std::invalid_argument("Gaussian seed ell is outside the band");
+    }
+    const BandTransform bt = make_band_transform(-2, m, p.ell_max_first, T);
+    std::vector<cplx> seed(nl, cplx(0.0, 0.0));
+    for (const auto& s : seeds)
+      if (s.m == m) seed[s.ell - bt.ell_min] += s.amplitude;
+    const std::vector<cplx> ang = bt.synthesize(seed);
+    std::vector<cplx> psi(static_cast<size_t>(N) * T), q(static_cast<size_t>(N) * T),
+        pn(static_cast<size_t>(N) * T);
+    for (int r = 0; r < N; ++r) {
+      const double z = (0.0 + in.h * r - center) / width;
+      double prof;
+      if (!p.compact_support) prof = std::exp(-z * z);
+      else if (std::abs(0.0 + in.h * r - center) >= width) prof = 0.0;
+      else prof = std::exp(1.0 - 1.0 / (1.0 - z * z));
+      for (int j = 0; j < T; ++j) psi[r * T + j] = prof * ang[j];
+    }
+    for (int j = 0; j < T; ++j)
+      sbp_derivative_line(p.scheme, N, 1.0 / in.h, &psi[j], T, &q[j], T);
+    double maxres = 0.0;
+    std::vector<cplx> mat(static_cast<size_t>(nl) * nl), rhs(nl), sol;
+    for (int r = 0; r < N; ++r) {
+      std::fill(mat.begin(), mat.end(), cplx(0.0, 0.0));
+      std::fill(rhs.begin(), rhs.end(), cplx(0.0, 0.0));
+      const RadialCoefficients rc =
+          radial_coefficients(p.mass, p.spin, p.compactification_length, -2, m, 0.0 + in.h * r);
+      for (int j = 0; j < T; ++j) {
+        const double ct = time_coefficient(rc, p.spin, in.sin_theta[j]);
+        const cdouble g = definition_coefficient(rc, p.spin, -2, in.cos_theta[j]);
+        const cplx inv_time = 1.0 / ct;
+        const cplx nodal =
+            inv_time * (-2.0 * rc.adv * q[r * T + j] + cplx(g.re, g.im) * psi[r * T + j]);
+        for (int a = 0; a < nl; ++a) {
+          const double aa = bt.A[static_cast<size_t>(a) * T + j];
+          rhs[a] += aa * nodal;
+          for (int b = 0; b < nl; ++b)
+            mat[a * nl + b] += aa * inv_time * bt.S[static_cast<size_t>(j) * nl + b];
+        }
+      }
+      if (!dense_complex_solve(nl, mat, rhs, sol))
+        throw std::runtime_error("singular Galerkin initialization matrix");
+      maxres = std::max(maxres, dense_complex_residual(nl, mat, sol, rhs));
+      const std::vector<cplx> pnod = bt.synthesize(sol);
+      for (int j = 0; j < T; ++j) pn[r * T + j] = pnod[j];
+    }
+    residual[w] = maxres;
+    // Projection onto the retained band: the modal coefficients are A x.
+    cdouble* out = host.data() + L.base;
+    for (int r = 0; r < N; ++r) {
+      const cplx* fields[3] = {&pn[r * T], &q[r * T], &psi[r * T]};
+      for (int f = 0; f < 3; ++f) {
+        const std::vector<cplx> c = bt.analyze(fields[f]);
+        for (int l = 0; l < nl; ++l)
+          out[static_cast<size_t>(r) * 3 * nl + f * nl + l] = {c[l].real(), c[l].imag()};
+      }
+    }
+  });
+  for (double r : residual)
+    if (!std::isfinite(r) || r > 2.0e-12)
+      throw std::runtime_error("Galerkin initial-time-derivative solve failed its residual check");
+  check(hipMemcpyAsync(d_y_, host.data(), host.size() * sizeof(cdouble), hipMemcpyHostToDevice,
+                       stream_),
+        "upload initial state");
+  check(hipStreamSynchronize(stream_), "initialize");
+  for (auto& in : inst_) {
+    in.passive.clear();
+    in.residual.clear();
+    in.time = 0.0;
+    in.steps_done = 0;
+    SourceActivation fresh;
+    in.activation = fresh;
+  }
+}
+
+// Strict band-validated import (pipeline_bands.hpp:108-232).
+double Plan::import_state(int i, const cplx* host, double tol) {
+  Instance& in = inst_.at(i);
+  const int M = in.M, N = in.N, T = in.T;
+  static const int spins[13] = {-2, -2, -2, -1, -2, 0, -2, -1, -1, 0, -2, -2, -2};
+  auto at = [&](int mode, int f, int r) { return host + ((static_cast<size_t>(mode) * 13 + f) * N + r) * T; };
+  double combined = 0.0;
+  for (int mi = 0; mi < M; ++mi) {
+    const int m = in.registry.modes()[mi];
+    for (int f = 0; f < 13; ++f) {
+      const bool active = f >= 10 ? in.registry.is_target(m) : in.registry.is_parent(m);
+      const int emax = f >= 10 ? in.params.ell_max_second : in.params.ell_max_first;
+      std::unique_ptr<BandTransform> bt;
+      if (active) bt = std::make_unique<BandTransform>(make_band_transform(spins[f], m, emax, T));
+      for (int r = 0; r < N; ++r) {
+        const cplx* v = at(mi, f, r);
+        std::vector<cplx> proj(T, cplx(0.0, 0.0));
+        if (active) proj = bt->synthesize(bt->analyze(v));
+        double line2 = 0.0, res2 = 0.0;
+        for (int j = 0; j < T; ++j) {
+          line2 += std::norm(v[j]);
+          res2 += std::norm(v[j] - proj[j]);
+        }
+        const double rel =
+            std::sqrt(res2) / std::max(std::sqrt(line2), std::numeric_limits<double>::min());
+        combined = std::max(combined, rel);
+      }
+    }
+  }
+  if (!std::isfinite(tol) || tol < 0.0)
+    throw std::invalid_argument("pipeline band tolerance must be nonnegative");
+  if (!std::isfinite(combined) || combined > tol)
+    throw std::runtime_error("imported pipeline state contains meaningful off-band angular content");
+  // Split: evolved fields (0..2 of first-order modes) go to the modal state
+  // plus their off-band residual; everything else is kept as passive data.
+  std::vector<cdouble> modal(static_cast<size_t>(modal_count_));
+  check(hipMemcpy(modal.data(), d_y_, modal.size() * sizeof(cdouble), hipMemcpyDeviceToHost),
+        "read state");
+  in.passive.assign(host, host + nodal_value_count(i));
+  in.residual.assign(static_cast<size_t>(nodal_value_count(i)), cplx(0.0, 0.0));
+  bool any_residual = false;
+  for (int mi = 0; mi < M; ++mi) {
+    const int li = in.line_of_mode[mi];
+    if (li < 0) continue;
+    const LineDesc& L = lines_[li];
+    const BandTransform bt = make_band_transform(-2, L.m, in.params.ell_max_first, T);
+    for (int f = 0; f < 3; ++f)
+      for (int r = 0; r < N; ++r) {
+        const cplx* v = at(mi, f, r);
+        const std::vector<cplx> c = bt.analyze(v);
+        const std::vector<cplx> back = bt.synthesize(c);
+        for (int l = 0; l < L.n_ell; ++l)
+          modal[L.base + static_cast<size_t>(r) * 3 * L.n_ell + f * L.n_ell + l] = {c[l].real(),
+                                                                                     c[l].imag()};
+        const size_t o = ((static_cast<size_t>(mi) * 13 + f) * N + r) * T;
+        for (int j = 0; j < T; ++j) {
+          in.residual[o + j] = v[j] - back[j];
+          if (in.residual[o + j] != cplx(0.0, 0.0)) any_residual = true;
+          in.passive[o + j] = 0.0;
+        }
+      }
+  }
+  if (!any_residual) in.residual.clear();
+  check(hipMemcpy(d_y_, modal.data(), modal.size() * sizeof(cdouble), hipMemcpyHostToDevice),
+        "write state");
+  return combined;
+}
+
+void Plan::export_state(int i, cplx* host) {
+  Instance& in = inst_.at(i);
+  const int M = in.M, N = in.N, T = in.T;
+  check(hipStreamSynchronize(stream_), "export sync");
+  const long long count = nodal_value_count(i);
+  if (in.passive.empty()) std::fill(host, host + count, cplx(0.0, 0.0));
+  else std::copy(in.passive.begin(), in.passive.end(), host);
+  for (int mi = 0; mi < M; ++mi) {
+    const int li = in.line_of_mode[mi];
+    if (li < 0) continue;
+    const LineDesc& L = lines_[li];
+    std::vector<cdouble> line(static_cast<size_t>(N) * 3 * L.n_ell);
+    check(hipMemcpy(line.data(), d_y_ + L.base, line.size() * sizeof(cdouble),
+                    hipMemcpyDeviceToHost),
+          "read line");
+    const BandTransform bt = make_band_transform(-2, L.m, in.params.ell_max_first, T);
+    for (int r = 0; r < N; ++r)
+      for (int f = 0; f < 3; ++f) {
+        std::vector<cplx> c(L.n_ell);
+        for (int l = 0; l < L.n_ell; ++l) {
+          const cdouble v = line[static_cast<size_t>(r) * 3 * L.n_ell + f * L.n_ell + l];
+          c[l] = cplx(v.re, v.im);
+        }
+        const std::vector<cplx> nod = bt.synthesize(c);
+        const size_t o = ((static_cast<size_t>(mi) * 13 + f) * N + r) * T;
+        for (int j = 0; j < T; ++j)
+          host[o + j] = in.residual.empty() ? nod[j] : nod[j] + in.residual[o + j];
+      }
+  }
+}
+
+void Plan::launch_step(double dt_override) {
+  for (const Group& g : groups_) {
+    if (g.count == 0) continue;
+    StageArgs a{};
+    a.tiles = d_tiles_ + g.first;
+    a.lines = d_lines_;
+    a.mats = d_mats_;
+    a.nodes = d_nodes_;
+    a.table = d_tables_ + g.scheme;
+    a.y = d_y_;
+    a.acc = d_acc_;
+    a.dt_override = dt_override;
+    const cdouble* in_of[5] = {nullptr, d_y_, d_ya_, d_yb_, d_ya_};
+    cdouble* out_of[5] = {nullptr, d_ya_, d_yb_, d_ya_, d_y_};
+    for (int s = 1; s <= 4; ++s) {
+      a.yin = in_of[s];
+      a.yout = out_of[s];
+      const bool timed = timing_ && !use_graph_;
+      if (timed) {
+        const size_t need = 2 * static_cast<size_t>(timing_launches_ + 1);
+        while (events_.size() < need) {
+          hipEvent_t e;
+          check(hipEventCreate(&e), "hipEventCreate");
+          events_.push_back(e);
+        }
+        check(hipEventRecord(events_[2 * timing_launches_], stream_), "event");
+      }
+      check(teuk::launch_stage(g.scheme, s, a, g.count, g.lds, stream_), "stage kernel");
+      if (timed) {
+        check(hipEventRecord(events_[2 * timing_launches_ + 1], stream_), "event");
+        ++timing_launches_;
+      }
+    }
+  }
+}
+
+void Plan::record_time_step(double dt_override) {
+  for (auto& in : inst_) {
+    in.time += dt_override > 0.0 ? dt_override : in.dt;
+    ++in.steps_done;
+  }
+}
+
+// SpatialPipeline::step (spatial_pipeline.hpp:590-620) with the Disabled
+// source policy: one classical RK4 step of every instance.
+void Plan::step(double t, double dt) {
+  if (!std::isfinite(t) || !std::isfinite(dt) || t < 0.0 || !(dt > 0.0))
+    throw std::invalid_argument("pipeline step requires finite positive time data");
+  if (!std::isfinite(t + dt)) throw std::invalid_argument("pipeline step end time is not finite");
+  launch_step(dt);
+  for (auto& in : inst_) {
+    in.time = t + dt;
+    ++in.steps_done;
+  }
+}
+
+void Plan::advance(int steps) {
+  if (steps < 0) throw std::invalid_argument("step count must be nonnegative");
+  if (use_graph_) {
+    if (!graph_exec_) {
+      hipGraph_t graph;
+      check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture");
+      launch_step(0.0);
+      check(hipStreamEndCapture(stream_, &graph), "end capture");
+      check(hipGraphInstantiate(&graph_exec_, graph, nullptr, nullptr, 0), "instantiate graph");
+      hipGraphDestroy(graph);
+    }
+    for (int s = 0; s < steps; ++s) {
+      check(hipGraphLaunch(graph_exec_, stream_), "graph launch");
+      record_time_step(0.0);
+    }
+    return;
+  }
+  for (int s = 0; s < steps; ++s) {
+    launch_step(0.0);
+    record_time_step(0.0);
+  }
+}
+
+void Plan::evaluate_rhs(int i, const cplx* d_stage, cplx* d_out, double t) {
+  (void)t;  // the linear first-order operator is autonomous
+  Instance& in = inst_.at(i);
+  if (in.tile_count == 0) {
+    check(launch_fill_zero(reinterpret_cast<cdouble*>(d_out), nodal_value_count(i), stream_), "zero");
+    return;
+  }
+  // Lines of one instance are contiguous in the line table.
+  int first_line = -1, n_lines = 0;
+  for (int mi = 0; mi < in.M; ++mi)
+    if (in.line_of_mode[mi] >= 0) {
+      if (first_line < 0) first_line = in.line_of_mode[mi];
+      ++n_lines;
+    }
+  NodalArgs na{d_lines_ + first_line, d_line_mode_ + first_line, d_mats_, n_lines, in.N, in.T,
+               in.M};
+  check(launch_nodal_to_modal(na, reinterpret_cast<const cdouble*>(d_stage), d_ya_, stream_),
+        "analysis");
+  const Group& g = groups_[in.scheme_group];
+  StageArgs a{};
+  a.tiles = d_tiles_ + in.tile_first;
+  a.lines = d_lines_;
+  a.mats = d_mats_;
+  a.nodes = d_nodes_;
+  a.table = d_tables_ + g.scheme;
+  a.yin = d_ya_;
+  a.kout = d_k_;
+  check(teuk::launch_stage(g.scheme, 0, a, in.tile_count, g.lds, stream_), "rhs kernel");
+  check(launch_fill_zero(reinterpret_cast<cdouble*>(d_out), nodal_value_count(i), stream_), "zero");
+  check(launch_modal_to_nodal(na, d_k_, reinterpret_cast<cdouble*>(d_out), stream_), "synthesis");
+}
+
+// EndpointWaveformSampler::sample (waveform_output.hpp:79-160).
+std::vector<WaveformRecordC> Plan::sample_waveform(int i) {
+  Instance& in = inst_.at(i);
+  check(hipStreamSynchronize(stream_), "waveform sync");
+  std::vector<WaveformRecordC> out;
+  const double L2 = in.params.compactification_length * in.params.compactification_length;
+  for (int order = 1; order <= 2; ++order) {
+    for (int boundary = 0; boundary < 2; ++boundary) {
+      const int r = boundary == 0 ? 0 : in.N - 1;
+      const double scale = boundary == 0 ? L2 : in.horizon;
+      for (int mi = 0; mi < in.M; ++mi) {
+        const int m = in.registry.modes()[mi];
+        std::vector<cplx> modal;
+        int ell_min;
+        if (order == 1) {
+          if (!in.registry.is_parent(m)) continue;
+          const LineDesc& L = lines_[in.line_of_mode[mi]];
+          std::vector<cdouble> row(L.n_ell);
+          check(hipMemcpy(row.data(),
+                          d_y_ + L.base + static_cast<long long>(r) * 3 * L.n_ell + 2 * L.n_ell,
+                          row.size() * sizeof(cdouble), hipMemcpyDeviceToHost),
+                "read endpoint");
+          ell_min = L.ell_min;
+          modal.resize(L.n_ell);
+          for (int l = 0; l < L.n_ell; ++l) modal[l] = cplx(row[l].re, row[l].im);
+          if (!in.residual.empty()) {
+            const BandTransform bt = make_band_transform(-2, m, in.params.ell_max_first, in.T);
+            const size_t o = ((static_cast<size_t>(mi) * 13 + 2) * in.N + r) * in.T;
+            const std::vector<cplx> c = bt.analyze(&in.residual[o]);
+            for (int l = 0; l < L.n_ell; ++l) modal[l] += c[l];
+          }
+        } else {
+          if (!in.registry.is_target(m)) continue;
+          const BandTransform bt = make_band_transform(-2, m, in.params.ell_max_second, in.T);
+          ell_min = bt.ell_min;
+          if (in.passive.empty()) {
+            modal.assign(bt.modes(), cplx(0.0, 0.0));
+          } else {
+            const size_t o = ((static_cast<size_t>(mi) * 13 + 12) * in.N + r) * in.T;
+            modal = bt.analyze(&in.passive[o]);
+          }
+        }
+        for (size_t k = 0; k < modal.size(); ++k) {
+          WaveformRecordC rec{};
+          rec.step = in.steps_done;
+          rec.time = in.time;
+          rec.boundary = boundary;
+          rec.order = order;
+          rec.m = m;
+          rec.ell = ell_min + static_cast<int>(k);
+          rec.rescaled_re = modal[k].real();
+          rec.rescaled_im = modal[k].imag();
+          rec.endpoint_re = scale * modal[k].real();
+          rec.endpoint_im = scale * modal[k].imag();
+          out.push_back(rec);
+        }
+      }
+    }
+  }
+  return out;
+}
+
+void Plan::sync() { check(hipStreamSynchronize(stream_), "sync"); }
+
+void Plan::set_graph(bool enable) {
+  use_graph_ = enable;
+  if (!enable && graph_exec_) {
+    hipGraphExecDestroy(graph_exec_);
+    graph_exec_ = nullptr;
+  }
+}
+
+void Plan::set_kernel_timing(bool enable) {
+  check(hipStreamSynchronize(stream_), "timing sync");
+  timing_ = enable;
+  timing_launches_ = 0;
+}
+
+void Plan::kernel_timing(double* total_ms, long long* launches) {
+  check(hipStreamSynchronize(stream_), "timing sync");
+  double ms = 0.0;
+  for (long long k = 0; k < timing_launches_; ++k) {
+    float e = 0.f;
+    check(hipEventElapsedTime(&e, events_[2 * k], events_[2 * k + 1]), "elapsed");
+    ms += e;
+  }
+  *total_ms = ms;
+  *launches = timing_launches_;
+}
+
+void Plan::set_lds_budget(int bytes) {
+  if (bytes < 16 * 1024 || bytes > 160 * 1024)
+    throw std::invalid_argument("LDS budget must be in [16 KiB, 160 KiB]");
+  lds_budget_ = bytes;
+  build_tiles();
+  upload_tables();
+  set_graph(use_graph_);
+  if (graph_exec_) {
+    hipGraphExecDestroy(graph_exec_);
+    graph_exec_ = nullptr;
+  }
+}
+
+void Plan::set_tile_rows(int rows) {
+  if (rows < 0) throw std::invalid_argument("tile rows must be nonnegative");
+  tile_rows_ = rows;
+  build_tiles();
+  upload_tables();
+  if (graph_exec_) {
+    hipGraphExecDestroy(graph_exec_);
+    graph_exec_ = nullptr;
+  }
+}
+
+void Plan::set_activation(int i, const SourceActivation& a) {
+  if (a.consecutive_passes < 0 || !std::isfinite(a.activation_time) ||
+      !std::isfinite(a.last_eligibility_time) || (a.active && a.activation_time < 0.0) ||
+      (!a.active && a.activation_time != -1.0) || a.last_eligibility_time < -1.0)
+    throw std::invalid_argument("invalid second-order source activation state");
+  inst_.at(i).activation = a;
+}
+
+}  // namespace teuk

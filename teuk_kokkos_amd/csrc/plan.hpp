@@ -16,8 +16,6 @@
 
 namespace teuk {
 
-struct StageArgs;
-
 struct WaveformRecordC {
   std::uint64_t step;
   double time;
@@ -93,8 +91,6 @@ class Plan {
   void build_tiles();
   void upload_tables();
   void launch_step(double dt_override);
-  void launch_stage(int stage, const cplx* yin, cplx* yout, double dt_override, int group_first,
-                    int tile_first, int tile_count);
   void record_time_step(double dt_override);
 
   std::vector<Instance> inst_;
@@ -125,7 +121,6 @@ class Plan {
   hipGraphExec_t graph_exec_ = nullptr;
   bool timing_ = false;
   std::vector<hipEvent_t> events_;
-  double timing_ms_ = 0.0;
   long long timing_launches_ = 0;
 };
 

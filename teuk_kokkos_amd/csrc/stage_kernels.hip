@@ -22,20 +22,6 @@
 
 namespace teuk {
 
-struct StageArgs {
-  const TileDesc* tiles;
-  const LineDesc* lines;
-  const double* mats;
-  const double* nodes;
-  const SchemeTable* table;
-  const cdouble* yin;   // RHS input (stage value)
-  const cdouble* y;     // state at the start of the step
-  cdouble* acc;         // y + dt/6 k1 + dt/3 k2 + dt/3 k3
-  cdouble* yout;        // next stage value, or y after stage 4
-  cdouble* kout;        // RHS output (stage 0 only)
-  double dt_override;   // > 0: use instead of LineDesc::dt
-};
-
 namespace {
 
 // h*D applied at global row `row` to values v(g) = base[(g - lo) * stride].
@@ -233,7 +219,7 @@ __global__ __launch_bounds__(kStageThreads) void teuk_stage_kernel(StageArgs a) 
   for (int i = tid; i < (tile.t1 - tile.t0) * nl; i += kStageThreads) {
     const int r = tile.t0 + i / nl, l = i % nl;
     const cdouble* u = U + (r - tile.lo) * us;
-    const cdouble P = u[l], Q = u[nl + l], S = u[2 * nl + l];
+    const cdouble Q = u[nl + l], S = u[2 * nl + l];
     const cdouble dpsi = apply_d<SCHEME>(T, U + 2 * nl + l, us, tile.lo, r, n, L.inv_h);
     cdouble qeff, dq;
     if (L.constrained) {
@@ -284,15 +270,7 @@ __global__ __launch_bounds__(kStageThreads) void teuk_stage_kernel(StageArgs a) 
 
 // ------------------------------------------------------------ band transforms
 // Nodal (mode, field, r, theta) <-> modal line layout, used by the
-// reference-layout evaluate_rhs entry point and by device-side import.
-struct NodalArgs {
-  const LineDesc* lines;
-  const int* line_mode;  // stored-mode index of each line
-  const double* mats;
-  int n_lines;
-  int nr, ntheta, modes;  // instance shape (13 fields)
-};
-
+// reference-layout evaluate_rhs entry point.
 __global__ __launch_bounds__(kStageThreads) void teuk_nodal_to_modal(NodalArgs a,
                                                                      const cdouble* nodal,
                                                                      cdouble* modal) {
