@@ -27,6 +27,15 @@ CASES = {
         spin=0.5, nr=37, radial_discretization="d4-2", ellmax_first=5, ellmax_second=5,
         first_order_modes=[-3, 0, 3], second_order_modes=[-3, 0, 3],
         initial_data__add_sharp_partner=True, seeds=[(3, 3, 1e-3 - 1e-3j), (2, 0, 5e-4)]),
+    # ell_max 24 / 32: the reference's A S differs from I by 6e-12 / 2e-9, so
+    # these exercise the Gram-corrected modal projection.
+    "kerr_ell24_gram": H.small_case(spin=0.95, nr=45, radial_discretization="d10-5",
+                                    ellmax_first=24, ellmax_second=24, first_order_modes=[0],
+                                    second_order_modes=[0], seeds=[(2, 0, 1e-3 + 1e-3j)]),
+    "kerr_ell32_gram_diss": H.small_case(
+        spin=0.999, compactification_length=1.0, nr=41, radial_discretization="d10-5",
+        dissipation=0.001, ellmax_first=32, ellmax_second=32,
+        seeds=[(2, 2, 1e-3), (5, 2, -3e-4j)]),
     "targets_only_modes": H.small_case(
         spin=0.3, ellmax_first=4, ellmax_second=6, first_order_modes=[-2, 2],
         second_order_modes=[-4, 0, 4]),
@@ -69,7 +78,8 @@ def test_steps_match_oracle(name):
     assert err <= TOL
 
 
-@pytest.mark.parametrize("name", ["kerr_d105", "kerr_d105_diss_constrained", "kerr_m0_m3_sharp"])
+@pytest.mark.parametrize("name", ["kerr_d105", "kerr_d105_diss_constrained", "kerr_m0_m3_sharp",
+                                  "kerr_ell32_gram_diss"])
 def test_evaluate_rhs_matches_oracle(name):
     import torch
     params = CASES[name]

@@ -21,7 +21,11 @@ struct LineDesc {
   long long base;   // cdouble offset of this line's block
   int nr, n_ell, ntheta, ell_min;
   int m, constrained, kerr, dissipate;
-  int mat_off;      // double offset of S (ntheta x n_ell) then A (n_ell x ntheta)
+  int gram;         // apply G = A S (its deviation from I exceeds roundoff)
+  int gram_pad;
+  // double offset of S (ntheta x n_ell), then A (n_ell x ntheta),
+  // G = A S (n_ell x n_ell) and B = G^-1 A (n_ell x ntheta)
+  int mat_off;
   int node_off;     // double offset of sin(theta_j)[ntheta] then cos(theta_j)[ntheta]
   double mass, spin, length, h, inv_h, gamma, eps, dt;
 };
@@ -51,11 +55,16 @@ __host__ __device__ inline int nodal_chunk_rows(int ntheta) {
   return r < 1 ? 1 : r;
 }
 
-__host__ __device__ inline int tile_lds_bytes(int rows, int mrows, int n_ell, int ntheta,
-                                              int constrained, int kerr) {
+__host__ __device__ inline int gram_doubles(int n_ell) {
+  return (n_ell * n_ell + 1) / 2 * 2;  // padded to 16 bytes
+}
+
+__host__ __device__ inline int tile_lds_bytes(int rows, int mrows, int orows, int n_ell,
+                                              int ntheta, int constrained, int kerr, int gram) {
   long long b = 16LL * rows * 3 * n_ell + 16LL * mrows * n_ell;
   if (constrained) b += 16LL * mrows * n_ell;
   if (kerr) b += 16LL * ntheta * n_ell + 16LL * nodal_chunk_rows(ntheta) * ntheta;
+  if (gram) b += 8LL * gram_doubles(n_ell) + 16LL * orows * 3 * n_ell;
   return static_cast<int>(b);
 }
 

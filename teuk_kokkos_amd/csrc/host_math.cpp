@@ -204,7 +204,61 @@ BandTransform make_band_transform(int spin, int m, int ell_max, int nodes) {
       t.A[static_cast<size_t>(k) * nodes + j] = 2.0 * kPi * w[j] * basis;
     }
   }
+  t.G.assign(static_cast<size_t>(nm) * nm, 0.0);
+  double dev = 0.0;
+  for (int a = 0; a < nm; ++a)
+    for (int b = 0; b < nm; ++b) {
+      long double g = 0.0L;
+      for (int j = 0; j < nodes; ++j)
+        g += static_cast<long double>(t.A[static_cast<size_t>(a) * nodes + j]) *
+             t.S[static_cast<size_t>(j) * nm + b];
+      t.G[a * nm + b] = static_cast<double>(g);
+      dev = std::max(dev, std::abs(t.G[a * nm + b] - (a == b ? 1.0 : 0.0)));
+    }
+  t.gram = dev > 1.0e-13;
+  // B = G^-1 A by Gauss-Jordan elimination with partial pivoting.
+  std::vector<double> g = t.G;
+  t.B = t.A;
+  for (int c = 0; c < nm; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < nm; ++r)
+      if (std::abs(g[r * nm + c]) > std::abs(g[piv * nm + c])) piv = r;
+    if (piv != c) {
+      for (int k = 0; k < nm; ++k) std::swap(g[c * nm + k], g[piv * nm + k]);
+      for (int j = 0; j < nodes; ++j) std::swap(t.B[c * nodes + j], t.B[piv * nodes + j]);
+    }
+    const double d = g[c * nm + c];
+    for (int k = 0; k < nm; ++k) g[c * nm + k] /= d;
+    for (int j = 0; j < nodes; ++j) t.B[c * nodes + j] /= d;
+    for (int r = 0; r < nm; ++r) {
+      if (r == c) continue;
+      const double f = g[r * nm + c];
+      if (f == 0.0) continue;
+      for (int k = 0; k < nm; ++k) g[r * nm + k] -= f * g[c * nm + k];
+      for (int j = 0; j < nodes; ++j) t.B[r * nodes + j] -= f * t.B[c * nodes + j];
+    }
+  }
   return t;
+}
+
+bool band_gram_needed(int spin, int m, int ell_max, int nodes) {
+  return make_band_transform(spin, m, ell_max, nodes).gram;
+}
+
+std::vector<cplx> BandTransform::coefficients_of(const cplx* nodal) const {
+  const std::vector<double>& M = gram ? B : A;
+  std::vector<cplx> out(modes(), cplx(0.0, 0.0));
+  for (int k = 0; k < modes(); ++k)
+    for (int j = 0; j < nodes; ++j) out[k] += M[static_cast<size_t>(k) * nodes + j] * nodal[j];
+  return out;
+}
+
+std::vector<cplx> BandTransform::gram_apply(const std::vector<cplx>& c) const {
+  if (!gram) return c;
+  std::vector<cplx> out(modes(), cplx(0.0, 0.0));
+  for (int a = 0; a < modes(); ++a)
+    for (int b = 0; b < modes(); ++b) out[a] += G[static_cast<size_t>(a) * modes() + b] * c[b];
+  return out;
 }
 
 std::vector<cplx> BandTransform::analyze(const cplx* nodal) const {

@@ -33,14 +33,23 @@ inline int minimum_ell(int spin, int m) {
 
 // Fixed-(s,m) band: synthesis S (ntheta x n_ell) and analysis
 // A (n_ell x ntheta, A = 2 pi w S^T) (angular.hpp:329-355).
+// G = A S equals the identity only to the accuracy of the long-double
+// Wigner-d sums: ~1e-14 up to ell ~ 16, 6e-12 at ell = 24 and 2e-9 at
+// ell = 32.  The nodal projection S A the reference applies every stage is
+// reproduced in modal coordinates with G (`gram` set when |G - I| > 1e-13);
+// B = G^-1 A maps a nodal vector S c back to exactly c.
 struct BandTransform {
   int spin = 0, m = 0, ell_min = 0, ell_max = 0, nodes = 0;
-  std::vector<double> S, A;
+  std::vector<double> S, A, G, B;
+  bool gram = false;
   int modes() const { return ell_max - ell_min + 1; }
-  std::vector<cplx> analyze(const cplx* nodal) const;
-  std::vector<cplx> synthesize(const std::vector<cplx>& modal) const;
+  std::vector<cplx> analyze(const cplx* nodal) const;         // A x
+  std::vector<cplx> synthesize(const std::vector<cplx>& modal) const;  // S c
+  std::vector<cplx> coefficients_of(const cplx* nodal) const;  // c with S c = x on range(S)
+  std::vector<cplx> gram_apply(const std::vector<cplx>& c) const;  // G c
 };
 BandTransform make_band_transform(int spin, int m, int ell_max, int nodes);
+bool band_gram_needed(int spin, int m, int ell_max, int nodes);
 
 // ------------------------------------------------------------ SBP
 // First-derivative closure description of one scheme.

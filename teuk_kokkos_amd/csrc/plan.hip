@@ -201,6 +201,10 @@ void Plan::build_lines() {
         off = static_cast<int>(mats_.size());
         mats_.insert(mats_.end(), bt.S.begin(), bt.S.end());
         mats_.insert(mats_.end(), bt.A.begin(), bt.A.end());
+        mats_.insert(mats_.end(), bt.G.begin(), bt.G.end());
+        mats_.resize(mats_.size() + (gram_doubles(bt.modes()) - bt.modes() * bt.modes()), 0.0);
+        const std::vector<double>& B = bt.gram ? bt.B : bt.A;
+        mats_.insert(mats_.end(), B.begin(), B.end());
         mat_index[key] = off;
       } else {
         off = it->second;
@@ -215,6 +219,7 @@ void Plan::build_lines() {
       L.constrained = p.reduction == Reduction::StageConstrained ? 1 : 0;
       L.kerr = p.spin != 0.0 ? 1 : 0;
       L.dissipate = p.dissipation > 0.0 ? 1 : 0;
+      L.gram = band_gram_needed(-2, m, p.ell_max_first, in.T) ? 1 : 0;
       L.mat_off = off;
       L.node_off = node_off;
       L.mass = p.mass;
@@ -272,8 +277,8 @@ void Plan::build_tiles() {
           }
           t.lo = lo;
           t.hi = hi;
-          t.lds_bytes = tile_lds_bytes(hi - lo, t.mhi - t.mlo, L.n_ell, L.ntheta, L.constrained,
-                                       L.kerr);
+          t.lds_bytes = tile_lds_bytes(hi - lo, t.mhi - t.mlo, t1 - t0, L.n_ell, L.ntheta,
+                                       L.constrained, L.kerr, L.gram);
         };
         // Largest row count within the LDS budget, capped for parallelism.
         int rows = tile_rows_ > 0 ? tile_rows_ : std::min(256, parallel_rows);
@@ -504,7 +509,7 @@ double Plan::import_state(int i, const cplx* host, double tol) {
     for (int f = 0; f < 3; ++f)
       for (int r = 0; r < N; ++r) {
         const cplx* v = at(mi, f, r);
-        const std::vector<cplx> c = bt.analyze(v);
+        const std::vector<cplx> c = bt.coefficients_of(v);
         const std::vector<cplx> back = bt.synthesize(c);
         for (int l = 0; l < L.n_ell; ++l)
           modal[L.base + static_cast<size_t>(r) * 3 * L.n_ell + f * L.n_ell + l] = {c[l].real(),
@@ -690,8 +695,10 @@ std::vector<WaveformRecordC> Plan::sample_waveform(int i) {
           ell_min = L.ell_min;
           modal.resize(L.n_ell);
           for (int l = 0; l < L.n_ell; ++l) modal[l] = cplx(row[l].re, row[l].im);
+          // The reference analyses the nodal endpoint S c (+ residual): A S c = G c.
+          const BandTransform bt = make_band_transform(-2, m, in.params.ell_max_first, in.T);
+          modal = bt.gram_apply(modal);
           if (!in.residual.empty()) {
-            const BandTransform bt = make_band_transform(-2, m, in.params.ell_max_first, in.T);
             const size_t o = ((static_cast<size_t>(mi) * 13 + 2) * in.N + r) * in.T;
             const std::vector<cplx> c = bt.analyze(&in.residual[o]);
             for (int l = 0; l < L.n_ell; ++l) modal[l] += c[l];

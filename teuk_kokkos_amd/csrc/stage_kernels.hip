@@ -216,8 +216,12 @@ __global__ __launch_bounds__(kStageThreads) void teuk_stage_kernel(StageArgs a) 
   // 3-4. point RHS on the output rows and the RK combination.
   const double dt = a.dt_override > 0.0 ? a.dt_override : L.dt;
   const double gamma = L.constrained ? 0.0 : L.gamma;
-  for (int i = tid; i < (tile.t1 - tile.t0) * nl; i += kStageThreads) {
-    const int r = tile.t0 + i / nl, l = i % nl;
+  const int orows = tile.t1 - tile.t0;
+  // Terms of the RHS that are linear in the stage with theta-independent
+  // coefficients: the reference forms them on the nodal stage S c and
+  // projects with S A, i.e. they carry G = A S in modal coordinates.
+  auto linear_terms = [&](int r, int l, const cdouble* lap_in_ptr, cdouble& xp, cdouble& xq,
+                          cdouble& xs) {
     const cdouble* u = U + (r - tile.lo) * us;
     const cdouble Q = u[nl + l], S = u[2 * nl + l];
     const cdouble dpsi = apply_d<SCHEME>(T, U + 2 * nl + l, us, tile.lo, r, n, L.inv_h);
@@ -229,25 +233,28 @@ __global__ __launch_bounds__(kStageThreads) void teuk_stage_kernel(StageArgs a) 
       qeff = Q;
       dq = apply_d<SCHEME>(T, U + nl + l, us, tile.lo, r, n, L.inv_h);
     }
-    const cdouble dpt = apply_d<SCHEME>(T, PT + l, nl, tile.mlo, r, n, L.inv_h);
     const RadialCoefficients rc = radial_coefficients(M, spin, len, -2, L.m, L.h * r);
     const int ell = L.ell_min + l;
     const double lam = -static_cast<double>((ell + 2) * (ell - 1));
-    cdouble kp = cadd(cadd(cscale(rc.principal, dq), cmul(rc.q, qeff)),
-                      cadd(cmul(rc.psi, S), cscale(lam, S)));
-    cdouble kq = csub(dpt, cscale(gamma, csub(Q, dpsi)));
-    cdouble ks = PT[(r - tile.mlo) * nl + l];
+    const cdouble lap_in = lap_in_ptr ? *lap_in_ptr : S;
+    xp = cadd(cadd(cscale(rc.principal, dq), cmul(rc.q, qeff)),
+              cadd(cmul(rc.psi, S), cscale(lam, lap_in)));
+    xq = cscale(-gamma, csub(Q, dpsi));
+    xs = cmake(0.0, 0.0);
     if (L.dissipate) {
-      kp = cadd(kp, apply_diss<SCHEME>(T, U + l, us, tile.lo, r, n, L.h, L.eps));
-      kq = cadd(kq, apply_diss<SCHEME>(T, U + nl + l, us, tile.lo, r, n, L.h, L.eps));
-      ks = cadd(ks, apply_diss<SCHEME>(T, U + 2 * nl + l, us, tile.lo, r, n, L.h, L.eps));
+      xp = cadd(xp, apply_diss<SCHEME>(T, U + l, us, tile.lo, r, n, L.h, L.eps));
+      xq = cadd(xq, apply_diss<SCHEME>(T, U + nl + l, us, tile.lo, r, n, L.h, L.eps));
+      xs = apply_diss<SCHEME>(T, U + 2 * nl + l, us, tile.lo, r, n, L.h, L.eps);
     }
+  };
+  auto store = [&](int r, int l, cdouble kp, cdouble kq, cdouble ks) {
     const long long g = L.base + static_cast<long long>(r) * us + l;
     if constexpr (STAGE == 0) {
       a.kout[g] = kp;
       a.kout[g + nl] = kq;
       a.kout[g + 2 * nl] = ks;
     } else {
+      const cdouble* u = U + (r - tile.lo) * us;
       const cdouble k[3] = {kp, kq, ks};
 #pragma unroll
       for (int f = 0; f < 3; ++f) {
@@ -265,6 +272,65 @@ __global__ __launch_bounds__(kStageThreads) void teuk_stage_kernel(StageArgs a) 
         }
       }
     }
+  };
+
+  if (!L.gram) {
+    for (int i = tid; i < orows * nl; i += kStageThreads) {
+      const int r = tile.t0 + i / nl, l = i % nl;
+      cdouble xp, xq, xs;
+      linear_terms(r, l, nullptr, xp, xq, xs);
+      const cdouble dpt = apply_d<SCHEME>(T, PT + l, nl, tile.mlo, r, n, L.inv_h);
+      const cdouble pt = PT[(r - tile.mlo) * nl + l];
+      store(r, l, xp, cadd(dpt, xq), cadd(pt, xs));
+    }
+    return;
+  }
+
+  // Gram path: G in LDS, pre-projection terms X in LDS, then k = G X.
+  double* Gm = reinterpret_cast<double*>(L.kerr ? NOD + nodal_chunk_rows(nt) * nt
+                                               : reinterpret_cast<cdouble*>(Smat));
+  cdouble* X = reinterpret_cast<cdouble*>(Gm + gram_doubles(nl));
+  {
+    const double* g = a.mats + L.mat_off + 2 * nt * nl;
+    for (int i = tid; i < nl * nl; i += kStageThreads) Gm[i] = g[i];
+  }
+  __syncthreads();
+  // X = pre-projection terms; the Laplacian acts on G Psi (the reference's
+  // S Lambda A applied to the nodal stage S c).
+  for (int i = tid; i < orows * nl; i += kStageThreads) {
+    const int r = tile.t0 + i / nl, l = i % nl;
+    const cdouble* u = U + (r - tile.lo) * us + 2 * nl;
+    double re = 0.0, im = 0.0;
+    for (int k = 0; k < nl; ++k) {
+      re += Gm[l * nl + k] * u[k].re;
+      im += Gm[l * nl + k] * u[k].im;
+    }
+    const cdouble gpsi = cmake(re, im);
+    cdouble xp, xq, xs;
+    linear_terms(r, l, &gpsi, xp, xq, xs);
+    cdouble* x = X + (r - tile.t0) * us;
+    x[l] = xp;
+    x[nl + l] = xq;
+    x[2 * nl + l] = xs;
+  }
+  __syncthreads();
+  for (int i = tid; i < orows * nl; i += kStageThreads) {
+    const int r = tile.t0 + i / nl, l = i % nl;
+    const cdouble* x = X + (r - tile.t0) * us;
+    const double* grow = Gm + l * nl;
+    double pr = 0, pi = 0, qr = 0, qi = 0, sr = 0, si = 0;
+    for (int k = 0; k < nl; ++k) {
+      const double gk = grow[k];
+      pr += gk * x[k].re;
+      pi += gk * x[k].im;
+      qr += gk * x[nl + k].re;
+      qi += gk * x[nl + k].im;
+      sr += gk * x[2 * nl + k].re;
+      si += gk * x[2 * nl + k].im;
+    }
+    const cdouble dpt = apply_d<SCHEME>(T, PT + l, nl, tile.mlo, r, n, L.inv_h);
+    const cdouble pt = PT[(r - tile.mlo) * nl + l];
+    store(r, l, cmake(pr, pi), cadd(dpt, cmake(qr, qi)), cadd(pt, cmake(sr, si)));
   }
 }
 
@@ -278,7 +344,8 @@ __global__ __launch_bounds__(kStageThreads) void teuk_nodal_to_modal(NodalArgs a
   const int line = blockIdx.x / a.nr, r = blockIdx.x % a.nr;
   const LineDesc L = a.lines[line];
   const int mode = a.line_mode[line];
-  const double* A = a.mats + L.mat_off + L.ntheta * L.n_ell;
+  // B = G^-1 A: the coefficients c of a nodal vector S c.
+  const double* A = a.mats + L.mat_off + 2 * L.ntheta * L.n_ell + gram_doubles(L.n_ell);
   for (int i = threadIdx.x; i < 3 * L.n_ell; i += kStageThreads) {
     const int f = i / L.n_ell, l = i % L.n_ell;
     const cdouble* v = nodal + ((static_cast<long long>(mode) * 13 + f) * a.nr + r) * a.ntheta;

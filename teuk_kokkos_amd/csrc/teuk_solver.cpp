@@ -1,0 +1,147 @@
+// teuk_solver: the reference's command-line driver (src/main.cpp:26-55,
+// include/teuk/solver_driver.hpp:86-353) on the HIP path, written purely
+// against the C ABI of libteuk_hip.so.
+//
+//   teuk_solver --config run.cfg [key=value ...]
+//   teuk_solver backend
+//   teuk_solver --help
+//
+// Writes resolved_config.cfg and waveforms.csv into output.directory, and the
+// evolution timing (step loop only, fenced at sampling steps) to stdout.
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <filesystem>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/teuk_hip.h"
+
+namespace {
+
+std::string value_of(const std::string& resolved, const std::string& key) {
+  std::istringstream in(resolved);
+  std::string line;
+  const std::string prefix = key + " = ";
+  while (std::getline(in, line))
+    if (line.compare(0, prefix.size(), prefix) == 0) return line.substr(prefix.size());
+  return "";
+}
+
+void write_waveforms(std::ofstream& out, teuk_plan* plan) {
+  int count = 0;
+  teuk_sample_waveform(plan, 0, nullptr, 0, &count);
+  std::vector<teuk_waveform_record> recs(count > 0 ? count : 1);
+  if (teuk_sample_waveform(plan, 0, recs.data(), count, &count) != TEUK_OK)
+    throw std::runtime_error(teuk_last_error(plan));
+  char buf[512];
+  for (int i = 0; i < count; ++i) {
+    const auto& r = recs[i];
+    std::snprintf(buf, sizeof(buf), "%llu,%.17g,%s,%d,%d,%d,%.17g,%.17g,%.17g,%.17g\n",
+                  static_cast<unsigned long long>(r.step), r.time,
+                  r.boundary == 0 ? "scri" : "horizon", r.order, r.m, r.ell, r.rescaled_re,
+                  r.rescaled_im, r.endpoint_re, r.endpoint_im);
+    out << buf;
+  }
+  out.flush();
+}
+
+int run(const std::string& path, const std::vector<std::string>& overrides) {
+  std::ifstream in(path);
+  if (!in) throw std::runtime_error("cannot open configuration file: " + path);
+  std::stringstream ss;
+  ss << in.rdbuf();
+  const std::string text = ss.str();
+  std::vector<const char*> ov;
+  for (const auto& o : overrides) ov.push_back(o.c_str());
+  size_t need = 0;
+  if (teuk_config_resolve(text.c_str(), ov.data(), static_cast<int>(ov.size()), nullptr, 0,
+                          &need) != TEUK_OK)
+    throw std::invalid_argument(teuk_config_last_error());
+  std::string resolved(need, '\0');
+  teuk_config_resolve(text.c_str(), ov.data(), static_cast<int>(ov.size()), resolved.data(), need,
+                      &need);
+  resolved.resize(need - 1);
+  if (value_of(resolved, "plus2.enabled") == "true")
+    throw std::runtime_error("plus2 mode '" + value_of(resolved, "plus2.mode") +
+                             "' is not production integrated; use plus2.enabled=false and "
+                             "plus2.mode=disabled");
+  const std::filesystem::path outdir = value_of(resolved, "output.directory");
+  std::filesystem::create_directories(outdir);
+  {
+    std::ofstream cfg(outdir / "resolved_config.cfg");
+    cfg << "# teuk_solver executable_version = 0.2.0\n"
+        << "# backend = hip gfx950 (modal fused RK4 stage kernels)\n"
+        << "# build_precision = complex_binary64\n"
+        << "# config_schema_version = 1\n"
+        << resolved;
+  }
+  teuk_plan* plan = nullptr;
+  if (teuk_plan_create(text.c_str(), ov.data(), static_cast<int>(ov.size()), &plan) != TEUK_OK)
+    throw std::invalid_argument(teuk_config_last_error());
+  struct Guard {
+    teuk_plan* p;
+    ~Guard() { teuk_plan_destroy(p); }
+  } guard{plan};
+  auto ok = [&](int rc) {
+    if (rc != TEUK_OK) throw std::runtime_error(teuk_last_error(plan));
+  };
+  ok(teuk_initialize(plan));
+  const int steps = std::stoi(value_of(resolved, "steps"));
+  const int every = std::stoi(value_of(resolved, "output.diagnostic_every"));
+  std::ofstream wf(outdir / "waveforms.csv");
+  wf << "step,time,boundary,order,m,ell,rescaled_re,rescaled_im,endpoint_re,endpoint_im\n";
+  write_waveforms(wf, plan);
+  double evolution = 0.0;
+  const auto total_start = std::chrono::steady_clock::now();
+  auto start = std::chrono::steady_clock::now();
+  for (int s = 1; s <= steps; ++s) {
+    ok(teuk_advance(plan, 1));
+    if (s % every == 0 || s == steps) {
+      ok(teuk_sync(plan));
+      evolution += std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+      write_waveforms(wf, plan);
+      start = std::chrono::steady_clock::now();
+    }
+  }
+  ok(teuk_sync(plan));
+  const double total =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - total_start).count();
+  const double points = teuk_plan_grid_points(plan) * steps;
+  std::cout << "backend=hip gfx950\n"
+            << "boundary_policy=zero-SAT; no incoming propagating modes\n"
+            << "source_policy=disabled\n"
+            << "evolution_wall_seconds=" << evolution << '\n'
+            << "total_wall_seconds=" << total << '\n'
+            << "grid_point_steps_per_second=" << points / evolution << '\n';
+  return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  try {
+    const std::string cmd = argc > 1 ? argv[1] : "--help";
+    if (cmd == "backend") {
+      std::cout << "teuk-mi355x 0.2.0\nexecution: HIP gfx950\n";
+      return 0;
+    }
+    if (cmd == "--config") {
+      if (argc < 3) throw std::invalid_argument("--config requires a configuration path");
+      std::vector<std::string> overrides(argv + 3, argv + argc);
+      return run(argv[2], overrides);
+    }
+    if (cmd == "help" || cmd == "--help" || cmd == "-h") {
+      std::cout << "Usage:\n  teuk_solver --config run.cfg [key=value ...]\n"
+                   "  teuk_solver backend\n  teuk_solver --help\n";
+      return 0;
+    }
+    throw std::invalid_argument("unknown command: " + cmd);
+  } catch (const std::exception& e) {
+    std::cerr << "error: " << e.what() << '\n';
+    return 2;
+  }
+}
