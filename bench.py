@@ -77,6 +77,9 @@ def main() -> int:
     ap.add_argument("--tile-rows", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="first run one device-to-device copy of this size (a counter "
+                         "calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (got --gpus "
@@ -85,6 +88,17 @@ def main() -> int:
 
     import teuk_kokkos_amd as T
     from teuk_kokkos_amd import workloads as W
+
+    if args.calibrate_copy_mib:
+        import ctypes
+        import numpy as np
+        n = args.calibrate_copy_mib << 20
+        src = T.DeviceArray(np.zeros(n // 16, dtype=np.complex128))
+        dst = T.DeviceArray(np.zeros(n // 16, dtype=np.complex128))
+        T.lib().teuk_memcpy(ctypes.c_void_p(dst.ptr), ctypes.c_void_p(src.ptr), n, 2)
+        print(f"[bench] calibration copy {n} bytes", file=sys.stderr, flush=True)
+        src.free()
+        dst.free()
 
     cases, desc = workload(args.workload)
     print(f"[bench] building plan: {len(cases)} instances", file=sys.stderr, flush=True)

@@ -165,6 +165,15 @@ int teuk_plan_set_kernel_timing(teuk_plan* plan, int enable);
 int teuk_plan_kernel_timing(const teuk_plan* plan, double* total_ms, long long* launches);
 
 /* ---------------------------------------------------------------------
+ * Device buffers for callers without a HIP runtime of their own (no
+ * reference counterpart: the reference's Kokkos::View allocations).
+ * kind: 0 host->device, 1 device->host, 2 device->device; synchronous.
+ * ------------------------------------------------------------------- */
+int teuk_device_alloc(size_t bytes, void** ptr);
+int teuk_device_free(void* ptr);
+int teuk_memcpy(void* dst, const void* src, size_t bytes, int kind);
+
+/* ---------------------------------------------------------------------
  * Host-side tables (no GPU needed; used by the CPU tests)
  * ------------------------------------------------------------------- */
 /* scheme: 0 d4-2, 1 d8-4, 2 d10-5.  h*D(row, col) and H(i)/h of an n-point

@@ -78,6 +78,9 @@ SIGNATURES = {
     "teuk_plan_tiles": (_I, [_P, _IP, _IP]),
     "teuk_plan_set_kernel_timing": (_I, [_P, _I]),
     "teuk_plan_kernel_timing": (_I, [_P, _DP, ctypes.POINTER(ctypes.c_longlong)]),
+    "teuk_device_alloc": (_I, [ctypes.c_size_t, ctypes.POINTER(_P)]),
+    "teuk_device_free": (_I, [_P]),
+    "teuk_memcpy": (_I, [_P, _P, ctypes.c_size_t, _I]),
     "teuk_sbp_derivative_entry": (_D, [_I, _I, _I, _I]),
     "teuk_sbp_norm_weight": (_D, [_I, _I, _I]),
     "teuk_band_matrices": (_I, [_I, _I, _I, _I, _DP, _DP]),
@@ -158,6 +161,44 @@ def band_matrices(spin: int, m: int, ell_max: int, ntheta: int):
     if got != nl:
         raise TeukInvalidArgument(lib().teuk_config_last_error().decode())
     return S, A
+
+
+class DeviceArray:
+    """A device buffer holding one complex128 numpy-shaped array."""
+
+    def __init__(self, host: np.ndarray):
+        self.shape = host.shape
+        self.nbytes = host.nbytes
+        ptr = ctypes.c_void_p()
+        rc = lib().teuk_device_alloc(self.nbytes, ctypes.byref(ptr))
+        if rc != TEUK_OK:
+            _raise(rc, lib().teuk_config_last_error().decode())
+        self.ptr = ptr.value
+        self.upload(host)
+
+    def upload(self, host: np.ndarray):
+        arr = np.ascontiguousarray(host, dtype=np.complex128)
+        assert arr.nbytes == self.nbytes
+        rc = lib().teuk_memcpy(ctypes.c_void_p(self.ptr), arr.ctypes.data_as(ctypes.c_void_p),
+                               self.nbytes, 0)
+        if rc != TEUK_OK:
+            _raise(rc, lib().teuk_config_last_error().decode())
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=np.complex128)
+        rc = lib().teuk_memcpy(out.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(self.ptr),
+                               self.nbytes, 1)
+        if rc != TEUK_OK:
+            _raise(rc, lib().teuk_config_last_error().decode())
+        return out
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            lib().teuk_device_free(ctypes.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        self.free()
 
 
 class Plan:

@@ -319,6 +319,34 @@ int teuk_plan_kernel_timing(const teuk_plan* plan, double* total_ms, long long* 
   });
 }
 
+int teuk_device_alloc(size_t bytes, void** ptr) {
+  return guarded(g_config_error, [&] {
+    if (!ptr) throw std::invalid_argument("null output pointer");
+    *ptr = nullptr;
+    const hipError_t e = hipMalloc(ptr, bytes ? bytes : 1);
+    if (e != hipSuccess) throw std::runtime_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+  });
+}
+
+int teuk_device_free(void* ptr) {
+  return guarded(g_config_error, [&] {
+    const hipError_t e = hipFree(ptr);
+    if (e != hipSuccess) throw std::runtime_error(std::string("hipFree: ") + hipGetErrorString(e));
+  });
+}
+
+int teuk_memcpy(void* dst, const void* src, size_t bytes, int kind) {
+  return guarded(g_config_error, [&] {
+    if (!dst || !src) throw std::invalid_argument("null copy pointer");
+    const hipMemcpyKind k = kind == 0   ? hipMemcpyHostToDevice
+                            : kind == 1 ? hipMemcpyDeviceToHost
+                            : kind == 2 ? hipMemcpyDeviceToDevice
+                                        : throw std::invalid_argument("unknown copy kind");
+    const hipError_t e = hipMemcpy(dst, src, bytes, k);
+    if (e != hipSuccess) throw std::runtime_error(std::string("hipMemcpy: ") + hipGetErrorString(e));
+  });
+}
+
 double teuk_sbp_derivative_entry(int scheme, int n, int row, int col) {
   if (scheme < 0 || scheme > 2) return 0.0;
   return teuk::sbp_derivative_entry(static_cast<teuk::RadialScheme>(scheme), n, row, col);
