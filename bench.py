@@ -77,6 +77,8 @@ def main() -> int:
     ap.add_argument("--tile-rows", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="sweep LDS budget x tile rows before timing (cached per workload)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this size (a counter "
                          "calibration for rocprofv3 --pmc)")
@@ -107,6 +109,12 @@ def main() -> int:
         plan.set_lds_budget(args.lds_budget)
     if args.tile_rows:
         plan.set_tile_rows(args.tile_rows)
+    if args.autotune:
+        from teuk_kokkos_amd import autotune
+        plan.initialize()
+        best = autotune.tune(plan, W.texts(cases), log=lambda m: print(m, file=sys.stderr,
+                                                                      flush=True))
+        print(f"[bench] autotuned: {best}", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     plan.initialize()
     print(f"[bench] initial data {time.perf_counter() - t0:.1f} s; tiles {plan.tiles()}",

@@ -61,7 +61,7 @@ __host__ __device__ inline int gram_doubles(int n_ell) {
 
 __host__ __device__ inline int tile_lds_bytes(int rows, int mrows, int orows, int n_ell,
                                               int ntheta, int constrained, int kerr, int gram) {
-  long long b = 16LL * rows * 3 * n_ell + 16LL * mrows * n_ell;
+  long long b = 80LL * mrows + 16LL * rows * 3 * n_ell + 16LL * mrows * n_ell;
   if (constrained) b += 16LL * mrows * n_ell;
   if (kerr) b += 16LL * ntheta * n_ell + 16LL * nodal_chunk_rows(ntheta) * ntheta;
   if (gram) b += 8LL * gram_doubles(n_ell) + 16LL * orows * 3 * n_ell;

@@ -106,6 +106,17 @@ __device__ __forceinline__ cdouble apply_diss(const SchemeTable& T, const cdoubl
 
 }  // namespace
 
+// Per-row radial coefficients, tabulated once per tile row in LDS.
+struct RowCoeffs {
+  double adv2;         // 2 K
+  double principal;    // H_R
+  double time_radial;  // theta-independent part of C_T
+  double g_real, g_imag_m;
+  double q_re, q_im, psi_re, psi_im;
+  double pad;
+};
+static_assert(sizeof(RowCoeffs) == 80, "RowCoeffs must stay 16-byte aligned");
+
 // STAGE 0: k -> kout.  STAGE 1..4: classical RK4 stage combination with the
 // accumulator carrying y (16 state vectors of HBM traffic per step).
 template <int SCHEME, int STAGE>
@@ -120,9 +131,17 @@ __global__ __launch_bounds__(kStageThreads) void teuk_stage_kernel(StageArgs a) 
   const int n = L.nr;
   const int rows = tile.hi - tile.lo;
   const int mrows = tile.mhi - tile.mlo;
+  const int orows = tile.t1 - tile.t0;
   const int us = 3 * nl;  // LDS row stride of the staged tile (cdouble)
 
-  cdouble* U = reinterpret_cast<cdouble*>(smem);
+  // (row, ell) and (row, theta) thread decompositions: one division each.
+  const int lane_l = tid % nl, row_l = tid / nl, step_l = kStageThreads / nl;
+  const bool on_l = row_l < step_l;
+  const int lane_t = tid % nt, row_t = tid / nt, step_t = kStageThreads / nt;
+  const bool on_t = row_t < step_t;
+
+  RowCoeffs* RC = reinterpret_cast<RowCoeffs*>(smem);
+  cdouble* U = reinterpret_cast<cdouble*>(RC + mrows);
   cdouble* PT = U + rows * us;
   cdouble* DP = PT + mrows * nl;
   cdouble* after_dp = L.constrained ? DP + mrows * nl : DP;
@@ -130,7 +149,9 @@ __global__ __launch_bounds__(kStageThreads) void teuk_stage_kernel(StageArgs a) 
   double* Amat = Smat + nt * nl;
   cdouble* NOD = reinterpret_cast<cdouble*>(Amat + nt * nl);
 
-  // 1. stage the tile rows [lo, hi) of all three fields.
+  const double M = L.mass, spin = L.spin, len = L.length;
+  // 1. stage the tile rows [lo, hi) of all three fields, the band matrices
+  //    and the per-row coefficients.
   {
     const cdouble* src = a.yin + L.base + static_cast<long long>(tile.lo) * us;
     const int total = rows * us;
@@ -139,76 +160,83 @@ __global__ __launch_bounds__(kStageThreads) void teuk_stage_kernel(StageArgs a) 
       const double* m = a.mats + L.mat_off;
       for (int i = tid; i < 2 * nt * nl; i += kStageThreads) Smat[i] = m[i];
     }
+    for (int j = tid; j < mrows; j += kStageThreads) {
+      const RadialCoefficients rc =
+          radial_coefficients(M, spin, len, -2, L.m, L.h * (tile.mlo + j));
+      RC[j] = {2.0 * rc.adv, rc.principal, rc.time_radial, rc.g_real, rc.g_imag_m,
+               rc.q.re,      rc.q.im,      rc.psi.re,      rc.psi.im, 0.0};
+    }
   }
   __syncthreads();
 
-  const double M = L.mass, spin = L.spin, len = L.length;
   // D Psi on the intermediate rows (stage-constrained Q_eff = D Psi).
   if (L.constrained) {
-    for (int i = tid; i < mrows * nl; i += kStageThreads) {
-      const int j = tile.mlo + i / nl, l = i % nl;
-      DP[i] = apply_d<SCHEME>(T, U + 2 * nl + l, us, tile.lo, j, n, L.inv_h);
-    }
+    if (on_l)
+      for (int j = tile.mlo + row_l; j < tile.mhi; j += step_l)
+        DP[(j - tile.mlo) * nl + lane_l] =
+            apply_d<SCHEME>(T, U + 2 * nl + lane_l, us, tile.lo, j, n, L.inv_h);
     __syncthreads();
   }
 
-  // 2. psi_t on rows [mlo, mhi).
-  if (!L.kerr) {
-    for (int i = tid; i < mrows * nl; i += kStageThreads) {
-      const int j = tile.mlo + i / nl, l = i % nl;
-      const RadialCoefficients rc = radial_coefficients(M, spin, len, -2, L.m, L.h * j);
-      const double ct = time_coefficient(rc, spin, 0.0);
-      const cdouble g = definition_coefficient(rc, spin, -2, 0.0);
+  // 2. psi_t on rows [mlo, mhi).  First X = P + 2K Q_eff (into PT's slots).
+  if (on_l)
+    for (int j = tile.mlo + row_l; j < tile.mhi; j += step_l) {
+      const int jj = j - tile.mlo;
       const cdouble* u = U + (j - tile.lo) * us;
-      const cdouble P = u[l];
-      const cdouble Q = L.constrained ? DP[i] : u[nl + l];
-      const cdouble S = u[2 * nl + l];
-      const double k2 = 2.0 * rc.adv;
-      const cdouble num = csub(cadd(P, cscale(k2, Q)), cmul(g, S));
-      PT[i] = cdivr(num, ct);
+      const cdouble Q = L.constrained ? DP[jj * nl + lane_l] : u[nl + lane_l];
+      const double k2 = RC[jj].adv2;
+      const cdouble P = u[lane_l];
+      if (!L.kerr) {
+        // a = 0: C_T and G_m are theta independent, the coupling is diagonal.
+        const cdouble g = cmake(RC[jj].g_real, RC[jj].g_imag_m);
+        const cdouble num = csub(cadd(P, cscale(k2, Q)), cmul(g, u[2 * nl + lane_l]));
+        PT[jj * nl + lane_l] = cdivr(num, RC[jj].time_radial);
+      } else {
+        PT[jj * nl + lane_l] = cadd(P, cscale(k2, Q));
+      }
     }
-    __syncthreads();
-  } else {
+  __syncthreads();
+  if (L.kerr) {
     const double* sinv = a.nodes + L.node_off;
     const double* cosv = sinv + nt;
+    const double sn = on_t ? sinv[lane_t] : 0.0, cs = on_t ? cosv[lane_t] : 0.0;
     const int ch = nodal_chunk_rows(nt);
     for (int c0 = tile.mlo; c0 < tile.mhi; c0 += ch) {
-      const int cn = min(ch, tile.mhi - c0);
-      for (int i = tid; i < cn * nt; i += kStageThreads) {
-        const int j = c0 + i / nt, th = i % nt;
-        const RadialCoefficients rc = radial_coefficients(M, spin, len, -2, L.m, L.h * j);
-        const double k2 = 2.0 * rc.adv;
-        const cdouble* u = U + (j - tile.lo) * us;
-        const cdouble* dq = DP + (j - tile.mlo) * nl;
-        const double* srow = Smat + th * nl;
-        double xr = 0.0, xi = 0.0, yr = 0.0, yi = 0.0;
-        for (int l = 0; l < nl; ++l) {
-          const double s = srow[l];
-          const cdouble P = u[l];
-          const cdouble Q = L.constrained ? dq[l] : u[nl + l];
-          const cdouble S = u[2 * nl + l];
-          xr += s * (P.re + k2 * Q.re);
-          xi += s * (P.im + k2 * Q.im);
-          yr += s * S.re;
-          yi += s * S.im;
+      const int c1 = min(c0 + ch, tile.mhi);
+      // synthesis and the pointwise (X - G_m Psi) / C_T at (j, theta)
+      if (on_t)
+        for (int j = c0 + row_t; j < c1; j += step_t) {
+          const int jj = j - tile.mlo;
+          const cdouble* x = PT + jj * nl;
+          const cdouble* ps = U + (j - tile.lo) * us + 2 * nl;
+          const double* srow = Smat + lane_t * nl;
+          double xr = 0.0, xi = 0.0, yr = 0.0, yi = 0.0;
+          for (int l = 0; l < nl; ++l) {
+            const double w = srow[l];
+            const cdouble xv = x[l], pv = ps[l];
+            xr += w * xv.re;
+            xi += w * xv.im;
+            yr += w * pv.re;
+            yi += w * pv.im;
+          }
+          const double ct = RC[jj].time_radial - spin * spin * sn * sn;
+          const cdouble g = cmake(RC[jj].g_real, RC[jj].g_imag_m + 2.0 * (-2.0 * spin * cs));
+          const cdouble num = csub(cmake(xr, xi), cmul(g, cmake(yr, yi)));
+          NOD[(j - c0) * nt + lane_t] = cdivr(num, ct);
         }
-        const double ct = time_coefficient(rc, spin, sinv[th]);
-        const cdouble g = definition_coefficient(rc, spin, -2, cosv[th]);
-        const cdouble num = csub(cmake(xr, xi), cmul(g, cmake(yr, yi)));
-        NOD[i] = cdivr(num, ct);
-      }
       __syncthreads();
-      for (int i = tid; i < cn * nl; i += kStageThreads) {
-        const int jj = i / nl, l = i % nl;
-        const double* arow = Amat + l * nt;
-        const cdouble* nod = NOD + jj * nt;
-        double re = 0.0, im = 0.0;
-        for (int th = 0; th < nt; ++th) {
-          re += arow[th] * nod[th].re;
-          im += arow[th] * nod[th].im;
+      // analysis A (nodal psi_t) -> PT rows of this chunk
+      if (on_l)
+        for (int j = c0 + row_l; j < c1; j += step_l) {
+          const double* arow = Amat + lane_l * nt;
+          const cdouble* nod = NOD + (j - c0) * nt;
+          double re = 0.0, im = 0.0;
+          for (int th = 0; th < nt; ++th) {
+            re += arow[th] * nod[th].re;
+            im += arow[th] * nod[th].im;
+          }
+          PT[(j - tile.mlo) * nl + lane_l] = cmake(re, im);
         }
-        PT[(c0 - tile.mlo + jj) * nl + l] = cmake(re, im);
-      }
       __syncthreads();
     }
   }
@@ -216,7 +244,6 @@ __global__ __launch_bounds__(kStageThreads) void teuk_stage_kernel(StageArgs a) 
   // 3-4. point RHS on the output rows and the RK combination.
   const double dt = a.dt_override > 0.0 ? a.dt_override : L.dt;
   const double gamma = L.constrained ? 0.0 : L.gamma;
-  const int orows = tile.t1 - tile.t0;
   // Terms of the RHS that are linear in the stage with theta-independent
   // coefficients: the reference forms them on the nodal stage S c and
   // projects with S A, i.e. they carry G = A S in modal coordinates.
@@ -233,12 +260,12 @@ __global__ __launch_bounds__(kStageThreads) void teuk_stage_kernel(StageArgs a) 
       qeff = Q;
       dq = apply_d<SCHEME>(T, U + nl + l, us, tile.lo, r, n, L.inv_h);
     }
-    const RadialCoefficients rc = radial_coefficients(M, spin, len, -2, L.m, L.h * r);
+    const RowCoeffs& rc = RC[r - tile.mlo];
     const int ell = L.ell_min + l;
     const double lam = -static_cast<double>((ell + 2) * (ell - 1));
     const cdouble lap_in = lap_in_ptr ? *lap_in_ptr : S;
-    xp = cadd(cadd(cscale(rc.principal, dq), cmul(rc.q, qeff)),
-              cadd(cmul(rc.psi, S), cscale(lam, lap_in)));
+    xp = cadd(cadd(cscale(rc.principal, dq), cmul(cmake(rc.q_re, rc.q_im), qeff)),
+              cadd(cmul(cmake(rc.psi_re, rc.psi_im), S), cscale(lam, lap_in)));
     xq = cscale(-gamma, csub(Q, dpsi));
     xs = cmake(0.0, 0.0);
     if (L.dissipate) {
@@ -275,14 +302,14 @@ __global__ __launch_bounds__(kStageThreads) void teuk_stage_kernel(StageArgs a) 
   };
 
   if (!L.gram) {
-    for (int i = tid; i < orows * nl; i += kStageThreads) {
-      const int r = tile.t0 + i / nl, l = i % nl;
-      cdouble xp, xq, xs;
-      linear_terms(r, l, nullptr, xp, xq, xs);
-      const cdouble dpt = apply_d<SCHEME>(T, PT + l, nl, tile.mlo, r, n, L.inv_h);
-      const cdouble pt = PT[(r - tile.mlo) * nl + l];
-      store(r, l, xp, cadd(dpt, xq), cadd(pt, xs));
-    }
+    if (on_l)
+      for (int r = tile.t0 + row_l; r < tile.t1; r += step_l) {
+        cdouble xp, xq, xs;
+        linear_terms(r, lane_l, nullptr, xp, xq, xs);
+        const cdouble dpt = apply_d<SCHEME>(T, PT + lane_l, nl, tile.mlo, r, n, L.inv_h);
+        const cdouble pt = PT[(r - tile.mlo) * nl + lane_l];
+        store(r, lane_l, xp, cadd(dpt, xq), cadd(pt, xs));
+      }
     return;
   }
 
@@ -297,41 +324,42 @@ __global__ __launch_bounds__(kStageThreads) void teuk_stage_kernel(StageArgs a) 
   __syncthreads();
   // X = pre-projection terms; the Laplacian acts on G Psi (the reference's
   // S Lambda A applied to the nodal stage S c).
-  for (int i = tid; i < orows * nl; i += kStageThreads) {
-    const int r = tile.t0 + i / nl, l = i % nl;
-    const cdouble* u = U + (r - tile.lo) * us + 2 * nl;
-    double re = 0.0, im = 0.0;
-    for (int k = 0; k < nl; ++k) {
-      re += Gm[l * nl + k] * u[k].re;
-      im += Gm[l * nl + k] * u[k].im;
+  if (on_l)
+    for (int r = tile.t0 + row_l; r < tile.t1; r += step_l) {
+      const cdouble* u = U + (r - tile.lo) * us + 2 * nl;
+      double re = 0.0, im = 0.0;
+      for (int k = 0; k < nl; ++k) {
+        re += Gm[lane_l * nl + k] * u[k].re;
+        im += Gm[lane_l * nl + k] * u[k].im;
+      }
+      const cdouble gpsi = cmake(re, im);
+      cdouble xp, xq, xs;
+      linear_terms(r, lane_l, &gpsi, xp, xq, xs);
+      cdouble* x = X + (r - tile.t0) * us;
+      x[lane_l] = xp;
+      x[nl + lane_l] = xq;
+      x[2 * nl + lane_l] = xs;
     }
-    const cdouble gpsi = cmake(re, im);
-    cdouble xp, xq, xs;
-    linear_terms(r, l, &gpsi, xp, xq, xs);
-    cdouble* x = X + (r - tile.t0) * us;
-    x[l] = xp;
-    x[nl + l] = xq;
-    x[2 * nl + l] = xs;
-  }
   __syncthreads();
-  for (int i = tid; i < orows * nl; i += kStageThreads) {
-    const int r = tile.t0 + i / nl, l = i % nl;
-    const cdouble* x = X + (r - tile.t0) * us;
-    const double* grow = Gm + l * nl;
-    double pr = 0, pi = 0, qr = 0, qi = 0, sr = 0, si = 0;
-    for (int k = 0; k < nl; ++k) {
-      const double gk = grow[k];
-      pr += gk * x[k].re;
-      pi += gk * x[k].im;
-      qr += gk * x[nl + k].re;
-      qi += gk * x[nl + k].im;
-      sr += gk * x[2 * nl + k].re;
-      si += gk * x[2 * nl + k].im;
+  if (on_l)
+    for (int r = tile.t0 + row_l; r < tile.t1; r += step_l) {
+      const cdouble* x = X + (r - tile.t0) * us;
+      const double* grow = Gm + lane_l * nl;
+      double pr = 0, pi = 0, qr = 0, qi = 0, sr = 0, si = 0;
+      for (int k = 0; k < nl; ++k) {
+        const double gk = grow[k];
+        pr += gk * x[k].re;
+        pi += gk * x[k].im;
+        qr += gk * x[nl + k].re;
+        qi += gk * x[nl + k].im;
+        sr += gk * x[2 * nl + k].re;
+        si += gk * x[2 * nl + k].im;
+      }
+      const cdouble dpt = apply_d<SCHEME>(T, PT + lane_l, nl, tile.mlo, r, n, L.inv_h);
+      const cdouble pt = PT[(r - tile.mlo) * nl + lane_l];
+      store(r, lane_l, cmake(pr, pi), cadd(dpt, cmake(qr, qi)), cadd(pt, cmake(sr, si)));
     }
-    const cdouble dpt = apply_d<SCHEME>(T, PT + l, nl, tile.mlo, r, n, L.inv_h);
-    const cdouble pt = PT[(r - tile.mlo) * nl + l];
-    store(r, l, cmake(pr, pi), cadd(dpt, cmake(qr, qi)), cadd(pt, cmake(sr, si)));
-  }
+  (void)orows;
 }
 
 // ------------------------------------------------------------ band transforms
