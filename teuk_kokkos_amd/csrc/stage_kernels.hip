@@ -35,7 +35,8 @@ __device__ __forceinline__ cdouble apply_d(const SchemeTable& T, const cdouble* 
   if (row < W) {
     const double* c = &T.left[row * kMaxClosureWidth];
     const cdouble* v = base + (0 - lo) * stride;
-#pragma unroll
+    // closure rows are rare: keep them compact to save registers
+#pragma unroll 2
     for (int k = 0; k < C; ++k) {
       const cdouble x = v[k * stride];
       re += c[k] * x.re;
@@ -44,7 +45,7 @@ __device__ __forceinline__ cdouble apply_d(const SchemeTable& T, const cdouble* 
   } else if (row + W >= n) {
     const double* c = &T.left[(n - 1 - row) * kMaxClosureWidth];
     const cdouble* v = base + (n - 1 - lo) * stride;
-#pragma unroll
+#pragma unroll 2
     for (int k = 0; k < C; ++k) {
       const cdouble x = v[-k * stride];
       re -= c[k] * x.re;
@@ -104,6 +105,24 @@ __device__ __forceinline__ cdouble apply_diss(const SchemeTable& T, const cdoubl
   return cmake(f * re, f * im);
 }
 
+// Global -> LDS copy with kCopyBatch loads in flight per thread before the
+// first LDS write (a plain loop would wait for every load in turn).
+constexpr int kCopyBatch = 8;
+template <class V>
+__device__ __forceinline__ void stage_copy(V* __restrict__ dst, const V* __restrict__ src,
+                                           int count, int tid) {
+  for (int base = tid; base < count; base += kCopyBatch * kStageThreads) {
+    V tmp[kCopyBatch];
+    // Clamped indices keep every load and store unconditional, so all loads
+    // are in flight before the first wait (out-of-range lanes rewrite the
+    // last element with its own value).
+#pragma unroll
+    for (int k = 0; k < kCopyBatch; ++k) tmp[k] = src[min(base + k * kStageThreads, count - 1)];
+#pragma unroll
+    for (int k = 0; k < kCopyBatch; ++k) dst[min(base + k * kStageThreads, count - 1)] = tmp[k];
+  }
+}
+
 }  // namespace
 
 // Per-row radial coefficients, tabulated once per tile row in LDS.
@@ -120,7 +139,7 @@ static_assert(sizeof(RowCoeffs) == 80, "RowCoeffs must stay 16-byte aligned");
 // STAGE 0: k -> kout.  STAGE 1..4: classical RK4 stage combination with the
 // accumulator carrying y (16 state vectors of HBM traffic per step).
 template <int SCHEME, int STAGE>
-__global__ __launch_bounds__(kStageThreads) void teuk_stage_kernel(StageArgs a) {
+__global__ __launch_bounds__(kStageThreads, 2) void teuk_stage_kernel(StageArgs a) {
   extern __shared__ double smem[];
   const TileDesc tile = a.tiles[blockIdx.x];
   const LineDesc L = a.lines[tile.line];
@@ -153,13 +172,8 @@ __global__ __launch_bounds__(kStageThreads) void teuk_stage_kernel(StageArgs a) 
   // 1. stage the tile rows [lo, hi) of all three fields, the band matrices
   //    and the per-row coefficients.
   {
-    const cdouble* src = a.yin + L.base + static_cast<long long>(tile.lo) * us;
-    const int total = rows * us;
-    for (int i = tid; i < total; i += kStageThreads) U[i] = src[i];
-    if (L.kerr) {
-      const double* m = a.mats + L.mat_off;
-      for (int i = tid; i < 2 * nt * nl; i += kStageThreads) Smat[i] = m[i];
-    }
+    stage_copy(U, a.yin + L.base + static_cast<long long>(tile.lo) * us, rows * us, tid);
+    if (L.kerr) stage_copy(Smat, a.mats + L.mat_off, 2 * nt * nl, tid);
     for (int j = tid; j < mrows; j += kStageThreads) {
       const RadialCoefficients rc =
           radial_coefficients(M, spin, len, -2, L.m, L.h * (tile.mlo + j));
@@ -211,6 +225,7 @@ __global__ __launch_bounds__(kStageThreads) void teuk_stage_kernel(StageArgs a) 
           const cdouble* ps = U + (j - tile.lo) * us + 2 * nl;
           const double* srow = Smat + lane_t * nl;
           double xr = 0.0, xi = 0.0, yr = 0.0, yi = 0.0;
+#pragma unroll 4
           for (int l = 0; l < nl; ++l) {
             const double w = srow[l];
             const cdouble xv = x[l], pv = ps[l];
@@ -231,6 +246,7 @@ __global__ __launch_bounds__(kStageThreads) void teuk_stage_kernel(StageArgs a) 
           const double* arow = Amat + lane_l * nt;
           const cdouble* nod = NOD + (j - c0) * nt;
           double re = 0.0, im = 0.0;
+#pragma unroll 4
           for (int th = 0; th < nt; ++th) {
             re += arow[th] * nod[th].re;
             im += arow[th] * nod[th].im;
@@ -274,7 +290,28 @@ __global__ __launch_bounds__(kStageThreads) void teuk_stage_kernel(StageArgs a) 
       xs = apply_diss<SCHEME>(T, U + 2 * nl + l, us, tile.lo, r, n, L.h, L.eps);
     }
   };
-  auto store = [&](int r, int l, cdouble kp, cdouble kq, cdouble ks) {
+  // RK registers of one output point, loaded before the stencil work so the
+  // global latency overlaps it (the stores below could alias these loads, so
+  // the compiler would not hoist them by itself).
+  struct RkIn {
+    cdouble y0[3], acc0[3];
+  };
+  auto rk_load = [&](int r, int l) {
+    RkIn v{};
+    const long long g = L.base + static_cast<long long>(r) * us + l;
+    if constexpr (STAGE == 2 || STAGE == 3) {
+#pragma unroll
+      for (int f = 0; f < 3; ++f) {
+        v.y0[f] = a.y[g + f * nl];
+        v.acc0[f] = a.acc[g + f * nl];
+      }
+    } else if constexpr (STAGE == 4) {
+#pragma unroll
+      for (int f = 0; f < 3; ++f) v.acc0[f] = a.acc[g + f * nl];
+    }
+    return v;
+  };
+  auto store = [&](int r, int l, const RkIn& in, cdouble kp, cdouble kq, cdouble ks) {
     const long long g = L.base + static_cast<long long>(r) * us + l;
     if constexpr (STAGE == 0) {
       a.kout[g] = kp;
@@ -291,11 +328,10 @@ __global__ __launch_bounds__(kStageThreads) void teuk_stage_kernel(StageArgs a) 
           a.acc[gi] = cadd(y0, cscale(dt / 6.0, k[f]));
           a.yout[gi] = cadd(y0, cscale(0.5 * dt, k[f]));
         } else if constexpr (STAGE == 2 || STAGE == 3) {
-          const cdouble y0 = a.y[gi];
-          a.acc[gi] = cadd(a.acc[gi], cscale(dt / 3.0, k[f]));
-          a.yout[gi] = cadd(y0, cscale(STAGE == 2 ? 0.5 * dt : dt, k[f]));
+          a.acc[gi] = cadd(in.acc0[f], cscale(dt / 3.0, k[f]));
+          a.yout[gi] = cadd(in.y0[f], cscale(STAGE == 2 ? 0.5 * dt : dt, k[f]));
         } else {
-          a.yout[gi] = cadd(a.acc[gi], cscale(dt / 6.0, k[f]));
+          a.yout[gi] = cadd(in.acc0[f], cscale(dt / 6.0, k[f]));
         }
       }
     }
@@ -304,11 +340,12 @@ __global__ __launch_bounds__(kStageThreads) void teuk_stage_kernel(StageArgs a) 
   if (!L.gram) {
     if (on_l)
       for (int r = tile.t0 + row_l; r < tile.t1; r += step_l) {
+        const RkIn in = rk_load(r, lane_l);
         cdouble xp, xq, xs;
         linear_terms(r, lane_l, nullptr, xp, xq, xs);
         const cdouble dpt = apply_d<SCHEME>(T, PT + lane_l, nl, tile.mlo, r, n, L.inv_h);
         const cdouble pt = PT[(r - tile.mlo) * nl + lane_l];
-        store(r, lane_l, xp, cadd(dpt, xq), cadd(pt, xs));
+        store(r, lane_l, in, xp, cadd(dpt, xq), cadd(pt, xs));
       }
     return;
   }
@@ -318,8 +355,7 @@ __global__ __launch_bounds__(kStageThreads) void teuk_stage_kernel(StageArgs a) 
                                                : reinterpret_cast<cdouble*>(Smat));
   cdouble* X = reinterpret_cast<cdouble*>(Gm + gram_doubles(nl));
   {
-    const double* g = a.mats + L.mat_off + 2 * nt * nl;
-    for (int i = tid; i < nl * nl; i += kStageThreads) Gm[i] = g[i];
+    stage_copy(Gm, a.mats + L.mat_off + 2 * nt * nl, nl * nl, tid);
   }
   __syncthreads();
   // X = pre-projection terms; the Laplacian acts on G Psi (the reference's
@@ -328,6 +364,7 @@ __global__ __launch_bounds__(kStageThreads) void teuk_stage_kernel(StageArgs a) 
     for (int r = tile.t0 + row_l; r < tile.t1; r += step_l) {
       const cdouble* u = U + (r - tile.lo) * us + 2 * nl;
       double re = 0.0, im = 0.0;
+#pragma unroll 4
       for (int k = 0; k < nl; ++k) {
         re += Gm[lane_l * nl + k] * u[k].re;
         im += Gm[lane_l * nl + k] * u[k].im;
@@ -343,9 +380,11 @@ __global__ __launch_bounds__(kStageThreads) void teuk_stage_kernel(StageArgs a) 
   __syncthreads();
   if (on_l)
     for (int r = tile.t0 + row_l; r < tile.t1; r += step_l) {
+      const RkIn in = rk_load(r, lane_l);
       const cdouble* x = X + (r - tile.t0) * us;
       const double* grow = Gm + lane_l * nl;
       double pr = 0, pi = 0, qr = 0, qi = 0, sr = 0, si = 0;
+#pragma unroll 4
       for (int k = 0; k < nl; ++k) {
         const double gk = grow[k];
         pr += gk * x[k].re;
@@ -357,7 +396,7 @@ __global__ __launch_bounds__(kStageThreads) void teuk_stage_kernel(StageArgs a) 
       }
       const cdouble dpt = apply_d<SCHEME>(T, PT + lane_l, nl, tile.mlo, r, n, L.inv_h);
       const cdouble pt = PT[(r - tile.mlo) * nl + lane_l];
-      store(r, lane_l, cmake(pr, pi), cadd(dpt, cmake(qr, qi)), cadd(pt, cmake(sr, si)));
+      store(r, lane_l, in, cmake(pr, pi), cadd(dpt, cmake(qr, qi)), cadd(pt, cmake(sr, si)));
     }
   (void)orows;
 }
