@@ -163,6 +163,13 @@ int teuk_plan_tiles(const teuk_plan* plan, int* tiles, int* max_lds_bytes);
  * (eager mode only); reset by enabling. */
 int teuk_plan_set_kernel_timing(teuk_plan* plan, int enable);
 int teuk_plan_kernel_timing(const teuk_plan* plan, double* total_ms, long long* launches);
+/* Execution groups: the tiles of one radial scheme, ell-tile class (ell_tiles
+ * = 1, 2 or 4 tiles of 16 held by the MFMA routines) and LDS occupancy, one
+ * stage-kernel launch per group and RK stage.  kernel_ms: the group's timed
+ * launches since teuk_plan_set_kernel_timing(plan, 1). */
+int teuk_plan_group_count(const teuk_plan* plan, int* count);
+int teuk_plan_group_info(const teuk_plan* plan, int group, int* scheme, int* ell_tiles,
+                         int* tiles, int* lds_bytes, double* kernel_ms);
 
 /* ---------------------------------------------------------------------
  * Device buffers for callers without a HIP runtime of their own (no

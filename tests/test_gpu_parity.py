@@ -36,6 +36,10 @@ CASES = {
         spin=0.999, compactification_length=1.0, nr=41, radial_discretization="d10-5",
         dissipation=0.001, ellmax_first=32, ellmax_second=32,
         seeds=[(2, 2, 1e-3), (5, 2, -3e-4j)]),
+    # 35 coefficients per line: the Kerr kernels' widest (4 x 16) ell tiling.
+    "kerr_ell36_wide": H.small_case(spin=0.8, nr=33, radial_discretization="d4-2",
+                                    ellmax_first=36, ellmax_second=36,
+                                    seeds=[(2, 2, 1e-3), (7, 2, 4e-4 + 1e-4j)]),
     "targets_only_modes": H.small_case(
         spin=0.3, ellmax_first=4, ellmax_second=6, first_order_modes=[-2, 2],
         second_order_modes=[-4, 0, 4]),

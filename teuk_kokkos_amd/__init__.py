@@ -78,6 +78,8 @@ SIGNATURES = {
     "teuk_plan_tiles": (_I, [_P, _IP, _IP]),
     "teuk_plan_set_kernel_timing": (_I, [_P, _I]),
     "teuk_plan_kernel_timing": (_I, [_P, _DP, ctypes.POINTER(ctypes.c_longlong)]),
+    "teuk_plan_group_count": (_I, [_P, _IP]),
+    "teuk_plan_group_info": (_I, [_P, _I, _IP, _IP, _IP, _IP, _DP]),
     "teuk_device_alloc": (_I, [ctypes.c_size_t, ctypes.POINTER(_P)]),
     "teuk_device_free": (_I, [_P]),
     "teuk_memcpy": (_I, [_P, _P, ctypes.c_size_t, _I]),
@@ -341,3 +343,18 @@ class Plan:
         n = ctypes.c_longlong()
         self._check(lib().teuk_plan_kernel_timing(self._h, ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
+
+    def groups(self):
+        """Execution groups (one stage launch each): scheme, ell_tiles, tiles,
+        lds_bytes and the group's timed kernel milliseconds."""
+        n = ctypes.c_int()
+        self._check(lib().teuk_plan_group_count(self._h, ctypes.byref(n)))
+        out = []
+        for g in range(n.value):
+            v = [ctypes.c_int() for _ in range(4)]
+            ms = ctypes.c_double()
+            self._check(lib().teuk_plan_group_info(self._h, g, *[ctypes.byref(x) for x in v],
+                                                   ctypes.byref(ms)))
+            out.append({"scheme": v[0].value, "ell_tiles": v[1].value, "tiles": v[2].value,
+                        "lds_bytes": v[3].value, "kernel_ms": ms.value})
+        return out

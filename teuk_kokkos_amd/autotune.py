@@ -14,23 +14,29 @@ import os
 
 DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning_cache.json")
 
-BUDGETS = (48 * 1024, 64 * 1024, 96 * 1024, 160 * 1024)
-ROWS = (0, 16, 32, 64, 128)
+# 160 KiB of LDS per CU: three, two and one workgroups per CU.
+BUDGETS = (53 * 1024, 64 * 1024, 80 * 1024, 160 * 1024)
+ROWS = (0, 32, 64, 128)
+
+
+# Bumped whenever the stage kernels' tiling changes, invalidating cached picks.
+TUNING_VERSION = 2
 
 
 def signature(config_texts) -> str:
-    h = hashlib.sha256()
+    h = hashlib.sha256(f"v{TUNING_VERSION}".encode())
     for t in config_texts:
         h.update(t.encode())
     return h.hexdigest()[:16]
 
 
 def _time(plan, steps: int) -> float:
+    """Kernel milliseconds per RK stage (all execution groups)."""
     plan.set_kernel_timing(True)
     plan.advance(steps)
-    ms, launches = plan.kernel_timing()
+    ms, _ = plan.kernel_timing()
     plan.set_kernel_timing(False)
-    return ms / max(launches, 1)
+    return ms / (4 * max(steps, 1))
 
 
 def tune(plan, config_texts, steps: int = 3, cache_path: str = DEFAULT_CACHE, log=None):
@@ -59,12 +65,12 @@ def tune(plan, config_texts, steps: int = 3, cache_path: str = DEFAULT_CACHE, lo
             _time(plan, 1)
             ms = _time(plan, steps)
             tiles, lds = plan.tiles()
-            results.append({"lds_budget": budget, "tile_rows": rows, "ms_per_launch": ms,
+            results.append({"lds_budget": budget, "tile_rows": rows, "ms_per_stage": ms,
                             "tiles": tiles, "max_lds": lds})
             if log:
                 log(f"[autotune] budget {budget} rows {rows}: {ms:.3f} ms/launch, "
                     f"{tiles} tiles, lds {lds}")
-    best = min(results, key=lambda r: r["ms_per_launch"])
+    best = min(results, key=lambda r: r["ms_per_stage"])
     plan.set_lds_budget(best["lds_budget"])
     plan.set_tile_rows(best["tile_rows"])
     if cache_path:

@@ -24,15 +24,21 @@ DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-p
           "-munsafe-fp-atomics"]
 
 LIB_SOURCES = ["config.cpp", "host_math.cpp", "capi.cpp", "plan.hip", "stage_kernels.hip"]
+# The stage kernels, one object per (radial scheme, ell-tile class) so the
+# instantiations compile in parallel.
+STAGE_INSTANCES = [(scheme, ub) for scheme in (0, 1, 2) for ub in (1, 2, 4)]
 
 
-def _compile(src: str) -> str:
-    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+def _compile(job) -> str:
+    src, defines = job if isinstance(job, tuple) else (job, ())
+    stem = os.path.splitext(src)[0] + "".join(f"_{d.split('=')[1]}" for d in defines)
+    obj = os.path.join(BUILD, stem + ".o")
     path = os.path.join(CSRC, src)
+    flags = [f"-D{d}" for d in defines]
     if src.endswith(".hip"):
-        cmd = [HIPCC, "-x", "hip", *DEVICE, *COMMON, "-c", path, "-o", obj]
+        cmd = [HIPCC, "-x", "hip", *DEVICE, *COMMON, *flags, "-c", path, "-o", obj]
     else:
-        cmd = [HIPCC, *COMMON, "-c", path, "-o", obj]
+        cmd = [HIPCC, *COMMON, *flags, "-c", path, "-o", obj]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
@@ -42,7 +48,9 @@ def _compile(src: str) -> str:
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
-        objs = list(ex.map(_compile, LIB_SOURCES))
+        jobs = [("stage_instance.hip", (f"TEUK_SCHEME={sc}", f"TEUK_UB={ub}"))
+                for sc, ub in STAGE_INSTANCES] + LIB_SOURCES
+        objs = list(ex.map(_compile, jobs))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     res = subprocess.run(cmd, capture_output=True, text=True)

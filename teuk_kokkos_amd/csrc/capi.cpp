@@ -319,6 +319,31 @@ int teuk_plan_kernel_timing(const teuk_plan* plan, double* total_ms, long long* 
   });
 }
 
+int teuk_plan_group_count(const teuk_plan* plan, int* count) {
+  auto* p = const_cast<teuk_plan*>(plan);
+  std::string dummy;
+  return guarded(p ? p->error : dummy, [&] {
+    if (!count) throw std::invalid_argument("null output pointer");
+    *count = plan_of(p).group_count();
+  });
+}
+
+int teuk_plan_group_info(const teuk_plan* plan, int group, int* scheme, int* ell_tiles,
+                         int* tiles, int* lds_bytes, double* kernel_ms) {
+  auto* p = const_cast<teuk_plan*>(plan);
+  std::string dummy;
+  return guarded(p ? p->error : dummy, [&] {
+    int s = 0, u = 0, t = 0, l = 0;
+    double ms = 0.0;
+    plan_of(p).group_info(group, &s, &u, &t, &l, &ms);
+    if (scheme) *scheme = s;
+    if (ell_tiles) *ell_tiles = u;
+    if (tiles) *tiles = t;
+    if (lds_bytes) *lds_bytes = l;
+    if (kernel_ms) *kernel_ms = ms;
+  });
+}
+
 int teuk_device_alloc(size_t bytes, void** ptr) {
   return guarded(g_config_error, [&] {
     if (!ptr) throw std::invalid_argument("null output pointer");

@@ -27,6 +27,8 @@ struct LineDesc {
   // G = A S (n_ell x n_ell) and B = G^-1 A (n_ell x ntheta)
   int mat_off;
   int node_off;     // double offset of sin(theta_j)[ntheta] then cos(theta_j)[ntheta]
+  int frag_off;     // double offset of the MFMA operand fragments of S and A
+  int gfrag_off;    // double offset of the MFMA operand fragments of G
   double mass, spin, length, h, inv_h, gamma, eps, dt;
 };
 
@@ -49,22 +51,53 @@ struct SchemeTable {
   double diss_interior[2 * kMaxDissOrder + 1];
 };
 
-// Rows of the nodal psi_t chunk processed at once by a workgroup.
-__host__ __device__ inline int nodal_chunk_rows(int ntheta) {
-  const int r = 512 / ntheta;
-  return r < 1 ? 1 : r;
+// FP64 MFMA (v_mfma_f64_16x16x4_f64) tiling of the Kerr angular coupling.
+// Operand layouts of one wave (lane = 0..63, lr = lane % 16, lg = lane / 16),
+// measured on gfx950:
+//   A operand (16 x 4): lane holds A[lr][lg];  B operand (4 x 16): B[lg][lr];
+//   accumulator (16 x 16): element v of lane holds D[lg + 4 v][lr].
+// Synthesis (S X^T)[theta][row] uses theta tiles of 16 and ell blocks of 4:
+//   S fragment (t, q), lane: S[16 t + lr][4 q + lg].
+// The analysis sum over theta runs over blocks (t, v) = {16 t + 4 v + g :
+// g = 0..3}, so the synthesis accumulator of a lane is already the A operand
+// of the analysis (no data movement between the two products):
+//   A fragment (t, v, u), lane: A[16 u + lr][16 t + 4 v + lg].
+// Entries outside the band or the theta grid are zero.
+__host__ __device__ inline int mfma_theta_tiles(int ntheta) { return (ntheta + 15) / 16; }
+__host__ __device__ inline int mfma_ell_blocks(int n_ell) { return (n_ell + 3) / 4; }
+__host__ __device__ inline int mfma_ell_tiles(int n_ell) { return (n_ell + 15) / 16; }
+__host__ __device__ inline int mfma_s_fragments(int ntheta, int n_ell) {
+  return mfma_theta_tiles(ntheta) * mfma_ell_blocks(n_ell);
+}
+__host__ __device__ inline int mfma_fragment_doubles(int ntheta, int n_ell) {
+  return 64 * mfma_theta_tiles(ntheta) * (mfma_ell_blocks(n_ell) + 4 * mfma_ell_tiles(n_ell));
+}
+// G = A S and M = G Lambda G as B operands of k = G X + M Psi, one fragment
+// per (output ell tile uo, contraction block kb) each:
+//   fragment (uo, kb), lane: G[16 uo + lr][4 kb + lg]; see gram_output_group.
+__host__ __device__ inline int gram_fragment_doubles(int n_ell) {
+  return 2 * 64 * mfma_ell_tiles(n_ell) * mfma_ell_blocks(n_ell);
+}
+// Widest band the MFMA routines hold (4 ell tiles of 16).
+inline constexpr int kMaxBandEll = 64;
+// ell tiles of 16 a kernel instantiation holds at once: 1, 2 or 4.
+__host__ __device__ inline int band_tiles_class(int n_ell) {
+  const int lt = mfma_ell_tiles(n_ell);
+  return lt <= 1 ? 1 : (lt <= 2 ? 2 : 4);
 }
 
 __host__ __device__ inline int gram_doubles(int n_ell) {
   return (n_ell * n_ell + 1) / 2 * 2;  // padded to 16 bytes
 }
 
-__host__ __device__ inline int tile_lds_bytes(int rows, int mrows, int orows, int n_ell,
-                                              int ntheta, int constrained, int kerr, int gram) {
-  long long b = 80LL * mrows + 16LL * rows * 3 * n_ell + 16LL * mrows * n_ell;
+// LDS of one tile: per-row coefficients, the staged rows, and psi_t / D Psi
+// buffers where they cannot overwrite P in place.  The MFMA operand
+// fragments are read from global memory.
+__host__ __device__ inline int tile_lds_bytes(int rows, int mrows, int n_ell, int constrained,
+                                              int dissipate) {
+  long long b = 80LL * mrows + 16LL * rows * 3 * n_ell;
+  if (dissipate) b += 16LL * mrows * n_ell;
   if (constrained) b += 16LL * mrows * n_ell;
-  if (kerr) b += 16LL * ntheta * n_ell + 16LL * nodal_chunk_rows(ntheta) * ntheta;
-  if (gram) b += 8LL * gram_doubles(n_ell) + 16LL * orows * 3 * n_ell;
   return static_cast<int>(b);
 }
 

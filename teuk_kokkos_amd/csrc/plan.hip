@@ -12,7 +12,7 @@
 
 namespace teuk {
 
-hipError_t launch_stage(int scheme, int stage, const StageArgs& a, int n_tiles, int lds,
+hipError_t launch_stage(int scheme, int stage, int ub, const StageArgs& a, int n_tiles, int lds,
                         hipStream_t s);
 hipError_t set_stage_lds_limit(int bytes);
 hipError_t launch_nodal_to_modal(const NodalArgs& a, const cdouble* nodal, cdouble* modal,
@@ -180,8 +180,59 @@ Plan::~Plan() {
   if (stream_) hipStreamDestroy(stream_);
 }
 
+// S and A of one band in the operand order of the Kerr MFMA kernels (the
+// fragment layout is described with mfma_fragment_doubles).
+static void append_mfma_fragments(const BandTransform& bt, int nt, std::vector<double>& out) {
+  const int nl = bt.modes();
+  const int T = mfma_theta_tiles(nt), K = mfma_ell_blocks(nl), LT = mfma_ell_tiles(nl);
+  const size_t first = out.size();
+  out.resize(first + mfma_fragment_doubles(nt, nl), 0.0);
+  double* s = out.data() + first;
+  double* a = s + 64 * T * K;
+  for (int t = 0; t < T; ++t)
+    for (int lane = 0; lane < 64; ++lane) {
+      const int lr = lane % 16, lg = lane / 16;
+      for (int q = 0; q < K; ++q) {
+        const int th = 16 * t + lr, l = 4 * q + lg;
+        s[(t * K + q) * 64 + lane] = th < nt && l < nl ? bt.S[th * nl + l] : 0.0;
+      }
+      for (int v = 0; v < 4; ++v)
+        for (int u = 0; u < LT; ++u) {
+          const int l = 16 * u + lr, th = 16 * t + 4 * v + lg;
+          a[((t * 4 + v) * LT + u) * 64 + lane] = th < nt && l < nl ? bt.A[l * nt + th] : 0.0;
+        }
+    }
+  // G = A S for k = G X, then M = G Lambda G for the Laplacian term
+  // (gram_output_group): fragment (uo, kb), lane: G[16 uo + lr][4 kb + lg]
+  std::vector<double> lap(static_cast<size_t>(nl) * nl, 0.0);
+  for (int i = 0; i < nl; ++i)
+    for (int k = 0; k < nl; ++k) {
+      long double acc = 0.0L;
+      for (int j = 0; j < nl; ++j) {
+        const int ell = bt.ell_min + j;
+        acc += static_cast<long double>(bt.G[i * nl + j]) *
+               static_cast<long double>(-(ell + 2) * (ell - 1)) *
+               static_cast<long double>(bt.G[j * nl + k]);
+      }
+      lap[i * nl + k] = static_cast<double>(acc);
+    }
+  const size_t gfirst = out.size();
+  out.resize(gfirst + gram_fragment_doubles(nl), 0.0);
+  for (int which = 0; which < 2; ++which) {
+    const std::vector<double>& mat = which == 0 ? bt.G : lap;
+    double* g = out.data() + gfirst + which * (gram_fragment_doubles(nl) / 2);
+    for (int uo = 0; uo < LT; ++uo)
+      for (int kb = 0; kb < K; ++kb)
+        for (int lane = 0; lane < 64; ++lane) {
+          const int i = 16 * uo + lane % 16, k = 4 * kb + lane / 16;
+          g[(uo * K + kb) * 64 + lane] = i < nl && k < nl ? mat[i * nl + k] : 0.0;
+        }
+  }
+}
+
 void Plan::build_lines() {
-  std::map<std::tuple<int, int, int>, int> mat_index;  // (m, E, ntheta) -> offset
+  // (m, E, ntheta) -> offsets of the dense matrices and of the MFMA fragments
+  std::map<std::tuple<int, int, int>, std::pair<int, int>> mat_index;
   long long base = 0;
   for (int i = 0; i < instance_count(); ++i) {
     Instance& in = inst_[i];
@@ -194,7 +245,7 @@ void Plan::build_lines() {
       const int m = in.registry.modes()[mi];
       if (!in.registry.is_parent(m)) continue;
       const auto key = std::make_tuple(m, p.ell_max_first, in.T);
-      int off;
+      int off, frag_off;
       auto it = mat_index.find(key);
       if (it == mat_index.end()) {
         const BandTransform bt = make_band_transform(-2, m, p.ell_max_first, in.T);
@@ -205,10 +256,18 @@ void Plan::build_lines() {
         mats_.resize(mats_.size() + (gram_doubles(bt.modes()) - bt.modes() * bt.modes()), 0.0);
         const std::vector<double>& B = bt.gram ? bt.B : bt.A;
         mats_.insert(mats_.end(), B.begin(), B.end());
-        mat_index[key] = off;
+        mats_.resize((mats_.size() + 1) / 2 * 2, 0.0);  // fragments start 16-byte aligned
+        frag_off = static_cast<int>(mats_.size());
+        append_mfma_fragments(bt, in.T, mats_);
+        mat_index[key] = {off, frag_off};
       } else {
-        off = it->second;
+        off = it->second.first;
+        frag_off = it->second.second;
       }
+      const int n_ell = p.ell_max_first - minimum_ell(-2, m) + 1;
+      if (n_ell > kMaxBandEll)
+        throw std::runtime_error("bands of more than 64 ell values (ellmax_first - max(2, |m|) "
+                                 "+ 1 > 64) are not supported by the stage kernels");
       LineDesc L{};
       L.base = base;
       L.nr = in.N;
@@ -221,6 +280,8 @@ void Plan::build_lines() {
       L.dissipate = p.dissipation > 0.0 ? 1 : 0;
       L.gram = band_gram_needed(-2, m, p.ell_max_first, in.T) ? 1 : 0;
       L.mat_off = off;
+      L.frag_off = frag_off;
+      L.gfrag_off = frag_off + mfma_fragment_doubles(in.T, n_ell);
       L.node_off = node_off;
       L.mass = p.mass;
       L.spin = p.spin;
@@ -247,65 +308,81 @@ void Plan::build_tiles() {
   long long total_rows = 0;
   for (const auto& L : lines_) total_rows += L.nr;
   // Enough workgroups to fill 256 CUs several times over.
-  int parallel_rows = static_cast<int>(std::max<long long>(16, total_rows / 1024));
-  for (int scheme = 0; scheme < 3; ++scheme) {
-    const RadialScheme rs = static_cast<RadialScheme>(scheme);
-    Group g{scheme, static_cast<int>(tiles_.size()), 0, 0};
-    for (int i = 0; i < instance_count(); ++i) {
-      Instance& in = inst_[i];
-      if (static_cast<int>(in.params.scheme) != scheme) continue;
-      in.scheme_group = static_cast<int>(groups_.size());
-      in.tile_first = static_cast<int>(tiles_.size());
-      for (int mi = 0; mi < in.M; ++mi) {
-        const int li = in.line_of_mode[mi];
-        if (li < 0) continue;
-        const LineDesc& L = lines_[li];
-        const int n = L.nr;
-        auto ranges = [&](int t0, int t1, TileDesc& t) {
-          t.t0 = t0;
-          t.t1 = t1;
-          t.mlo = derivative_columns(rs, n, t0).first;
-          t.mhi = derivative_columns(rs, n, t1 - 1).second;
-          int lo = t.mlo, hi = t.mhi;
-          if (L.dissipate) {
-            lo = std::min(lo, dissipation_columns(rs, n, t0).first);
-            hi = std::max(hi, dissipation_columns(rs, n, t1 - 1).second);
-          }
-          if (L.constrained) {
-            lo = std::min(lo, derivative_columns(rs, n, t.mlo).first);
-            hi = std::max(hi, derivative_columns(rs, n, t.mhi - 1).second);
-          }
-          t.lo = lo;
-          t.hi = hi;
-          t.lds_bytes = tile_lds_bytes(hi - lo, t.mhi - t.mlo, t1 - t0, L.n_ell, L.ntheta,
-                                       L.constrained, L.kerr, L.gram);
-        };
-        // Largest row count within the LDS budget, capped for parallelism.
-        int rows = tile_rows_ > 0 ? tile_rows_ : std::min(256, parallel_rows);
-        rows = std::max(1, std::min(rows, n));
-        while (rows > 1) {
-          TileDesc probe{};
-          const int a = n / 2 - rows / 2;
-          ranges(a, a + rows, probe);
-          if (probe.lds_bytes <= lds_budget_) break;
-          rows = rows > 16 ? rows - 8 : rows - 1;
+  const int parallel_rows = static_cast<int>(std::max<long long>(16, total_rows / 1024));
+  std::vector<std::vector<TileDesc>> per(instance_count());
+  std::vector<int> inst_lds(instance_count(), 0), inst_ub(instance_count(), 1);
+  for (int i = 0; i < instance_count(); ++i) {
+    Instance& in = inst_[i];
+    const RadialScheme rs = in.params.scheme;
+    for (int mi = 0; mi < in.M; ++mi) {
+      const int li = in.line_of_mode[mi];
+      if (li < 0) continue;
+      const LineDesc& L = lines_[li];
+      const int n = L.nr;
+      inst_ub[i] = std::max(inst_ub[i], band_tiles_class(L.n_ell));
+      auto ranges = [&](int t0, int t1, TileDesc& t) {
+        t.t0 = t0;
+        t.t1 = t1;
+        t.mlo = derivative_columns(rs, n, t0).first;
+        t.mhi = derivative_columns(rs, n, t1 - 1).second;
+        int lo = t.mlo, hi = t.mhi;
+        if (L.dissipate) {
+          lo = std::min(lo, dissipation_columns(rs, n, t0).first);
+          hi = std::max(hi, dissipation_columns(rs, n, t1 - 1).second);
         }
-        const int nt = (n + rows - 1) / rows;
-        const int basec = n / nt, rem = n % nt;
-        int t0 = 0;
-        for (int k = 0; k < nt; ++k) {
+        if (L.constrained) {
+          lo = std::min(lo, derivative_columns(rs, n, t.mlo).first);
+          hi = std::max(hi, derivative_columns(rs, n, t.mhi - 1).second);
+        }
+        t.lo = lo;
+        t.hi = hi;
+        t.lds_bytes = tile_lds_bytes(hi - lo, t.mhi - t.mlo, L.n_ell, L.constrained,
+                                     L.dissipate);
+      };
+      // Largest row count whose tiles (boundary-closure tiles included) all
+      // fit the LDS budget, capped for parallelism.
+      int rows = tile_rows_ > 0 ? tile_rows_ : std::min(256, parallel_rows);
+      rows = std::max(1, std::min(rows, n));
+      std::vector<TileDesc> line_tiles;
+      for (;;) {
+        line_tiles.clear();
+        const int count = (n + rows - 1) / rows;
+        const int basec = n / count, rem = n % count;
+        int t0 = 0, worst = 0;
+        for (int k = 0; k < count; ++k) {
           const int len = basec + (k < rem ? 1 : 0);
           TileDesc t{};
           t.line = li;
           ranges(t0, t0 + len, t);
-          max_lds_ = std::max(max_lds_, t.lds_bytes);
-          g.lds = std::max(g.lds, t.lds_bytes);
-          tiles_.push_back(t);
+          worst = std::max(worst, t.lds_bytes);
+          line_tiles.push_back(t);
           t0 += len;
         }
+        if (worst <= lds_budget_ || rows == 1) break;
+        rows = rows > 16 ? rows - 4 : rows - 1;
       }
-      in.tile_count = static_cast<int>(tiles_.size()) - in.tile_first;
+      for (const TileDesc& t : line_tiles) {
+        inst_lds[i] = std::max(inst_lds[i], t.lds_bytes);
+        per[i].push_back(t);
+      }
     }
+  }
+  // One launch per (scheme, ell-tile class); a launch reserves its largest
+  // tile's LDS (at most the budget) for every workgroup.
+  std::map<std::tuple<int, int, int>, std::vector<int>> buckets;
+  for (int i = 0; i < instance_count(); ++i)
+    buckets[{static_cast<int>(inst_[i].params.scheme), inst_ub[i], 0}].push_back(i);
+  for (const auto& [key, members] : buckets) {
+    Group g{std::get<0>(key), std::get<1>(key), static_cast<int>(tiles_.size()), 0, 0};
+    for (int i : members) {
+      Instance& in = inst_[i];
+      in.scheme_group = static_cast<int>(groups_.size());
+      in.tile_first = static_cast<int>(tiles_.size());
+      in.tile_count = static_cast<int>(per[i].size());
+      tiles_.insert(tiles_.end(), per[i].begin(), per[i].end());
+      g.lds = std::max(g.lds, inst_lds[i]);
+    }
+    max_lds_ = std::max(max_lds_, g.lds);
     g.count = static_cast<int>(tiles_.size()) - g.first;
     groups_.push_back(g);
   }
@@ -586,9 +663,11 @@ void Plan::launch_step(double dt_override) {
         }
         check(hipEventRecord(events_[2 * timing_launches_], stream_), "event");
       }
-      check(teuk::launch_stage(g.scheme, s, a, g.count, g.lds, stream_), "stage kernel");
+      check(teuk::launch_stage(g.scheme, s, g.ub, a, g.count, g.lds, stream_), "stage kernel");
       if (timed) {
         check(hipEventRecord(events_[2 * timing_launches_ + 1], stream_), "event");
+        event_group_.resize(timing_launches_ + 1);
+        event_group_[timing_launches_] = static_cast<int>(&g - groups_.data());
         ++timing_launches_;
       }
     }
@@ -665,7 +744,7 @@ void Plan::evaluate_rhs(int i, const cplx* d_stage, cplx* d_out, double t) {
   a.table = d_tables_ + g.scheme;
   a.yin = d_ya_;
   a.kout = d_k_;
-  check(teuk::launch_stage(g.scheme, 0, a, in.tile_count, g.lds, stream_), "rhs kernel");
+  check(teuk::launch_stage(g.scheme, 0, g.ub, a, in.tile_count, g.lds, stream_), "rhs kernel");
   check(launch_fill_zero(reinterpret_cast<cdouble*>(d_out), nodal_value_count(i), stream_), "zero");
   check(launch_modal_to_nodal(na, d_k_, reinterpret_cast<cdouble*>(d_out), stream_), "synthesis");
 }
@@ -760,6 +839,25 @@ void Plan::kernel_timing(double* total_ms, long long* launches) {
   }
   *total_ms = ms;
   *launches = timing_launches_;
+}
+
+void Plan::group_info(int g, int* scheme, int* ub, int* tiles, int* lds_bytes,
+                      double* kernel_ms) {
+  if (g < 0 || g >= group_count()) throw std::invalid_argument("group index out of range");
+  const Group& gr = groups_[g];
+  *scheme = gr.scheme;
+  *ub = gr.ub;
+  *tiles = gr.count;
+  *lds_bytes = gr.lds;
+  check(hipStreamSynchronize(stream_), "timing sync");
+  double ms = 0.0;
+  for (long long k = 0; k < timing_launches_; ++k) {
+    if (event_group_[k] != g) continue;
+    float e = 0.f;
+    check(hipEventElapsedTime(&e, events_[2 * k], events_[2 * k + 1]), "elapsed");
+    ms += e;
+  }
+  *kernel_ms = ms;
 }
 
 void Plan::set_lds_budget(int bytes) {

@@ -81,6 +81,10 @@ class Plan {
   void set_graph(bool enable);
   void set_kernel_timing(bool enable);
   void kernel_timing(double* total_ms, long long* launches);
+  // Execution groups: one stage launch each (tiles of one radial scheme, ell
+  // tile class and occupancy); kernel_ms sums the group's timed launches.
+  int group_count() const { return static_cast<int>(groups_.size()); }
+  void group_info(int g, int* scheme, int* ub, int* tiles, int* lds_bytes, double* kernel_ms);
   void set_lds_budget(int bytes);   // re-tiles
   void set_tile_rows(int rows);     // 0 = automatic
   SourceActivation activation(int i) const { return inst_.at(i).activation; }
@@ -99,12 +103,12 @@ class Plan {
   std::vector<double> mats_, nodes_;
   std::vector<TileDesc> tiles_;
   struct Group {
-    int scheme, first, count, lds;
+    int scheme, ub, first, count, lds;  // ub: ell tiles of 16 per MFMA pass
   };
   std::vector<Group> groups_;
   long long modal_count_ = 0;
   int max_lds_ = 0;
-  int lds_budget_ = 64 * 1024;
+  int lds_budget_ = 80 * 1024;  // two workgroups per CU (160 KiB LDS)
   int tile_rows_ = 0;
 
   hipStream_t stream_ = nullptr;
@@ -121,6 +125,7 @@ class Plan {
   hipGraphExec_t graph_exec_ = nullptr;
   bool timing_ = false;
   std::vector<hipEvent_t> events_;
+  std::vector<int> event_group_;
   long long timing_launches_ = 0;
 };
 

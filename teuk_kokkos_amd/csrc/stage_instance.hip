@@ -1,0 +1,44 @@
+// One (radial scheme, ell-tile class) instantiation of the stage kernels:
+// compiled once per pair with -DTEUK_SCHEME=<0|1|2> -DTEUK_UB=<1|2|4>.
+#include "stage_kernels.hpp"
+
+#if !defined(TEUK_SCHEME) || !defined(TEUK_UB)
+#error "stage_instance.hip needs -DTEUK_SCHEME and -DTEUK_UB"
+#endif
+
+#define TEUK_CAT5(a, b, c, d, e) a##b##c##d##e
+#define TEUK_NAME(prefix, s, u) TEUK_CAT5(prefix, s, _, u, )
+
+namespace teuk {
+
+hipError_t TEUK_NAME(launch_stage_instance_, TEUK_SCHEME, TEUK_UB)(int stage, const StageArgs& a,
+                                                                  int n_tiles, int lds,
+                                                                  hipStream_t s) {
+  constexpr int S = TEUK_SCHEME, U = TEUK_UB;
+  const dim3 grid(n_tiles), block(kStageThreads);
+  switch (stage) {
+    case 0: hipLaunchKernelGGL((teuk_stage_kernel<S, 0, U>), grid, block, lds, s, a); break;
+    case 1: hipLaunchKernelGGL((teuk_stage_kernel<S, 1, U>), grid, block, lds, s, a); break;
+    case 2: hipLaunchKernelGGL((teuk_stage_kernel<S, 2, U>), grid, block, lds, s, a); break;
+    case 3: hipLaunchKernelGGL((teuk_stage_kernel<S, 3, U>), grid, block, lds, s, a); break;
+    default: hipLaunchKernelGGL((teuk_stage_kernel<S, 4, U>), grid, block, lds, s, a); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t TEUK_NAME(set_lds_limit_instance_, TEUK_SCHEME, TEUK_UB)(int bytes) {
+  constexpr int S = TEUK_SCHEME, U = TEUK_UB;
+  const void* fns[] = {reinterpret_cast<const void*>(&teuk_stage_kernel<S, 0, U>),
+                       reinterpret_cast<const void*>(&teuk_stage_kernel<S, 1, U>),
+                       reinterpret_cast<const void*>(&teuk_stage_kernel<S, 2, U>),
+                       reinterpret_cast<const void*>(&teuk_stage_kernel<S, 3, U>),
+                       reinterpret_cast<const void*>(&teuk_stage_kernel<S, 4, U>)};
+  for (const void* f : fns) {
+    const hipError_t e =
+        hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+}  // namespace teuk

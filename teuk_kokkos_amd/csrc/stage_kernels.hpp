@@ -1,0 +1,565 @@
+// Fused per-RK-stage Teukolsky kernels for gfx950.
+//
+// One workgroup owns one radial tile of one (instance, m) line and does the
+// whole linear stage for it:
+//   1. stage the tile (+ stencil halo) of (P, Q, Psi) modal coefficients
+//      into LDS with coalesced 16-byte loads;
+//   2. psi_t = A [(S(P + 2K Q_eff) - G_m S Psi) / C_T] on the halo rows
+//      (the theta-dependent Kerr coupling as a small synthesis / pointwise /
+//      analysis contraction in LDS; diagonal when a = 0);
+//   3. SBP radial derivatives D Q_eff, D Psi, D psi_t and the compatible
+//      dissipation from LDS, the point RHS
+//        P_t = H_R D Q_eff + q Q_eff + psi Psi + Lambda_l Psi
+//        Q_t = D psi_t - gamma (Q - D Psi)
+//        Psi_t = psi_t
+//      (reference: full_spatial.hpp:99-280, teukolsky.hpp:117-146), and
+//   4. the classical RK4 stage combination written straight to HBM.
+// The band projection of every stage derivative (spatial_pipeline.hpp:
+// 499-500, 756-766) is implicit: the state lives in the retained band.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "device_layout.hpp"
+
+namespace teuk {
+
+namespace {
+
+// h*D applied at global row `row` to values v(g) = base[(g - lo) * stride].
+template <int SCHEME>
+__device__ __forceinline__ cdouble apply_d(const SchemeTable& T, const cdouble* base, int stride,
+                                           int lo, int row, int n, double inv_h) {
+  constexpr int W = SCHEME == 2 ? 11 : (SCHEME == 1 ? 8 : 4);
+  constexpr int C = SCHEME == 2 ? 16 : (SCHEME == 1 ? 12 : 6);
+  constexpr int B = SCHEME == 2 ? 5 : (SCHEME == 1 ? 4 : 2);
+  double re = 0.0, im = 0.0;
+  if (row < W) {
+    const double* c = &T.left[row * kMaxClosureWidth];
+    const cdouble* v = base + (0 - lo) * stride;
+    // closure rows are rare: keep them compact to save registers
+#pragma unroll 2
+    for (int k = 0; k < C; ++k) {
+      const cdouble x = v[k * stride];
+      re += c[k] * x.re;
+      im += c[k] * x.im;
+    }
+  } else if (row + W >= n) {
+    const double* c = &T.left[(n - 1 - row) * kMaxClosureWidth];
+    const cdouble* v = base + (n - 1 - lo) * stride;
+#pragma unroll 2
+    for (int k = 0; k < C; ++k) {
+      const cdouble x = v[-k * stride];
+      re -= c[k] * x.re;
+      im -= c[k] * x.im;
+    }
+  } else {
+    const cdouble* v = base + (row - lo) * stride;
+#pragma unroll
+    for (int k = B; k >= 1; --k) {
+      const cdouble a = v[k * stride];
+      const cdouble b = v[-k * stride];
+      re += T.interior[k] * (a.re - b.re);
+      im += T.interior[k] * (a.im - b.im);
+    }
+  }
+  return cmake(inv_h * re, inv_h * im);
+}
+
+// -(eps/h) H^-1 A^T A at global row `row` (radial_discretization.hpp:
+// 165-232), with the A^T A rows tabulated on the host.
+template <int SCHEME>
+__device__ __forceinline__ cdouble apply_diss(const SchemeTable& T, const cdouble* base,
+                                              int stride, int lo, int row, int n, double h,
+                                              double eps) {
+  constexpr int O = SCHEME == 2 ? 6 : (SCHEME == 1 ? 5 : 3);
+  constexpr int W = SCHEME == 2 ? 11 : (SCHEME == 1 ? 8 : 4);
+  double re = 0.0, im = 0.0;
+  if (row < O) {
+    const double* c = T.diss_left[row];
+    const cdouble* v = base + (0 - lo) * stride;
+    for (int k = 0; k <= row + O; ++k) {
+      const cdouble x = v[k * stride];
+      re += c[k] * x.re;
+      im += c[k] * x.im;
+    }
+  } else if (row > n - 1 - O) {
+    const int d = n - 1 - row;
+    const double* c = T.diss_left[d];
+    const cdouble* v = base + (n - 1 - lo) * stride;
+    for (int k = 0; k <= d + O; ++k) {
+      const cdouble x = v[-k * stride];
+      re += c[k] * x.re;
+      im += c[k] * x.im;
+    }
+  } else {
+    const cdouble* v = base + (row - lo) * stride;
+#pragma unroll
+    for (int k = -O; k <= O; ++k) {
+      const cdouble x = v[k * stride];
+      re += T.diss_interior[k + O] * x.re;
+      im += T.diss_interior[k + O] * x.im;
+    }
+  }
+  const int dist = row < n - 1 - row ? row : n - 1 - row;
+  const double hw = dist < W ? T.norm[dist] : 1.0;
+  const double f = -eps / (h * hw);
+  return cmake(f * re, f * im);
+}
+
+// Global -> LDS copy with kCopyBatch loads in flight per thread before the
+// first LDS write (a plain loop would wait for every load in turn).
+constexpr int kCopyBatch = 8;
+template <class V>
+__device__ __forceinline__ void stage_copy(V* __restrict__ dst, const V* __restrict__ src,
+                                           int count, int tid) {
+  for (int base = tid; base < count; base += kCopyBatch * kStageThreads) {
+    V tmp[kCopyBatch];
+    // Clamped indices keep every load and store unconditional, so all loads
+    // are in flight before the first wait (out-of-range lanes rewrite the
+    // last element with its own value).
+#pragma unroll
+    for (int k = 0; k < kCopyBatch; ++k) tmp[k] = src[min(base + k * kStageThreads, count - 1)];
+#pragma unroll
+    for (int k = 0; k < kCopyBatch; ++k) dst[min(base + k * kStageThreads, count - 1)] = tmp[k];
+  }
+}
+
+}  // namespace
+
+// Per-row radial coefficients, tabulated once per tile row in LDS.
+struct RowCoeffs {
+  double adv2;         // 2 K
+  double principal;    // H_R
+  double time_radial;  // theta-independent part of C_T
+  double g_real, g_imag_m;
+  double q_re, q_im, psi_re, psi_im;
+  double pad;
+};
+static_assert(sizeof(RowCoeffs) == 80, "RowCoeffs must stay 16-byte aligned");
+
+typedef double mfma_acc __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ mfma_acc mfma_f64(double a, double b, mfma_acc c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// Kerr coupling psi_t = A[(S X - G_m S Psi) / C_T] of the 16 rows
+// [g0, g0 + 16) (tile-relative, clipped to mrows) by one wave, with
+// X = P + 2K Q_eff read from PT (row stride pts) and overwritten there by
+// psi_t:
+//   synthesis   (S X^T)[theta][row]     16 x 16 tiles, FP64 MFMA
+//   pointwise   ((S X) - G_m (S Psi)) / C_T on the accumulator registers
+//   analysis    psi[row][ell] += N[row][theta] A^T[theta][ell]
+// The accumulator of the synthesis is the A operand of the analysis, so the
+// nodal values never leave registers.  The S and A fragments come from
+// global memory (cache resident, shared by every tile of the band), one
+// theta tile ahead.  UB = ell tiles of 16 held at once (K <= 4 UB blocks).
+template <int UB>
+__device__ __forceinline__ void kerr_psi_group(int nl, int nt, double spin, const RowCoeffs* RC,
+                                               cdouble* PT, int pts, const cdouble* psi, int us,
+                                               int g0, int mrows, const double* Sf,
+                                               const double* Af, const double* sinv,
+                                               const double* cosv, int lane) {
+  const int T = mfma_theta_tiles(nt), K = mfma_ell_blocks(nl), LT = mfma_ell_tiles(nl);
+  const int lr = lane & 15, lg = lane >> 4;
+  const int jj = g0 + lr;
+  const bool row_ok = jj < mrows;
+  const int jc = row_ok ? jj : mrows - 1;
+  const cdouble* x = PT + jc * pts;
+  const cdouble* ps = psi + jc * us;
+  const double tau = RC[jc].time_radial, gre = RC[jc].g_real, gim = RC[jc].g_imag_m;
+  const double a2 = spin * spin, c4 = -4.0 * spin;
+  mfma_acc ar[UB], ai[UB];
+#pragma unroll
+  for (int u = 0; u < UB; ++u) ar[u] = ai[u] = mfma_acc{0.0, 0.0, 0.0, 0.0};
+  // S fragment (t, q) sits at flat index t K + q: three in flight ahead.
+  const int nf = T * K;
+  const double* sf = Sf + lane;
+  double s0 = sf[0], s1 = nf > 1 ? sf[64] : 0.0, s2 = nf > 2 ? sf[128] : 0.0;
+  int f = 0;
+  for (int t = 0; t < T; ++t) {
+    // this tile's A fragments arrive during the synthesis
+    double afr[4][UB];
+    const double* af = Af + t * 4 * LT * 64 + lane;
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+      for (int u = 0; u < UB; ++u) afr[v][u] = u < LT ? af[(v * LT + u) * 64] : 0.0;
+    mfma_acc sxr = {0.0, 0.0, 0.0, 0.0}, sxi = sxr, spr = sxr, spi = sxr;
+#pragma unroll 2
+    for (int q = 0; q < K; ++q, ++f) {
+      const double s = s0;
+      s0 = s1;
+      s1 = s2;
+      s2 = f + 3 < nf ? sf[(f + 3) * 64] : 0.0;
+      // B operands X, Psi of (row jj, ell 4 q + lg) from LDS
+      const int l = 4 * q + lg;
+      const bool ok = row_ok && l < nl;
+      const int lc = ok ? l : 0;
+      const cdouble xv = x[lc], pv = ps[lc];
+      sxr = mfma_f64(s, ok ? xv.re : 0.0, sxr);
+      sxi = mfma_f64(s, ok ? xv.im : 0.0, sxi);
+      spr = mfma_f64(s, ok ? pv.re : 0.0, spr);
+      spi = mfma_f64(s, ok ? pv.im : 0.0, spi);
+    }
+    // element v of this lane: theta = 16 t + lg + 4 v, radial row jj
+    double nre[4], nim[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int th = 16 * t + lg + 4 * v;
+      const bool tok = th < nt;
+      const int tc = tok ? th : 0;
+      const double sn = sinv[tc], cs = cosv[tc];
+      const double ct = tau - a2 * sn * sn;
+      const double gi = gim + c4 * cs;
+      const double pr = spr[v], pi = spi[v];
+      const double num_re = sxr[v] - (gre * pr - gi * pi);
+      const double num_im = sxi[v] - (gre * pi + gi * pr);
+      nre[v] = tok ? num_re / ct : 0.0;
+      nim[v] = tok ? num_im / ct : 0.0;
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+      for (int u = 0; u < UB; ++u)
+        if (u < LT) {
+          ar[u] = mfma_f64(nre[v], afr[v][u], ar[u]);
+          ai[u] = mfma_f64(nim[v], afr[v][u], ai[u]);
+        }
+  }
+  // element w of this lane: psi_t at row g0 + lg + 4 w, ell 16 u + lr.  The
+  // wave's LDS reads of X were issued before these writes, and no other wave
+  // touches rows of this group, so X can be overwritten in place.
+#pragma unroll
+  for (int u = 0; u < UB; ++u)
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      const int row = g0 + lg + 4 * w, l = 16 * u + lr;
+      if (u < LT && row < mrows && l < nl) PT[row * pts + l] = cmake(ar[u][w], ai[u][w]);
+    }
+}
+
+// Classical RK4 stage combination of one value (STAGE 0: k -> kout).  The
+// accumulator carries y, so a step moves 16 state vectors through HBM:
+//   S1: acc = y + dt/6 k, Ya = y + dt/2 k      S2/S3: acc += dt/3 k,
+//   Y = y + c dt k                             S4: y = acc + dt/6 k.
+struct RkIn {
+  cdouble y0, acc0;
+};
+
+// need_y1: stage 1 also reads its input from HBM (the staged copy of P was
+// overwritten by psi_t).
+template <int STAGE>
+__device__ __forceinline__ RkIn rk_load(const StageArgs& a, long long gi, bool need_y1 = false) {
+  RkIn v{};
+  if constexpr (STAGE == 1) {
+    if (need_y1) v.y0 = a.yin[gi];
+  } else if constexpr (STAGE == 2 || STAGE == 3) {
+    v.y0 = a.y[gi];
+    v.acc0 = a.acc[gi];
+  } else if constexpr (STAGE == 4) {
+    v.acc0 = a.acc[gi];
+  }
+  return v;
+}
+
+// y1: the stage-1 input (= y).
+template <int STAGE>
+__device__ __forceinline__ void rk_store(const StageArgs& a, long long gi, const RkIn& in,
+                                         cdouble y1, cdouble k, double dt) {
+  if constexpr (STAGE == 0) {
+    a.kout[gi] = k;
+  } else if constexpr (STAGE == 1) {
+    a.acc[gi] = cadd(y1, cscale(dt / 6.0, k));
+    a.yout[gi] = cadd(y1, cscale(0.5 * dt, k));
+  } else if constexpr (STAGE == 2 || STAGE == 3) {
+    a.acc[gi] = cadd(in.acc0, cscale(dt / 3.0, k));
+    a.yout[gi] = cadd(in.y0, cscale(STAGE == 2 ? 0.5 * dt : dt, k));
+  } else {
+    a.yout[gi] = cadd(in.acc0, cscale(dt / 6.0, k));
+  }
+}
+
+// Per-tile context shared by the output routines.
+struct TileCtx {
+  const LineDesc* L;
+  const TileDesc* tile;
+  const SchemeTable* T;
+  const RowCoeffs* RC;
+  const cdouble* U;   // staged rows [lo, hi): P | Q | Psi per row
+  const cdouble* PT;  // psi_t on rows [mlo, mhi), row stride pts
+  const cdouble* DP;  // D Psi on rows [mlo, mhi) (stage-constrained only)
+  int pts;
+  bool inplace;       // psi_t overwrote P in U (P is then read from HBM)
+  double dt, gamma;
+};
+
+// Pre-projection (theta-independent, linear) terms of the RHS at (r, l):
+// the reference forms them on the nodal stage S c and projects with S A,
+// i.e. they carry G = A S in modal coordinates (full_spatial.hpp:99-280).
+//   P: H_R D Q_eff + q Q_eff + psi Psi [+ Lambda_l lap_in] (+ dissipation)
+//   Q: -gamma (Q - D Psi) (+ dissipation)        Psi: (dissipation)
+template <int SCHEME, int FIELD, bool WITH_LAP = true>
+__device__ __forceinline__ cdouble pre_projection(const TileCtx& c, int r, int l, cdouble lap_in) {
+  const LineDesc& L = *c.L;
+  const TileDesc& tile = *c.tile;
+  const int nl = L.n_ell, us = 3 * nl, n = L.nr;
+  const cdouble* u = c.U + (r - tile.lo) * us;
+  cdouble x;
+  if constexpr (FIELD == 0) {
+    cdouble qeff, dq;
+    if (L.constrained) {
+      qeff = c.DP[(r - tile.mlo) * nl + l];
+      dq = apply_d<SCHEME>(*c.T, c.DP + l, nl, tile.mlo, r, n, L.inv_h);
+    } else {
+      qeff = u[nl + l];
+      dq = apply_d<SCHEME>(*c.T, c.U + nl + l, us, tile.lo, r, n, L.inv_h);
+    }
+    const RowCoeffs& rc = c.RC[r - tile.mlo];
+    const double lam = -static_cast<double>((L.ell_min + l + 2) * (L.ell_min + l - 1));
+    const cdouble radial = cadd(cscale(rc.principal, dq), cmul(cmake(rc.q_re, rc.q_im), qeff));
+    const cdouble coupling = cmul(cmake(rc.psi_re, rc.psi_im), u[2 * nl + l]);
+    if constexpr (WITH_LAP)
+      x = cadd(radial, cadd(coupling, cscale(lam, lap_in)));
+    else
+      x = cadd(radial, coupling);
+  } else if constexpr (FIELD == 1) {
+    const cdouble dpsi = apply_d<SCHEME>(*c.T, c.U + 2 * nl + l, us, tile.lo, r, n, L.inv_h);
+    x = cscale(-c.gamma, csub(u[nl + l], dpsi));
+  } else {
+    x = cmake(0.0, 0.0);
+  }
+  if (L.dissipate)
+    x = cadd(x, apply_diss<SCHEME>(*c.T, c.U + FIELD * nl + l, us, tile.lo, r, n, L.h, L.eps));
+  return x;
+}
+
+// Output rows of the tile, one (row, ell) point per thread, for bands whose
+// A S is the identity to roundoff.
+template <int SCHEME, int STAGE>
+__device__ __forceinline__ void plain_output(const StageArgs& a, const TileCtx& c, int tid) {
+  const LineDesc& L = *c.L;
+  const TileDesc& tile = *c.tile;
+  const int nl = L.n_ell, us = 3 * nl, n = L.nr;
+  const int lane_l = tid % nl, row_l = tid / nl, step_l = kStageThreads / nl;
+  if (row_l >= step_l) return;
+  // RK inputs requested one row ahead of their use
+  RkIn in[3], nxt[3];
+  auto fetch = [&](RkIn (&v)[3], int r) {
+    const long long g = L.base + static_cast<long long>(r) * us + lane_l;
+#pragma unroll
+    for (int f = 0; f < 3; ++f) v[f] = rk_load<STAGE>(a, g + f * nl, f == 0 && c.inplace);
+  };
+  if (tile.t0 + row_l < tile.t1) fetch(in, tile.t0 + row_l);
+  for (int r = tile.t0 + row_l; r < tile.t1; r += step_l) {
+    const long long g = L.base + static_cast<long long>(r) * us + lane_l;
+    if (r + step_l < tile.t1) fetch(nxt, r + step_l);
+    const cdouble* u = c.U + (r - tile.lo) * us;
+    const cdouble kp = pre_projection<SCHEME, 0>(c, r, lane_l, u[2 * nl + lane_l]);
+    const cdouble xq = pre_projection<SCHEME, 1>(c, r, lane_l, u[2 * nl + lane_l]);
+    const cdouble xs = pre_projection<SCHEME, 2>(c, r, lane_l, u[2 * nl + lane_l]);
+    const cdouble dpt = apply_d<SCHEME>(*c.T, c.PT + lane_l, c.pts, tile.mlo, r, n, L.inv_h);
+    const cdouble pt = c.PT[(r - tile.mlo) * c.pts + lane_l];
+    rk_store<STAGE>(a, g, in[0], c.inplace ? in[0].y0 : u[lane_l], kp, c.dt);
+    rk_store<STAGE>(a, g + nl, in[1], u[nl + lane_l], cadd(dpt, xq), c.dt);
+    rk_store<STAGE>(a, g + 2 * nl, in[2], u[2 * nl + lane_l], cadd(pt, xs), c.dt);
+#pragma unroll
+    for (int f = 0; f < 3; ++f) in[f] = nxt[f];
+  }
+}
+
+// Output rows [g0, g0 + 16) of the tile (relative to t0) by one wave for
+// bands that need G = A S: k = G X (+ M Psi for P, M = G Lambda G: the
+// reference's S Lambda A acting on the nodal stage) with FP64 MFMA,
+//   k[row][ell] = sum_k X[row][k] G[ell][k]:
+//   A operand X      lane: row g0 + lr, k = 4 kb + lg (pre-projection terms
+//                    straight from the stencils),
+//   B operand G^T    fragment (uo, kb), lane: G[16 uo + lr][4 kb + lg],
+//   accumulator      element w: row g0 + lg + 4 w, ell 16 uo + lr,
+// so each RK load or store instruction covers 16 consecutive ell of four
+// rows.  UB = ell tiles of 16 accumulated at once.
+template <int SCHEME, int STAGE, int UB>
+__device__ __forceinline__ void gram_output_group(const StageArgs& a, const TileCtx& c,
+                                                  const double* Gf, int g0, int lane) {
+  const LineDesc& L = *c.L;
+  const TileDesc& tile = *c.tile;
+  const int nl = L.n_ell, us = 3 * nl, n = L.nr;
+  const int LT = mfma_ell_tiles(nl), K = mfma_ell_blocks(nl);
+  const int lr = lane & 15, lg = lane >> 4;
+  const int orows = tile.t1 - tile.t0;
+  // operand row of this lane
+  const int r_in = tile.t0 + min(g0 + lr, orows - 1);
+  const cdouble* u_in = c.U + (r_in - tile.lo) * us;
+  const double* Mf = Gf + gram_fragment_doubles(nl) / 2;
+  const cdouble zero = cmake(0.0, 0.0);
+  // accumulator position (uo, w): output row and ell, clamped for addresses
+  auto out_row = [&](int w) { return tile.t0 + min(g0 + lg + 4 * w, orows - 1); };
+  auto out_ok = [&](int uo, int w) { return g0 + lg + 4 * w < orows && 16 * uo + lr < nl; };
+  auto out_index = [&](int F, int uo, int w) {
+    return L.base + static_cast<long long>(out_row(w)) * us + F * nl + min(16 * uo + lr, nl - 1);
+  };
+
+  auto field = [&](auto field_tag) {
+    constexpr int F = decltype(field_tag)::value;
+    const bool need_y1 = F == 0 && c.inplace;
+    constexpr int NR = UB <= 2 ? UB : 1;  // ell tiles of RK inputs requested early
+    RkIn in[NR][4];
+    if constexpr (UB <= 2) {
+#pragma unroll
+      for (int uo = 0; uo < UB; ++uo)
+#pragma unroll
+        for (int w = 0; w < 4; ++w)
+          if (uo < LT) in[uo][w] = rk_load<STAGE>(a, out_index(F, uo, w), need_y1);
+    }
+    mfma_acc ar[UB], ai[UB];
+#pragma unroll
+    for (int uo = 0; uo < UB; ++uo) ar[uo] = ai[uo] = mfma_acc{0.0, 0.0, 0.0, 0.0};
+    // without dissipation the Psi pre-projection terms vanish
+    if (F != 2 || L.dissipate) {
+#pragma unroll 2
+      for (int kb = 0; kb < K; ++kb) {
+        const int l = 4 * kb + lg;
+        cdouble x = zero, ps = zero;
+        if (l < nl) {
+          x = pre_projection<SCHEME, F, false>(c, r_in, l, zero);
+          if constexpr (F == 0) ps = u_in[2 * nl + l];
+        }
+#pragma unroll
+        for (int uo = 0; uo < UB; ++uo) {
+          if (uo >= LT) break;
+          const double g = Gf[(uo * K + kb) * 64 + lane];
+          ar[uo] = mfma_f64(x.re, g, ar[uo]);
+          ai[uo] = mfma_f64(x.im, g, ai[uo]);
+          if constexpr (F == 0) {
+            const double m = Mf[(uo * K + kb) * 64 + lane];
+            ar[uo] = mfma_f64(ps.re, m, ar[uo]);
+            ai[uo] = mfma_f64(ps.im, m, ai[uo]);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int uo = 0; uo < UB; ++uo) {
+      if (uo >= LT) break;
+      const int slot = UB <= 2 ? uo : 0;
+      if constexpr (UB > 2) {
+#pragma unroll
+        for (int w = 0; w < 4; ++w) in[0][w] = rk_load<STAGE>(a, out_index(F, uo, w), need_y1);
+      }
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        if (!out_ok(uo, w)) continue;
+        const int r = out_row(w), l = 16 * uo + lr;
+        cdouble k = cmake(ar[uo][w], ai[uo][w]);
+        if constexpr (F == 1)
+          k = cadd(k, apply_d<SCHEME>(*c.T, c.PT + l, c.pts, tile.mlo, r, n, L.inv_h));
+        if constexpr (F == 2) k = cadd(k, c.PT[(r - tile.mlo) * c.pts + l]);
+        const cdouble y1 = need_y1 ? in[slot][w].y0 : c.U[(r - tile.lo) * us + F * nl + l];
+        rk_store<STAGE>(a, out_index(F, uo, w), in[slot][w], y1, k, c.dt);
+      }
+    }
+  };
+  field(std::integral_constant<int, 0>{});
+  field(std::integral_constant<int, 1>{});
+  field(std::integral_constant<int, 2>{});
+}
+
+// One workgroup = one radial tile of one line; UB = ell tiles of 16 the MFMA
+// routines hold at once (1, 2 or 4).
+template <int SCHEME, int STAGE, int UB>
+__global__ __launch_bounds__(kStageThreads, 2) void teuk_stage_kernel(StageArgs a) {
+  extern __shared__ __align__(16) double smem[];
+  const TileDesc tile = a.tiles[blockIdx.x];
+  const LineDesc L = a.lines[tile.line];
+  const SchemeTable& T = *a.table;
+  const int tid = threadIdx.x;
+  const int nl = L.n_ell;
+  const int nt = L.ntheta;
+  const int n = L.nr;
+  const int rows = tile.hi - tile.lo;
+  const int mrows = tile.mhi - tile.mlo;
+  const int orows = tile.t1 - tile.t0;
+  const int us = 3 * nl;  // LDS row stride of the staged tile (cdouble)
+
+  // (row, ell) thread decomposition: one division.
+  const int lane_l = tid % nl, row_l = tid / nl, step_l = kStageThreads / nl;
+  const bool on_l = row_l < step_l;
+  const int wave = tid >> 6, lane = tid & 63;
+
+  // LDS: RC[mrows] | U[rows][P|Q|Psi] | PT[mrows] (dissipation only) |
+  // DP[mrows] (stage-constrained only).  Without dissipation psi_t replaces
+  // P in U (P's only other use, the stage-1 input, is read from HBM).
+  const bool inplace = !L.dissipate;
+  RowCoeffs* RC = reinterpret_cast<RowCoeffs*>(smem);
+  cdouble* U = reinterpret_cast<cdouble*>(RC + mrows);
+  cdouble* PT = inplace ? U + (tile.mlo - tile.lo) * us : U + rows * us;
+  const int pts = inplace ? us : nl;
+  cdouble* DP = (inplace ? U + rows * us : PT + mrows * nl);
+
+  const double M = L.mass, spin = L.spin, len = L.length;
+  // 1. stage the tile rows [lo, hi) of all three fields and the per-row
+  //    coefficients.
+  stage_copy(U, a.yin + L.base + static_cast<long long>(tile.lo) * us, rows * us, tid);
+  for (int j = tid; j < mrows; j += kStageThreads) {
+    const RadialCoefficients rc =
+        radial_coefficients(M, spin, len, -2, L.m, L.h * (tile.mlo + j));
+    RC[j] = {2.0 * rc.adv, rc.principal, rc.time_radial, rc.g_real, rc.g_imag_m,
+             rc.q.re,      rc.q.im,      rc.psi.re,      rc.psi.im, 0.0};
+  }
+  __syncthreads();
+
+  // D Psi on the intermediate rows (stage-constrained Q_eff = D Psi).
+  if (L.constrained) {
+    if (on_l)
+      for (int j = tile.mlo + row_l; j < tile.mhi; j += step_l)
+        DP[(j - tile.mlo) * nl + lane_l] =
+            apply_d<SCHEME>(T, U + 2 * nl + lane_l, us, tile.lo, j, n, L.inv_h);
+    __syncthreads();
+  }
+
+  // 2. psi_t on rows [mlo, mhi).  First X = P + 2K Q_eff (into PT's slots).
+  if (on_l)
+    for (int j = tile.mlo + row_l; j < tile.mhi; j += step_l) {
+      const int jj = j - tile.mlo;
+      const cdouble* u = U + (j - tile.lo) * us;
+      const cdouble Q = L.constrained ? DP[jj * nl + lane_l] : u[nl + lane_l];
+      const double k2 = RC[jj].adv2;
+      const cdouble P = u[lane_l];
+      if (!L.kerr) {
+        // a = 0: C_T and G_m are theta independent, the coupling is diagonal.
+        const cdouble g = cmake(RC[jj].g_real, RC[jj].g_imag_m);
+        const cdouble num = csub(cadd(P, cscale(k2, Q)), cmul(g, u[2 * nl + lane_l]));
+        PT[jj * pts + lane_l] = cdivr(num, RC[jj].time_radial);
+      } else {
+        PT[jj * pts + lane_l] = cadd(P, cscale(k2, Q));
+      }
+    }
+  __syncthreads();
+  if (L.kerr) {
+    const double* sinv = a.nodes + L.node_off;
+    const double* cosv = sinv + nt;
+    const double* Sf = a.mats + L.frag_off;
+    const double* Af = Sf + 64 * mfma_s_fragments(nt, nl);
+    const cdouble* psi = U + (tile.mlo - tile.lo) * us + 2 * nl;
+    for (int g0 = 16 * wave; g0 < mrows; g0 += 16 * (kStageThreads / 64))
+      kerr_psi_group<UB>(nl, nt, spin, RC, PT, pts, psi, us, g0, mrows, Sf, Af, sinv, cosv,
+                         lane);
+    __syncthreads();
+  }
+
+  // 3-4. point RHS on the output rows and the RK combination.
+  TileCtx c{&L,  &tile,   &T, RC, U, PT, DP, pts, inplace,
+            a.dt_override > 0.0 ? a.dt_override : L.dt, L.constrained ? 0.0 : L.gamma};
+  if (!L.gram) {
+    plain_output<SCHEME, STAGE>(a, c, tid);
+    return;
+  }
+  const double* Gf = a.mats + L.gfrag_off;
+  for (int g0 = 16 * wave; g0 < orows; g0 += 16 * (kStageThreads / 64))
+    gram_output_group<SCHEME, STAGE, UB>(a, c, Gf, g0, lane);
+}
+
+}  // namespace teuk
