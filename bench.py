@@ -130,23 +130,27 @@ def main() -> int:
     plan.sync()
     seconds = time.perf_counter() - t0
     kernel_ms, launches = plan.kernel_timing()
+    step_ms, timed_steps = plan.step_timing()
+    groups = plan.groups()
     plan.set_kernel_timing(False)
 
     grid = plan.grid_points
     modal = plan.modal_points
     value = grid * args.steps / seconds
     roofline = None
-    groups = plan.groups()
-    if launches:
-        # One RK stage = one launch per execution group; its algorithmic
-        # bytes over the summed launch durations of the stage.
-        stage_s = kernel_ms / (4 * args.steps) * 1e-3
+    if timed_steps:
+        # One RK stage = one launch per execution group, the groups running
+        # concurrently on streams of their own: a stage's algorithmic bytes
+        # over a quarter of the GPU time of a step (HIP events on the plan's
+        # stream at the fork and the join).
+        stage_s = step_ms / (4 * timed_steps) * 1e-3
         achieved = BYTES_PER_MODAL_STEP / 4.0 * modal / stage_s / 1e9
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                     "kernel": "teuk_stage_kernel (4 RK stages)",
                     "avg_stage_ms": stage_s * 1e3,
                     "launches_per_stage": launches / (4 * args.steps),
+                    "sum_of_launches_per_stage_ms": kernel_ms / (4 * args.steps),
                     "groups": [{k: (round(v / (4 * args.steps), 4) if k == "kernel_ms" else v)
                                 for k, v in g.items()} for g in groups]}
     out = {

@@ -163,8 +163,11 @@ int teuk_plan_tiles(const teuk_plan* plan, int* tiles, int* max_lds_bytes);
  * (eager mode only); reset by enabling. */
 int teuk_plan_set_kernel_timing(teuk_plan* plan, int enable);
 int teuk_plan_kernel_timing(const teuk_plan* plan, double* total_ms, long long* launches);
-/* Execution groups: the tiles of one radial scheme, ell-tile class (ell_tiles
- * = 1, 2 or 4 tiles of 16 held by the MFMA routines) and LDS occupancy, one
+/* GPU time of whole RK4 steps (all execution groups, which run on streams of
+ * their own between a fork from and a join into the plan's stream). */
+int teuk_plan_step_timing(const teuk_plan* plan, double* total_ms, long long* steps);
+/* Execution groups: the tiles of one radial scheme and ell-tile class
+ * (ell_tiles = 1, 2 or 4 tiles of 16 held by the MFMA routines), one
  * stage-kernel launch per group and RK stage.  kernel_ms: the group's timed
  * launches since teuk_plan_set_kernel_timing(plan, 1). */
 int teuk_plan_group_count(const teuk_plan* plan, int* count);

@@ -78,6 +78,7 @@ SIGNATURES = {
     "teuk_plan_tiles": (_I, [_P, _IP, _IP]),
     "teuk_plan_set_kernel_timing": (_I, [_P, _I]),
     "teuk_plan_kernel_timing": (_I, [_P, _DP, ctypes.POINTER(ctypes.c_longlong)]),
+    "teuk_plan_step_timing": (_I, [_P, _DP, ctypes.POINTER(ctypes.c_longlong)]),
     "teuk_plan_group_count": (_I, [_P, _IP]),
     "teuk_plan_group_info": (_I, [_P, _I, _IP, _IP, _IP, _IP, _DP]),
     "teuk_device_alloc": (_I, [ctypes.c_size_t, ctypes.POINTER(_P)]),
@@ -342,6 +343,14 @@ class Plan:
         ms = ctypes.c_double()
         n = ctypes.c_longlong()
         self._check(lib().teuk_plan_kernel_timing(self._h, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def step_timing(self):
+        """(GPU milliseconds, steps) of the RK4 steps timed since
+        set_kernel_timing(True)."""
+        ms = ctypes.c_double()
+        n = ctypes.c_longlong()
+        self._check(lib().teuk_plan_step_timing(self._h, ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
 
     def groups(self):

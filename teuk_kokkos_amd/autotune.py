@@ -34,9 +34,9 @@ def _time(plan, steps: int) -> float:
     """Kernel milliseconds per RK stage (all execution groups)."""
     plan.set_kernel_timing(True)
     plan.advance(steps)
-    ms, _ = plan.kernel_timing()
+    ms, n = plan.step_timing()
     plan.set_kernel_timing(False)
-    return ms / (4 * max(steps, 1))
+    return ms / (4 * max(n, 1))
 
 
 def tune(plan, config_texts, steps: int = 3, cache_path: str = DEFAULT_CACHE, log=None):

@@ -319,6 +319,18 @@ int teuk_plan_kernel_timing(const teuk_plan* plan, double* total_ms, long long* 
   });
 }
 
+int teuk_plan_step_timing(const teuk_plan* plan, double* total_ms, long long* steps) {
+  auto* p = const_cast<teuk_plan*>(plan);
+  std::string dummy;
+  return guarded(p ? p->error : dummy, [&] {
+    double ms = 0.0;
+    long long n = 0;
+    plan_of(p).step_timing(&ms, &n);
+    if (total_ms) *total_ms = ms;
+    if (steps) *steps = n;
+  });
+}
+
 int teuk_plan_group_count(const teuk_plan* plan, int* count) {
   auto* p = const_cast<teuk_plan*>(plan);
   std::string dummy;

@@ -51,6 +51,10 @@ struct SchemeTable {
   double diss_interior[2 * kMaxDissOrder + 1];
 };
 
+// The scheme table's size in LDS, padded to 16 bytes.
+inline constexpr int kSchemeTableDoubles = (sizeof(SchemeTable) + 15) / 16 * 2;
+static_assert(sizeof(SchemeTable) % 8 == 0, "SchemeTable is copied as doubles");
+
 // FP64 MFMA (v_mfma_f64_16x16x4_f64) tiling of the Kerr angular coupling.
 // Operand layouts of one wave (lane = 0..63, lr = lane % 16, lg = lane / 16),
 // measured on gfx950:
@@ -90,14 +94,15 @@ __host__ __device__ inline int gram_doubles(int n_ell) {
   return (n_ell * n_ell + 1) / 2 * 2;  // padded to 16 bytes
 }
 
-// LDS of one tile: per-row coefficients, the staged rows, and psi_t / D Psi
-// buffers where they cannot overwrite P in place.  The MFMA operand
-// fragments are read from global memory.
-__host__ __device__ inline int tile_lds_bytes(int rows, int mrows, int n_ell, int constrained,
-                                              int dissipate) {
-  long long b = 80LL * mrows + 16LL * rows * 3 * n_ell;
+// LDS of one tile: per-row coefficients, the staged rows, psi_t / D Psi
+// buffers where they cannot overwrite P in place, and the node table of the
+// Kerr coupling.  The MFMA operand fragments are read from global memory.
+__host__ __device__ inline int tile_lds_bytes(int rows, int mrows, int n_ell, int ntheta,
+                                              int constrained, int dissipate, int kerr) {
+  long long b = 8LL * kSchemeTableDoubles + 80LL * mrows + 16LL * rows * 3 * n_ell;
   if (dissipate) b += 16LL * mrows * n_ell;
   if (constrained) b += 16LL * mrows * n_ell;
+  if (kerr) b += 16LL * ntheta;  // sin and cos of the nodes
   return static_cast<int>(b);
 }
 

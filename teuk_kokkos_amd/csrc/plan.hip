@@ -171,6 +171,12 @@ Plan::~Plan() {
   if (stream_) hipStreamSynchronize(stream_);
   if (graph_exec_) hipGraphExecDestroy(graph_exec_);
   for (hipEvent_t e : events_) hipEventDestroy(e);
+  for (hipEvent_t e : joins_) hipEventDestroy(e);
+  if (fork_) hipEventDestroy(fork_);
+  for (hipStream_t st : side_) {
+    hipStreamSynchronize(st);
+    hipStreamDestroy(st);
+  }
   for (void* p : {static_cast<void*>(d_lines_), static_cast<void*>(d_tiles_),
                   static_cast<void*>(d_mats_), static_cast<void*>(d_nodes_),
                   static_cast<void*>(d_tables_), static_cast<void*>(d_line_mode_),
@@ -336,8 +342,8 @@ void Plan::build_tiles() {
         }
         t.lo = lo;
         t.hi = hi;
-        t.lds_bytes = tile_lds_bytes(hi - lo, t.mhi - t.mlo, L.n_ell, L.constrained,
-                                     L.dissipate);
+        t.lds_bytes = tile_lds_bytes(hi - lo, t.mhi - t.mlo, L.n_ell, L.ntheta, L.constrained,
+                                     L.dissipate, L.kerr);
       };
       // Largest row count whose tiles (boundary-closure tiles included) all
       // fit the LDS budget, capped for parallelism.
@@ -360,6 +366,30 @@ void Plan::build_tiles() {
         }
         if (worst <= lds_budget_ || rows == 1) break;
         rows = rows > 16 ? rows - 4 : rows - 1;
+      }
+      // Kerr lines: the psi_t rows of a tile (output rows plus the stencil
+      // halo) go to the four waves in MFMA groups of 16, so prefer tiles
+      // whose interior psi_t rows fill whole rounds of 64.
+      if (L.kerr && tile_rows_ <= 0 && n > 2 * rows) {
+        TileDesc probe{};
+        ranges(n / 2 - rows / 2, n / 2 - rows / 2 + rows, probe);
+        const int halo = (probe.mhi - probe.mlo) - rows;
+        const int aligned = (rows + halo) / 64 * 64 - halo;
+        if (aligned >= 16 && aligned < rows) {
+          rows = aligned;
+          line_tiles.clear();
+          const int count = (n + rows - 1) / rows;
+          const int basec = n / count, rem = n % count;
+          int t0 = 0;
+          for (int k = 0; k < count; ++k) {
+            const int len = basec + (k < rem ? 1 : 0);
+            TileDesc t{};
+            t.line = li;
+            ranges(t0, t0 + len, t);
+            line_tiles.push_back(t);
+            t0 += len;
+          }
+        }
       }
       for (const TileDesc& t : line_tiles) {
         inst_lds[i] = std::max(inst_lds[i], t.lds_bytes);
@@ -636,41 +666,77 @@ void Plan::export_state(int i, cplx* host) {
   }
 }
 
+hipEvent_t Plan::timing_event() {
+  const size_t k = timing_events_used_++;
+  while (events_.size() <= k) {
+    hipEvent_t e;
+    check(hipEventCreate(&e), "hipEventCreate");
+    events_.push_back(e);
+  }
+  return events_[k];
+}
+
+// One RK4 step of every instance.  Execution groups touch disjoint lines, so
+// each runs its four stage launches on a stream of its own (forked from and
+// joined back into the plan's stream): their tails and their different
+// bottlenecks (MFMA-heavy wide bands, memory-bound narrow ones) overlap.
 void Plan::launch_step(double dt_override) {
-  for (const Group& g : groups_) {
-    if (g.count == 0) continue;
-    StageArgs a{};
-    a.tiles = d_tiles_ + g.first;
-    a.lines = d_lines_;
-    a.mats = d_mats_;
-    a.nodes = d_nodes_;
-    a.table = d_tables_ + g.scheme;
-    a.y = d_y_;
-    a.acc = d_acc_;
-    a.dt_override = dt_override;
-    const cdouble* in_of[5] = {nullptr, d_y_, d_ya_, d_yb_, d_ya_};
-    cdouble* out_of[5] = {nullptr, d_ya_, d_yb_, d_ya_, d_y_};
-    for (int s = 1; s <= 4; ++s) {
-      a.yin = in_of[s];
-      a.yout = out_of[s];
-      const bool timed = timing_ && !use_graph_;
-      if (timed) {
-        const size_t need = 2 * static_cast<size_t>(timing_launches_ + 1);
-        while (events_.size() < need) {
-          hipEvent_t e;
-          check(hipEventCreate(&e), "hipEventCreate");
-          events_.push_back(e);
+  const bool timed = timing_ && !use_graph_;
+  hipEvent_t step_begin = nullptr;
+  if (timed) {
+    step_begin = timing_event();
+    check(hipEventRecord(step_begin, stream_), "event");
+  }
+  const int ng = static_cast<int>(groups_.size());
+  while (static_cast<int>(side_.size()) < ng - 1) {
+    hipStream_t st;
+    check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+    side_.push_back(st);
+    hipEvent_t e;
+    check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+    joins_.push_back(e);
+  }
+  if (!fork_) check(hipEventCreateWithFlags(&fork_, hipEventDisableTiming), "hipEventCreate");
+  if (ng > 1) check(hipEventRecord(fork_, stream_), "fork");
+  for (int gi = 0; gi < ng; ++gi) {
+    const Group& g = groups_[gi];
+    hipStream_t st = gi == 0 ? stream_ : side_[gi - 1];
+    if (gi > 0) check(hipStreamWaitEvent(st, fork_, 0), "fork wait");
+    if (g.count > 0) {
+      StageArgs a{};
+      a.tiles = d_tiles_ + g.first;
+      a.lines = d_lines_;
+      a.mats = d_mats_;
+      a.nodes = d_nodes_;
+      a.table = d_tables_ + g.scheme;
+      a.y = d_y_;
+      a.acc = d_acc_;
+      a.dt_override = dt_override;
+      const cdouble* in_of[5] = {nullptr, d_y_, d_ya_, d_yb_, d_ya_};
+      cdouble* out_of[5] = {nullptr, d_ya_, d_yb_, d_ya_, d_y_};
+      for (int s = 1; s <= 4; ++s) {
+        a.yin = in_of[s];
+        a.yout = out_of[s];
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (timed) {
+          e0 = timing_event();
+          check(hipEventRecord(e0, st), "event");
         }
-        check(hipEventRecord(events_[2 * timing_launches_], stream_), "event");
-      }
-      check(teuk::launch_stage(g.scheme, s, g.ub, a, g.count, g.lds, stream_), "stage kernel");
-      if (timed) {
-        check(hipEventRecord(events_[2 * timing_launches_ + 1], stream_), "event");
-        event_group_.resize(timing_launches_ + 1);
-        event_group_[timing_launches_] = static_cast<int>(&g - groups_.data());
-        ++timing_launches_;
+        check(teuk::launch_stage(g.scheme, s, g.ub, a, g.count, g.lds, st), "stage kernel");
+        if (timed) {
+          e1 = timing_event();
+          check(hipEventRecord(e1, st), "event");
+          launch_events_.push_back({e0, e1, gi});
+        }
       }
     }
+    if (gi > 0) check(hipEventRecord(joins_[gi - 1], st), "join");
+  }
+  for (int gi = 1; gi < ng; ++gi) check(hipStreamWaitEvent(stream_, joins_[gi - 1], 0), "join wait");
+  if (timed) {
+    hipEvent_t step_end = timing_event();
+    check(hipEventRecord(step_end, stream_), "event");
+    step_events_.push_back({step_begin, step_end});
   }
 }
 
@@ -826,19 +892,31 @@ void Plan::set_graph(bool enable) {
 void Plan::set_kernel_timing(bool enable) {
   check(hipStreamSynchronize(stream_), "timing sync");
   timing_ = enable;
-  timing_launches_ = 0;
+  timing_events_used_ = 0;
+  launch_events_.clear();
+  step_events_.clear();
+}
+
+static double elapsed_ms(hipEvent_t a, hipEvent_t b) {
+  float e = 0.f;
+  check(hipEventElapsedTime(&e, a, b), "elapsed");
+  return e;
 }
 
 void Plan::kernel_timing(double* total_ms, long long* launches) {
   check(hipStreamSynchronize(stream_), "timing sync");
   double ms = 0.0;
-  for (long long k = 0; k < timing_launches_; ++k) {
-    float e = 0.f;
-    check(hipEventElapsedTime(&e, events_[2 * k], events_[2 * k + 1]), "elapsed");
-    ms += e;
-  }
+  for (const auto& le : launch_events_) ms += elapsed_ms(le.begin, le.end);
   *total_ms = ms;
-  *launches = timing_launches_;
+  *launches = static_cast<long long>(launch_events_.size());
+}
+
+void Plan::step_timing(double* total_ms, long long* steps) {
+  check(hipStreamSynchronize(stream_), "timing sync");
+  double ms = 0.0;
+  for (const auto& se : step_events_) ms += elapsed_ms(se.first, se.second);
+  *total_ms = ms;
+  *steps = static_cast<long long>(step_events_.size());
 }
 
 void Plan::group_info(int g, int* scheme, int* ub, int* tiles, int* lds_bytes,
@@ -851,12 +929,8 @@ void Plan::group_info(int g, int* scheme, int* ub, int* tiles, int* lds_bytes,
   *lds_bytes = gr.lds;
   check(hipStreamSynchronize(stream_), "timing sync");
   double ms = 0.0;
-  for (long long k = 0; k < timing_launches_; ++k) {
-    if (event_group_[k] != g) continue;
-    float e = 0.f;
-    check(hipEventElapsedTime(&e, events_[2 * k], events_[2 * k + 1]), "elapsed");
-    ms += e;
-  }
+  for (const auto& le : launch_events_)
+    if (le.group == g) ms += elapsed_ms(le.begin, le.end);
   *kernel_ms = ms;
 }
 

@@ -81,6 +81,8 @@ class Plan {
   void set_graph(bool enable);
   void set_kernel_timing(bool enable);
   void kernel_timing(double* total_ms, long long* launches);
+  // GPU time of whole steps (fork of the group streams to their join).
+  void step_timing(double* total_ms, long long* steps);
   // Execution groups: one stage launch each (tiles of one radial scheme, ell
   // tile class and occupancy); kernel_ms sums the group's timed launches.
   int group_count() const { return static_cast<int>(groups_.size()); }
@@ -124,9 +126,18 @@ class Plan {
   bool use_graph_ = false;
   hipGraphExec_t graph_exec_ = nullptr;
   bool timing_ = false;
-  std::vector<hipEvent_t> events_;
-  std::vector<int> event_group_;
-  long long timing_launches_ = 0;
+  hipEvent_t timing_event();
+  std::vector<hipEvent_t> events_;  // pool, reused after set_kernel_timing
+  size_t timing_events_used_ = 0;
+  struct LaunchEvents {
+    hipEvent_t begin, end;
+    int group;
+  };
+  std::vector<LaunchEvents> launch_events_;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> step_events_;
+  std::vector<hipStream_t> side_;  // streams of execution groups 1..n-1
+  std::vector<hipEvent_t> joins_;
+  hipEvent_t fork_ = nullptr;
 };
 
 }  // namespace teuk

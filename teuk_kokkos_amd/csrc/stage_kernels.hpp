@@ -156,13 +156,15 @@ __device__ __forceinline__ mfma_acc mfma_f64(double a, double b, mfma_acc c) {
 // The accumulator of the synthesis is the A operand of the analysis, so the
 // nodal values never leave registers.  The S and A fragments come from
 // global memory (cache resident, shared by every tile of the band), one
-// theta tile ahead.  UB = ell tiles of 16 held at once (K <= 4 UB blocks).
+// theta tile ahead; sin and cos of the nodes from LDS.  UB = ell tiles of 16
+// held at once (K <= 4 UB blocks).
 template <int UB>
 __device__ __forceinline__ void kerr_psi_group(int nl, int nt, double spin, const RowCoeffs* RC,
                                                cdouble* PT, int pts, const cdouble* psi, int us,
                                                int g0, int mrows, const double* Sf,
-                                               const double* Af, const double* sinv,
-                                               const double* cosv, int lane) {
+                                               const double* Af, const double* nodes,
+                                               int lane) {
+  constexpr int KB = 4 * UB;
   const int T = mfma_theta_tiles(nt), K = mfma_ell_blocks(nl), LT = mfma_ell_tiles(nl);
   const int lr = lane & 15, lg = lane >> 4;
   const int jj = g0 + lr;
@@ -172,38 +174,40 @@ __device__ __forceinline__ void kerr_psi_group(int nl, int nt, double spin, cons
   const cdouble* ps = psi + jc * us;
   const double tau = RC[jc].time_radial, gre = RC[jc].g_real, gim = RC[jc].g_imag_m;
   const double a2 = spin * spin, c4 = -4.0 * spin;
+  const double* sinv = nodes;
+  const double* cosv = nodes + nt;
   mfma_acc ar[UB], ai[UB];
 #pragma unroll
   for (int u = 0; u < UB; ++u) ar[u] = ai[u] = mfma_acc{0.0, 0.0, 0.0, 0.0};
-  // S fragment (t, q) sits at flat index t K + q: three in flight ahead.
-  const int nf = T * K;
-  const double* sf = Sf + lane;
-  double s0 = sf[0], s1 = nf > 1 ? sf[64] : 0.0, s2 = nf > 2 ? sf[128] : 0.0;
-  int f = 0;
-  for (int t = 0; t < T; ++t) {
-    // this tile's A fragments arrive during the synthesis
-    double afr[4][UB];
-    const double* af = Af + t * 4 * LT * 64 + lane;
+  // S and A fragments of one theta tile; the next tile's are in flight while
+  // the current one is processed.
+  double s_cur[KB], s_nxt[KB], a_cur[4][UB], a_nxt[4][UB];
+  auto load_tile = [&](int t, double (&sf)[KB], double (&af)[4][UB]) {
+    const double* s = Sf + t * K * 64 + lane;
+#pragma unroll
+    for (int q = 0; q < KB; ++q) sf[q] = q < K ? s[q * 64] : 0.0;
+    const double* a = Af + t * 4 * LT * 64 + lane;
 #pragma unroll
     for (int v = 0; v < 4; ++v)
 #pragma unroll
-      for (int u = 0; u < UB; ++u) afr[v][u] = u < LT ? af[(v * LT + u) * 64] : 0.0;
+      for (int u = 0; u < UB; ++u) af[v][u] = u < LT ? a[(v * LT + u) * 64] : 0.0;
+  };
+  load_tile(0, s_cur, a_cur);
+  for (int t = 0; t < T; ++t) {
+    if (t + 1 < T) load_tile(t + 1, s_nxt, a_nxt);
     mfma_acc sxr = {0.0, 0.0, 0.0, 0.0}, sxi = sxr, spr = sxr, spi = sxr;
-#pragma unroll 2
-    for (int q = 0; q < K; ++q, ++f) {
-      const double s = s0;
-      s0 = s1;
-      s1 = s2;
-      s2 = f + 3 < nf ? sf[(f + 3) * 64] : 0.0;
+#pragma unroll
+    for (int q = 0; q < KB; ++q) {
+      if (q >= K) break;
       // B operands X, Psi of (row jj, ell 4 q + lg) from LDS
       const int l = 4 * q + lg;
       const bool ok = row_ok && l < nl;
       const int lc = ok ? l : 0;
       const cdouble xv = x[lc], pv = ps[lc];
-      sxr = mfma_f64(s, ok ? xv.re : 0.0, sxr);
-      sxi = mfma_f64(s, ok ? xv.im : 0.0, sxi);
-      spr = mfma_f64(s, ok ? pv.re : 0.0, spr);
-      spi = mfma_f64(s, ok ? pv.im : 0.0, spi);
+      sxr = mfma_f64(s_cur[q], ok ? xv.re : 0.0, sxr);
+      sxi = mfma_f64(s_cur[q], ok ? xv.im : 0.0, sxi);
+      spr = mfma_f64(s_cur[q], ok ? pv.re : 0.0, spr);
+      spi = mfma_f64(s_cur[q], ok ? pv.im : 0.0, spi);
     }
     // element v of this lane: theta = 16 t + lg + 4 v, radial row jj
     double nre[4], nim[4];
@@ -226,9 +230,15 @@ __device__ __forceinline__ void kerr_psi_group(int nl, int nt, double spin, cons
 #pragma unroll
       for (int u = 0; u < UB; ++u)
         if (u < LT) {
-          ar[u] = mfma_f64(nre[v], afr[v][u], ar[u]);
-          ai[u] = mfma_f64(nim[v], afr[v][u], ai[u]);
+          ar[u] = mfma_f64(nre[v], a_cur[v][u], ar[u]);
+          ai[u] = mfma_f64(nim[v], a_cur[v][u], ai[u]);
         }
+#pragma unroll
+    for (int q = 0; q < KB; ++q) s_cur[q] = s_nxt[q];
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+      for (int u = 0; u < UB; ++u) a_cur[v][u] = a_nxt[v][u];
   }
   // element w of this lane: psi_t at row g0 + lg + 4 w, ell 16 u + lr.  The
   // wave's LDS reads of X were issued before these writes, and no other wave
@@ -419,8 +429,18 @@ __device__ __forceinline__ void gram_output_group(const StageArgs& a, const Tile
     for (int uo = 0; uo < UB; ++uo) ar[uo] = ai[uo] = mfma_acc{0.0, 0.0, 0.0, 0.0};
     // without dissipation the Psi pre-projection terms vanish
     if (F != 2 || L.dissipate) {
-#pragma unroll 2
+      // G (and M) fragments of block kb + 1 are in flight during block kb
+      double g_cur[UB], m_cur[UB], g_nxt[UB], m_nxt[UB];
+      auto load_block = [&](int kb, double (&g)[UB], double (&m)[UB]) {
+#pragma unroll
+        for (int uo = 0; uo < UB; ++uo) {
+          g[uo] = uo < LT ? Gf[(uo * K + kb) * 64 + lane] : 0.0;
+          if constexpr (F == 0) m[uo] = uo < LT ? Mf[(uo * K + kb) * 64 + lane] : 0.0;
+        }
+      };
+      load_block(0, g_cur, m_cur);
       for (int kb = 0; kb < K; ++kb) {
+        if (kb + 1 < K) load_block(kb + 1, g_nxt, m_nxt);
         const int l = 4 * kb + lg;
         cdouble x = zero, ps = zero;
         if (l < nl) {
@@ -430,14 +450,17 @@ __device__ __forceinline__ void gram_output_group(const StageArgs& a, const Tile
 #pragma unroll
         for (int uo = 0; uo < UB; ++uo) {
           if (uo >= LT) break;
-          const double g = Gf[(uo * K + kb) * 64 + lane];
-          ar[uo] = mfma_f64(x.re, g, ar[uo]);
-          ai[uo] = mfma_f64(x.im, g, ai[uo]);
+          ar[uo] = mfma_f64(x.re, g_cur[uo], ar[uo]);
+          ai[uo] = mfma_f64(x.im, g_cur[uo], ai[uo]);
           if constexpr (F == 0) {
-            const double m = Mf[(uo * K + kb) * 64 + lane];
-            ar[uo] = mfma_f64(ps.re, m, ar[uo]);
-            ai[uo] = mfma_f64(ps.im, m, ai[uo]);
+            ar[uo] = mfma_f64(ps.re, m_cur[uo], ar[uo]);
+            ai[uo] = mfma_f64(ps.im, m_cur[uo], ai[uo]);
           }
+        }
+#pragma unroll
+        for (int uo = 0; uo < UB; ++uo) {
+          g_cur[uo] = g_nxt[uo];
+          if constexpr (F == 0) m_cur[uo] = m_nxt[uo];
         }
       }
     }
@@ -470,11 +493,10 @@ __device__ __forceinline__ void gram_output_group(const StageArgs& a, const Tile
 // One workgroup = one radial tile of one line; UB = ell tiles of 16 the MFMA
 // routines hold at once (1, 2 or 4).
 template <int SCHEME, int STAGE, int UB>
-__global__ __launch_bounds__(kStageThreads, 2) void teuk_stage_kernel(StageArgs a) {
+__global__ __launch_bounds__(kStageThreads, UB == 1 ? 3 : 2) void teuk_stage_kernel(StageArgs a) {
   extern __shared__ __align__(16) double smem[];
   const TileDesc tile = a.tiles[blockIdx.x];
   const LineDesc L = a.lines[tile.line];
-  const SchemeTable& T = *a.table;
   const int tid = threadIdx.x;
   const int nl = L.n_ell;
   const int nt = L.ntheta;
@@ -489,20 +511,29 @@ __global__ __launch_bounds__(kStageThreads, 2) void teuk_stage_kernel(StageArgs 
   const bool on_l = row_l < step_l;
   const int wave = tid >> 6, lane = tid & 63;
 
-  // LDS: RC[mrows] | U[rows][P|Q|Psi] | PT[mrows] (dissipation only) |
-  // DP[mrows] (stage-constrained only).  Without dissipation psi_t replaces
+  // LDS: SBP table | RC[mrows] | U[rows][P|Q|Psi] | PT[mrows] (dissipation only) |
+  // DP[mrows] (stage-constrained only) | sin, cos of the nodes (Kerr only).  Without dissipation psi_t replaces
   // P in U (P's only other use, the stage-1 input, is read from HBM).
   const bool inplace = !L.dissipate;
-  RowCoeffs* RC = reinterpret_cast<RowCoeffs*>(smem);
+  // The SBP coefficient table lives in LDS: closure rows index it per lane,
+  // which from global memory costs a full load latency per tap.
+  SchemeTable* TS = reinterpret_cast<SchemeTable*>(smem);
+  const SchemeTable& T = *TS;
+  RowCoeffs* RC = reinterpret_cast<RowCoeffs*>(smem + kSchemeTableDoubles);
   cdouble* U = reinterpret_cast<cdouble*>(RC + mrows);
   cdouble* PT = inplace ? U + (tile.mlo - tile.lo) * us : U + rows * us;
   const int pts = inplace ? us : nl;
   cdouble* DP = (inplace ? U + rows * us : PT + mrows * nl);
+  double* NODES = reinterpret_cast<double*>(L.constrained ? DP + mrows * nl : DP);  // sin | cos
 
   const double M = L.mass, spin = L.spin, len = L.length;
   // 1. stage the tile rows [lo, hi) of all three fields and the per-row
   //    coefficients.
   stage_copy(U, a.yin + L.base + static_cast<long long>(tile.lo) * us, rows * us, tid);
+  for (int j = tid; j < static_cast<int>(sizeof(SchemeTable) / 8); j += kStageThreads)
+    smem[j] = reinterpret_cast<const double*>(a.table)[j];
+  if (L.kerr)
+    for (int j = tid; j < 2 * nt; j += kStageThreads) NODES[j] = a.nodes[L.node_off + j];
   for (int j = tid; j < mrows; j += kStageThreads) {
     const RadialCoefficients rc =
         radial_coefficients(M, spin, len, -2, L.m, L.h * (tile.mlo + j));
@@ -539,14 +570,11 @@ __global__ __launch_bounds__(kStageThreads, 2) void teuk_stage_kernel(StageArgs 
     }
   __syncthreads();
   if (L.kerr) {
-    const double* sinv = a.nodes + L.node_off;
-    const double* cosv = sinv + nt;
     const double* Sf = a.mats + L.frag_off;
     const double* Af = Sf + 64 * mfma_s_fragments(nt, nl);
     const cdouble* psi = U + (tile.mlo - tile.lo) * us + 2 * nl;
     for (int g0 = 16 * wave; g0 < mrows; g0 += 16 * (kStageThreads / 64))
-      kerr_psi_group<UB>(nl, nt, spin, RC, PT, pts, psi, us, g0, mrows, Sf, Af, sinv, cosv,
-                         lane);
+      kerr_psi_group<UB>(nl, nt, spin, RC, PT, pts, psi, us, g0, mrows, Sf, Af, NODES, lane);
     __syncthreads();
   }
 
