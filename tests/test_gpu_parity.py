@@ -197,3 +197,57 @@ def test_invalid_inputs_fail_closed():
             plan.step(0.0, -1.0)
         with pytest.raises(T.TeukInvalidArgument):
             plan.step(float("nan"), 1e-3)
+
+
+def _two_group_cases():
+    """Instances in different execution groups (ell-tile classes 1 and 2)."""
+    return [CASES["kerr_d105"], CASES["kerr_ell24_gram"], CASES["kerr_m0_m3_sharp"]]
+
+
+def test_graph_replay_equals_eager_across_execution_groups():
+    cases = _two_group_cases()
+    out = []
+    for graph in (False, True):
+        with T.Plan([W.config_text(**dict(c)) for c in cases]) as plan:
+            assert len(plan.groups()) >= 2
+            plan.initialize()
+            plan.set_graph(graph)
+            plan.advance(2)
+            plan.sync()
+            plan.advance(3)
+            out.append([plan.export_state(i) for i in range(len(cases))])
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
+
+
+def test_evaluate_rhs_for_an_instance_of_a_later_group():
+    cases = _two_group_cases()
+    i = 1  # the ell_max 24 instance
+    ref = H.oracle_initialized(cases[i])
+    state = ref.get_state()
+    expected = ref.evaluate_rhs(state)
+    with T.Plan([W.config_text(**dict(c)) for c in cases]) as plan:
+        plan.initialize()
+        d_in = T.DeviceArray(state)
+        d_out = T.DeviceArray(np.zeros_like(state))
+        plan.evaluate_rhs_device(d_in.ptr, d_out.ptr, instance=i)
+        plan.sync()
+        got = d_out.download()
+    assert H.rel_err(got, expected) <= TOL
+
+
+def test_step_and_group_timing_account_for_every_launch():
+    cases = _two_group_cases()
+    with T.Plan([W.config_text(**dict(c)) for c in cases]) as plan:
+        plan.initialize()
+        groups = plan.groups()
+        plan.set_kernel_timing(True)
+        plan.advance(3)
+        ms, steps = plan.step_timing()
+        total, launches = plan.kernel_timing()
+        per_group = plan.groups()
+        plan.set_kernel_timing(False)
+    assert steps == 3 and ms > 0.0
+    assert launches == 3 * 4 * len(groups)
+    assert abs(sum(g["kernel_ms"] for g in per_group) - total) <= 1e-9 * max(total, 1.0)
+    assert sum(g["tiles"] for g in groups) == sum(g["tiles"] for g in per_group)
