@@ -75,6 +75,9 @@ def main() -> int:
     ap.add_argument("--graph", action="store_true", help="replay each step as a hipGraph")
     ap.add_argument("--lds-budget", type=int, default=0)
     ap.add_argument("--tile-rows", type=int, default=0)
+    ap.add_argument("--segment-rows", type=int, default=-1,
+                    help="marching-kernel rows per workgroup (0: tile kernel only; "
+                         "default: the plan's)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--autotune", action="store_true",
@@ -109,6 +112,8 @@ def main() -> int:
         plan.set_lds_budget(args.lds_budget)
     if args.tile_rows:
         plan.set_tile_rows(args.tile_rows)
+    if args.segment_rows >= 0:
+        plan.set_segment_rows(args.segment_rows)
     if args.autotune:
         from teuk_kokkos_amd import autotune
         plan.initialize()

@@ -158,6 +158,11 @@ int teuk_plan_set_graph(teuk_plan* plan, int enable);
  * both re-tile the plan. */
 int teuk_plan_set_lds_budget(teuk_plan* plan, int bytes);
 int teuk_plan_set_tile_rows(teuk_plan* plan, int rows);
+/* Output rows per workgroup of the marching kernel, which streams radial
+ * segments of the instances whose lines all have at most 16 ell and neither
+ * dissipation nor the stage constraint (0, the default: the tile kernel for
+ * every instance); re-tiles the plan. */
+int teuk_plan_set_segment_rows(teuk_plan* plan, int rows);
 int teuk_plan_tiles(const teuk_plan* plan, int* tiles, int* max_lds_bytes);
 /* HIP-event timing of every stage-kernel launch on the plan's stream
  * (eager mode only); reset by enabling. */
@@ -167,7 +172,8 @@ int teuk_plan_kernel_timing(const teuk_plan* plan, double* total_ms, long long* 
  * their own between a fork from and a join into the plan's stream). */
 int teuk_plan_step_timing(const teuk_plan* plan, double* total_ms, long long* steps);
 /* Execution groups: the tiles of one radial scheme and ell-tile class
- * (ell_tiles = 1, 2 or 4 tiles of 16 held by the MFMA routines), one
+ * (ell_tiles = 1, 2 or 4 tiles of 16 held by the MFMA routines, 0 for the
+ * segments of the marching kernel), one
  * stage-kernel launch per group and RK stage.  kernel_ms: the group's timed
  * launches since teuk_plan_set_kernel_timing(plan, 1). */
 int teuk_plan_group_count(const teuk_plan* plan, int* count);

@@ -75,6 +75,7 @@ SIGNATURES = {
     "teuk_plan_set_graph": (_I, [_P, _I]),
     "teuk_plan_set_lds_budget": (_I, [_P, _I]),
     "teuk_plan_set_tile_rows": (_I, [_P, _I]),
+    "teuk_plan_set_segment_rows": (_I, [_P, _I]),
     "teuk_plan_tiles": (_I, [_P, _IP, _IP]),
     "teuk_plan_set_kernel_timing": (_I, [_P, _I]),
     "teuk_plan_kernel_timing": (_I, [_P, _DP, ctypes.POINTER(ctypes.c_longlong)]),
@@ -329,6 +330,10 @@ class Plan:
 
     def set_tile_rows(self, rows: int):
         self._check(lib().teuk_plan_set_tile_rows(self._h, rows))
+
+    def set_segment_rows(self, rows: int):
+        """Output rows per marching-kernel workgroup (0: tile kernel only)."""
+        self._check(lib().teuk_plan_set_segment_rows(self._h, rows))
 
     def tiles(self):
         n = ctypes.c_int()

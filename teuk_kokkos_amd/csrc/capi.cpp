@@ -291,6 +291,11 @@ int teuk_plan_set_tile_rows(teuk_plan* plan, int rows) {
   return guarded(plan ? plan->error : dummy, [&] { plan_of(plan).set_tile_rows(rows); });
 }
 
+int teuk_plan_set_segment_rows(teuk_plan* plan, int rows) {
+  std::string dummy;
+  return guarded(plan ? plan->error : dummy, [&] { plan_of(plan).set_segment_rows(rows); });
+}
+
 int teuk_plan_tiles(const teuk_plan* plan, int* tiles, int* max_lds_bytes) {
   auto* p = const_cast<teuk_plan*>(plan);
   std::string dummy;

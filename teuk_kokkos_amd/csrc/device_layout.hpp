@@ -106,6 +106,43 @@ __host__ __device__ inline int tile_lds_bytes(int rows, int mrows, int n_ell, in
   return static_cast<int>(b);
 }
 
+// Compile-time shape of radial scheme 0 (D4-2), 1 (D8-4), 2 (D10-5): W
+// closure rows per boundary, C closure columns, B interior half-width, O the
+// dissipation operator's half-width.
+template <int SCHEME>
+struct SchemeShape {
+  static constexpr int W = SCHEME == 2 ? 11 : (SCHEME == 1 ? 8 : 4);
+  static constexpr int C = SCHEME == 2 ? 16 : (SCHEME == 1 ? 12 : 6);
+  static constexpr int B = SCHEME == 2 ? 5 : (SCHEME == 1 ? 4 : 2);
+  static constexpr int O = SCHEME == 2 ? 6 : (SCHEME == 1 ? 5 : 3);
+};
+
+// Marching stage kernel: one workgroup streams the output rows of a radial
+// segment, kMarchRows new rows per step, keeping the last kMarchKeep staged
+// rows (the stencil reach to the left) in LDS for the next step.  It serves
+// lines of one ell tile without dissipation in free-damped mode.  The rows of
+// a step are requested from HBM one step ahead into registers: kMarchPrefetch
+// complex values per thread cover kMarchRows rows of 3 x 16 coefficients.
+inline constexpr int kMarchRows = 64;
+inline constexpr int kMarchKeep = 16;
+inline constexpr int kMarchPrefetch = kMarchRows * 3 * 16 / kStageThreads;
+static_assert(kMarchPrefetch * kStageThreads == kMarchRows * 3 * 16, "whole prefetch rounds");
+static_assert(kMarchKeep >= SchemeShape<2>::C && kMarchKeep >= 2 * SchemeShape<2>::B,
+              "kept rows cover every stencil's reach to the left");
+
+__host__ __device__ inline bool march_line(const LineDesc& L) {
+  return L.n_ell <= 16 && !L.dissipate && !L.constrained;
+}
+
+// LDS of the marching kernel: the scheme table, per-row coefficients and
+// staged rows of kMarchKeep + kMarchRows rows, and the Kerr node table.
+__host__ __device__ inline int march_lds_bytes(int n_ell, int ntheta, int kerr) {
+  const int rows = kMarchKeep + kMarchRows;
+  long long b = 8LL * kSchemeTableDoubles + 80LL * rows + 16LL * rows * 3 * n_ell;
+  if (kerr) b += 16LL * ntheta;
+  return static_cast<int>(b);
+}
+
 // Kernel arguments of one stage launch.
 struct StageArgs {
   const TileDesc* tiles;

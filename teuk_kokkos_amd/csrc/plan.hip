@@ -396,9 +396,43 @@ void Plan::build_tiles() {
         per[i].push_back(t);
       }
     }
+    // Instances whose lines all suit the marching kernel use it instead:
+    // segments of about segment_rows_ output rows per workgroup.
+    bool march = segment_rows_ > 0 && in.M > 0;
+    for (int mi = 0; mi < in.M && march; ++mi) {
+      const int li = in.line_of_mode[mi];
+      if (li >= 0 && !march_line(lines_[li])) march = false;
+    }
+    if (march && !per[i].empty()) {
+      per[i].clear();
+      inst_lds[i] = 0;
+      inst_ub[i] = 0;
+      for (int mi = 0; mi < in.M; ++mi) {
+        const int li = in.line_of_mode[mi];
+        if (li < 0) continue;
+        const LineDesc& L = lines_[li];
+        const int n = L.nr;
+        const int count = std::max(1, (n + segment_rows_ - 1) / segment_rows_);
+        const int basec = n / count, rem = n % count;
+        int t0 = 0;
+        for (int k = 0; k < count; ++k) {
+          const int len = basec + (k < rem ? 1 : 0);
+          TileDesc t{};
+          t.line = li;
+          t.t0 = t0;
+          t.t1 = t0 + len;
+          t.lo = t.mlo = std::max(0, t0 - kMarchKeep);
+          t.hi = t.mhi = std::min(n, t0 + len + kMarchKeep);
+          t.lds_bytes = march_lds_bytes(L.n_ell, L.ntheta, L.kerr);
+          inst_lds[i] = std::max(inst_lds[i], t.lds_bytes);
+          per[i].push_back(t);
+          t0 += len;
+        }
+      }
+    }
   }
-  // One launch per (scheme, ell-tile class); a launch reserves its largest
-  // tile's LDS (at most the budget) for every workgroup.
+  // One launch per (scheme, ell-tile class or 0 for the marching kernel); a
+  // launch reserves its largest tile's LDS for every workgroup.
   std::map<std::tuple<int, int, int>, std::vector<int>> buckets;
   for (int i = 0; i < instance_count(); ++i)
     buckets[{static_cast<int>(inst_[i].params.scheme), inst_ub[i], 0}].push_back(i);
@@ -941,6 +975,17 @@ void Plan::set_lds_budget(int bytes) {
   build_tiles();
   upload_tables();
   set_graph(use_graph_);
+  if (graph_exec_) {
+    hipGraphExecDestroy(graph_exec_);
+    graph_exec_ = nullptr;
+  }
+}
+
+void Plan::set_segment_rows(int rows) {
+  if (rows < 0) throw std::invalid_argument("segment rows must be nonnegative");
+  segment_rows_ = rows;
+  build_tiles();
+  upload_tables();
   if (graph_exec_) {
     hipGraphExecDestroy(graph_exec_);
     graph_exec_ = nullptr;

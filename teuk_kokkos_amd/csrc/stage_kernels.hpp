@@ -32,9 +32,9 @@ namespace {
 template <int SCHEME>
 __device__ __forceinline__ cdouble apply_d(const SchemeTable& T, const cdouble* base, int stride,
                                            int lo, int row, int n, double inv_h) {
-  constexpr int W = SCHEME == 2 ? 11 : (SCHEME == 1 ? 8 : 4);
-  constexpr int C = SCHEME == 2 ? 16 : (SCHEME == 1 ? 12 : 6);
-  constexpr int B = SCHEME == 2 ? 5 : (SCHEME == 1 ? 4 : 2);
+  constexpr int W = SchemeShape<SCHEME>::W;
+  constexpr int C = SchemeShape<SCHEME>::C;
+  constexpr int B = SchemeShape<SCHEME>::B;
   double re = 0.0, im = 0.0;
   if (row < W) {
     const double* c = &T.left[row * kMaxClosureWidth];
@@ -74,8 +74,8 @@ template <int SCHEME>
 __device__ __forceinline__ cdouble apply_diss(const SchemeTable& T, const cdouble* base,
                                               int stride, int lo, int row, int n, double h,
                                               double eps) {
-  constexpr int O = SCHEME == 2 ? 6 : (SCHEME == 1 ? 5 : 3);
-  constexpr int W = SCHEME == 2 ? 11 : (SCHEME == 1 ? 8 : 4);
+  constexpr int O = SchemeShape<SCHEME>::O;
+  constexpr int W = SchemeShape<SCHEME>::W;
   double re = 0.0, im = 0.0;
   if (row < O) {
     const double* c = T.diss_left[row];
@@ -298,7 +298,8 @@ struct TileCtx {
   const LineDesc* L;
   const TileDesc* tile;
   const SchemeTable* T;
-  const RowCoeffs* RC;
+  const RowCoeffs* RC;  // coefficients of row r at RC[r - rc_lo]
+  int rc_lo;
   const cdouble* U;   // staged rows [lo, hi): P | Q | Psi per row
   const cdouble* PT;  // psi_t on rows [mlo, mhi), row stride pts
   const cdouble* DP;  // D Psi on rows [mlo, mhi) (stage-constrained only)
@@ -328,7 +329,7 @@ __device__ __forceinline__ cdouble pre_projection(const TileCtx& c, int r, int l
       qeff = u[nl + l];
       dq = apply_d<SCHEME>(*c.T, c.U + nl + l, us, tile.lo, r, n, L.inv_h);
     }
-    const RowCoeffs& rc = c.RC[r - tile.mlo];
+    const RowCoeffs& rc = c.RC[r - c.rc_lo];
     const double lam = -static_cast<double>((L.ell_min + l + 2) * (L.ell_min + l - 1));
     const cdouble radial = cadd(cscale(rc.principal, dq), cmul(cmake(rc.q_re, rc.q_im), qeff));
     const cdouble coupling = cmul(cmake(rc.psi_re, rc.psi_im), u[2 * nl + l]);
@@ -490,6 +491,99 @@ __device__ __forceinline__ void gram_output_group(const StageArgs& a, const Tile
   field(std::integral_constant<int, 2>{});
 }
 
+// LDS pointers of one pass of the stage kernels.  A pass covers the rows of
+// a TileDesc: [lo, hi) staged (U, RC; buffer row = row - lo), [mlo, mhi) whose
+// psi_t (and D Psi when stage-constrained) it computes, [t0, t1) output rows.
+struct StageLds {
+  const SchemeTable* T;
+  RowCoeffs* RC;   // coefficients of row j at RC[j - rc_lo]
+  int rc_lo;
+  cdouble* U;      // rows [lo, hi): P | Q | Psi
+  cdouble* PT;     // psi_t of row j at PT[(j - mlo) * pts]
+  int pts;
+  cdouble* DP;     // D Psi on rows [mlo, mhi) (stage-constrained only)
+  double* NODES;   // sin | cos of the nodes (Kerr only)
+  bool inplace;    // psi_t overwrites P in U
+};
+
+// Per-row coefficients of rows [r0, r1) into RC (row j at RC[j - lo]).
+__device__ __forceinline__ void row_coefficients(const LineDesc& L, RowCoeffs* RC, int lo, int r0,
+                                                 int r1, int tid) {
+  for (int j = r0 + tid; j < r1; j += kStageThreads) {
+    const RadialCoefficients rc =
+        radial_coefficients(L.mass, L.spin, L.length, -2, L.m, L.h * j);
+    RC[j - lo] = {2.0 * rc.adv, rc.principal, rc.time_radial, rc.g_real, rc.g_imag_m,
+                  rc.q.re,      rc.q.im,      rc.psi.re,      rc.psi.im, 0.0};
+  }
+}
+
+// psi_t on rows [mlo, mhi): D Psi (stage-constrained), X = P + 2K Q_eff,
+// then the Kerr coupling (or the diagonal a = 0 form).  Ends with a barrier.
+// CONSTRAINED = false compiles the stage-constrained form out.
+template <int SCHEME, int UB, bool CONSTRAINED = true>
+__device__ __forceinline__ void psi_t_phase(const StageArgs& a, const LineDesc& L,
+                                            const TileDesc& tile, const StageLds& s, int tid) {
+  const int nl = L.n_ell, nt = L.ntheta, n = L.nr, us = 3 * nl;
+  const int mrows = tile.mhi - tile.mlo;
+  const int lane_l = tid % nl, row_l = tid / nl, step_l = kStageThreads / nl;
+  const bool on_l = row_l < step_l;
+  const int wave = tid >> 6, lane = tid & 63;
+  const RowCoeffs* RC = s.RC;
+  const cdouble* U = s.U;
+  const bool constrained = CONSTRAINED && L.constrained;
+  if (constrained) {
+    if (on_l)
+      for (int j = tile.mlo + row_l; j < tile.mhi; j += step_l)
+        s.DP[(j - tile.mlo) * nl + lane_l] =
+            apply_d<SCHEME>(*s.T, U + 2 * nl + lane_l, us, tile.lo, j, n, L.inv_h);
+    __syncthreads();
+  }
+  if (on_l)
+    for (int j = tile.mlo + row_l; j < tile.mhi; j += step_l) {
+      const int jj = j - tile.mlo;
+      const RowCoeffs& rc = RC[j - s.rc_lo];
+      const cdouble* u = U + (j - tile.lo) * us;
+      const cdouble Q = constrained ? s.DP[jj * nl + lane_l] : u[nl + lane_l];
+      const double k2 = rc.adv2;
+      const cdouble P = u[lane_l];
+      if (!L.kerr) {
+        // a = 0: C_T and G_m are theta independent, the coupling is diagonal.
+        const cdouble g = cmake(rc.g_real, rc.g_imag_m);
+        const cdouble num = csub(cadd(P, cscale(k2, Q)), cmul(g, u[2 * nl + lane_l]));
+        s.PT[jj * s.pts + lane_l] = cdivr(num, rc.time_radial);
+      } else {
+        s.PT[jj * s.pts + lane_l] = cadd(P, cscale(k2, Q));
+      }
+    }
+  __syncthreads();
+  if (L.kerr) {
+    const double* Sf = a.mats + L.frag_off;
+    const double* Af = Sf + 64 * mfma_s_fragments(nt, nl);
+    const cdouble* psi = U + (tile.mlo - tile.lo) * us + 2 * nl;
+    for (int g0 = 16 * wave; g0 < mrows; g0 += 16 * (kStageThreads / 64))
+      kerr_psi_group<UB>(nl, nt, L.spin, RC + (tile.mlo - s.rc_lo), s.PT, s.pts, psi, us, g0,
+                         mrows, Sf, Af, s.NODES, lane);
+    __syncthreads();
+  }
+}
+
+// Point RHS and RK combination on the output rows [t0, t1).
+template <int SCHEME, int STAGE, int UB>
+__device__ __forceinline__ void output_phase(const StageArgs& a, const LineDesc& L,
+                                             const TileDesc& tile, const StageLds& s, int tid) {
+  const int wave = tid >> 6, lane = tid & 63;
+  const int orows = tile.t1 - tile.t0;
+  TileCtx c{&L,     &tile, s.T,       s.RC,      s.rc_lo, s.U, s.PT, s.DP, s.pts, s.inplace,
+            a.dt_override > 0.0 ? a.dt_override : L.dt, L.constrained ? 0.0 : L.gamma};
+  if (!L.gram) {
+    plain_output<SCHEME, STAGE>(a, c, tid);
+    return;
+  }
+  const double* Gf = a.mats + L.gfrag_off;
+  for (int g0 = 16 * wave; g0 < orows; g0 += 16 * (kStageThreads / 64))
+    gram_output_group<SCHEME, STAGE, UB>(a, c, Gf, g0, lane);
+}
+
 // One workgroup = one radial tile of one line; UB = ell tiles of 16 the MFMA
 // routines hold at once (1, 2 or 4).
 template <int SCHEME, int STAGE, int UB>
@@ -500,94 +594,132 @@ __global__ __launch_bounds__(kStageThreads, UB == 1 ? 3 : 2) void teuk_stage_ker
   const int tid = threadIdx.x;
   const int nl = L.n_ell;
   const int nt = L.ntheta;
-  const int n = L.nr;
   const int rows = tile.hi - tile.lo;
   const int mrows = tile.mhi - tile.mlo;
-  const int orows = tile.t1 - tile.t0;
   const int us = 3 * nl;  // LDS row stride of the staged tile (cdouble)
 
-  // (row, ell) thread decomposition: one division.
-  const int lane_l = tid % nl, row_l = tid / nl, step_l = kStageThreads / nl;
-  const bool on_l = row_l < step_l;
-  const int wave = tid >> 6, lane = tid & 63;
+  // LDS: SBP table | RC[mrows] | U[rows][P|Q|Psi] | PT[mrows] (dissipation
+  // only) | DP[mrows] (stage-constrained only) | sin, cos of the nodes (Kerr
+  // only).  Without dissipation psi_t replaces P in U (P's only other use,
+  // the stage-1 input, is read from HBM).  The SBP coefficient table lives in
+  // LDS: closure rows index it per lane, which from global memory costs a
+  // full load latency per tap.
+  StageLds s{};
+  s.inplace = !L.dissipate;
+  s.T = reinterpret_cast<const SchemeTable*>(smem);
+  s.RC = reinterpret_cast<RowCoeffs*>(smem + kSchemeTableDoubles);
+  s.rc_lo = tile.mlo;
+  s.U = reinterpret_cast<cdouble*>(s.RC + mrows);
+  s.PT = s.inplace ? s.U + (tile.mlo - tile.lo) * us : s.U + rows * us;
+  s.pts = s.inplace ? us : nl;
+  s.DP = s.inplace ? s.U + rows * us : s.PT + mrows * nl;
+  s.NODES = reinterpret_cast<double*>(L.constrained ? s.DP + mrows * nl : s.DP);
 
-  // LDS: SBP table | RC[mrows] | U[rows][P|Q|Psi] | PT[mrows] (dissipation only) |
-  // DP[mrows] (stage-constrained only) | sin, cos of the nodes (Kerr only).  Without dissipation psi_t replaces
-  // P in U (P's only other use, the stage-1 input, is read from HBM).
-  const bool inplace = !L.dissipate;
-  // The SBP coefficient table lives in LDS: closure rows index it per lane,
-  // which from global memory costs a full load latency per tap.
-  SchemeTable* TS = reinterpret_cast<SchemeTable*>(smem);
-  const SchemeTable& T = *TS;
-  RowCoeffs* RC = reinterpret_cast<RowCoeffs*>(smem + kSchemeTableDoubles);
-  cdouble* U = reinterpret_cast<cdouble*>(RC + mrows);
-  cdouble* PT = inplace ? U + (tile.mlo - tile.lo) * us : U + rows * us;
-  const int pts = inplace ? us : nl;
-  cdouble* DP = (inplace ? U + rows * us : PT + mrows * nl);
-  double* NODES = reinterpret_cast<double*>(L.constrained ? DP + mrows * nl : DP);  // sin | cos
-
-  const double M = L.mass, spin = L.spin, len = L.length;
-  // 1. stage the tile rows [lo, hi) of all three fields and the per-row
-  //    coefficients.
-  stage_copy(U, a.yin + L.base + static_cast<long long>(tile.lo) * us, rows * us, tid);
+  // 1. stage the tile rows [lo, hi) of all three fields, the tables and the
+  //    per-row coefficients.
+  stage_copy(s.U, a.yin + L.base + static_cast<long long>(tile.lo) * us, rows * us, tid);
   for (int j = tid; j < static_cast<int>(sizeof(SchemeTable) / 8); j += kStageThreads)
     smem[j] = reinterpret_cast<const double*>(a.table)[j];
   if (L.kerr)
-    for (int j = tid; j < 2 * nt; j += kStageThreads) NODES[j] = a.nodes[L.node_off + j];
-  for (int j = tid; j < mrows; j += kStageThreads) {
-    const RadialCoefficients rc =
-        radial_coefficients(M, spin, len, -2, L.m, L.h * (tile.mlo + j));
-    RC[j] = {2.0 * rc.adv, rc.principal, rc.time_radial, rc.g_real, rc.g_imag_m,
-             rc.q.re,      rc.q.im,      rc.psi.re,      rc.psi.im, 0.0};
-  }
+    for (int j = tid; j < 2 * nt; j += kStageThreads) s.NODES[j] = a.nodes[L.node_off + j];
+  row_coefficients(L, s.RC, tile.mlo, tile.mlo, tile.mhi, tid);
   __syncthreads();
 
-  // D Psi on the intermediate rows (stage-constrained Q_eff = D Psi).
-  if (L.constrained) {
-    if (on_l)
-      for (int j = tile.mlo + row_l; j < tile.mhi; j += step_l)
-        DP[(j - tile.mlo) * nl + lane_l] =
-            apply_d<SCHEME>(T, U + 2 * nl + lane_l, us, tile.lo, j, n, L.inv_h);
-    __syncthreads();
-  }
+  // 2. psi_t on rows [mlo, mhi); 3-4. the point RHS and the RK combination.
+  psi_t_phase<SCHEME, UB>(a, L, tile, s, tid);
+  output_phase<SCHEME, STAGE, UB>(a, L, tile, s, tid);
+}
 
-  // 2. psi_t on rows [mlo, mhi).  First X = P + 2K Q_eff (into PT's slots).
-  if (on_l)
-    for (int j = tile.mlo + row_l; j < tile.mhi; j += step_l) {
-      const int jj = j - tile.mlo;
-      const cdouble* u = U + (j - tile.lo) * us;
-      const cdouble Q = L.constrained ? DP[jj * nl + lane_l] : u[nl + lane_l];
-      const double k2 = RC[jj].adv2;
-      const cdouble P = u[lane_l];
-      if (!L.kerr) {
-        // a = 0: C_T and G_m are theta independent, the coupling is diagonal.
-        const cdouble g = cmake(RC[jj].g_real, RC[jj].g_imag_m);
-        const cdouble num = csub(cadd(P, cscale(k2, Q)), cmul(g, u[2 * nl + lane_l]));
-        PT[jj * pts + lane_l] = cdivr(num, RC[jj].time_radial);
-      } else {
-        PT[jj * pts + lane_l] = cadd(P, cscale(k2, Q));
+// Marching form for narrow bands (n_ell <= 16) without dissipation in
+// free-damped mode: one workgroup streams the output rows [s0, s1) of one
+// line in steps of kMarchRows rows.  The LDS buffer holds kMarchKeep +
+// kMarchRows rows; after each step its last kMarchKeep rows (with their
+// psi_t, already computed) move to the front and the next rows, requested
+// from HBM into registers one step ahead, fill the rest.  Every row is read
+// once and its psi_t computed once per segment, and the loads overlap the
+// previous step's work.
+template <int SCHEME, int STAGE>
+__global__ __launch_bounds__(kStageThreads, 2) void teuk_march_kernel(StageArgs a) {
+  extern __shared__ __align__(16) double smem[];
+  // The segment's output rows are [t0, t1) of its descriptor.
+  const TileDesc seg = a.tiles[blockIdx.x];
+  const LineDesc L = a.lines[seg.line];
+  const int tid = threadIdx.x;
+  const int nl = L.n_ell, nt = L.ntheta, n = L.nr, us = 3 * nl;
+  constexpr int B = SchemeShape<SCHEME>::B;
+  constexpr int W = SchemeShape<SCHEME>::W;
+  constexpr int kRows = kMarchKeep + kMarchRows;
+
+  StageLds s{};
+  s.inplace = true;
+  s.T = reinterpret_cast<const SchemeTable*>(smem);
+  s.RC = reinterpret_cast<RowCoeffs*>(smem + kSchemeTableDoubles);
+  s.U = reinterpret_cast<cdouble*>(s.RC + kRows);
+  s.PT = s.U + kMarchKeep * us;
+  s.pts = us;
+  s.DP = s.U;  // unused: no stage constraint
+  s.NODES = reinterpret_cast<double*>(s.U + kRows * us);
+
+  // Rows [load, load_end) are brought in, kMarchRows per step.
+  const int load_end = min(n, seg.t1 + kMarchKeep);
+  int load = max(0, seg.t0 - kMarchKeep);
+  int next_out = seg.t0;
+  const cdouble* src = a.yin + L.base;
+  cdouble pre[kMarchPrefetch];
+  auto prefetch = [&](int r0) {
+    const int count = (min(r0 + kMarchRows, load_end) - r0) * us;
+    const cdouble* p = src + static_cast<long long>(r0) * us;
+#pragma unroll
+    for (int k = 0; k < kMarchPrefetch; ++k) pre[k] = p[min(tid + k * kStageThreads, count - 1)];
+  };
+  prefetch(load);
+  for (int j = tid; j < static_cast<int>(sizeof(SchemeTable) / 8); j += kStageThreads)
+    smem[j] = reinterpret_cast<const double*>(a.table)[j];
+  if (L.kerr)
+    for (int j = tid; j < 2 * nt; j += kStageThreads) s.NODES[j] = a.nodes[L.node_off + j];
+  for (;;) {
+    const int cnt = min(kMarchRows, load_end - load);
+    // buffer row b holds global row load - kMarchKeep + b
+    TileDesc t{};
+    t.line = seg.line;
+    t.lo = load - kMarchKeep;
+    t.hi = load + cnt;
+    t.mlo = load;
+    t.mhi = load + cnt;
+    {
+      const int count = cnt * us;
+      cdouble* dst = s.U + kMarchKeep * us;
+#pragma unroll
+      for (int k = 0; k < kMarchPrefetch; ++k) {
+        const int e = tid + k * kStageThreads;
+        if (e < count) dst[e] = pre[k];
       }
     }
-  __syncthreads();
-  if (L.kerr) {
-    const double* Sf = a.mats + L.frag_off;
-    const double* Af = Sf + 64 * mfma_s_fragments(nt, nl);
-    const cdouble* psi = U + (tile.mlo - tile.lo) * us + 2 * nl;
-    for (int g0 = 16 * wave; g0 < mrows; g0 += 16 * (kStageThreads / 64))
-      kerr_psi_group<UB>(nl, nt, spin, RC, PT, pts, psi, us, g0, mrows, Sf, Af, NODES, lane);
+    s.rc_lo = t.lo;
+    row_coefficients(L, s.RC, t.lo, load, load + cnt, tid);
+    if (load + cnt < load_end) prefetch(load + cnt);
     __syncthreads();
+    psi_t_phase<SCHEME, 1, false>(a, L, t, s, tid);
+    // output rows whose stencil columns are all staged
+    const int end = load + cnt;
+    int last = end == n ? seg.t1 : min(min(seg.t1, end - B), n - W);
+    last = max(last, next_out);
+    t.t0 = next_out;
+    t.t1 = last;
+    if (t.t1 > t.t0) output_phase<SCHEME, STAGE, 1>(a, L, t, s, tid);
+    next_out = last;
+    if (next_out >= seg.t1 || end >= load_end) break;
+    __syncthreads();
+    // keep the last kMarchKeep rows (their psi_t included) at the front
+    {
+      const int count = kMarchKeep * us;
+      const cdouble* from = s.U + kMarchRows * us;
+      for (int e = tid; e < count; e += kStageThreads) s.U[e] = from[e];
+      for (int j = tid; j < kMarchKeep; j += kStageThreads) s.RC[j] = s.RC[kMarchRows + j];
+    }
+    __syncthreads();
+    load = end;
   }
-
-  // 3-4. point RHS on the output rows and the RK combination.
-  TileCtx c{&L,  &tile,   &T, RC, U, PT, DP, pts, inplace,
-            a.dt_override > 0.0 ? a.dt_override : L.dt, L.constrained ? 0.0 : L.gamma};
-  if (!L.gram) {
-    plain_output<SCHEME, STAGE>(a, c, tid);
-    return;
-  }
-  const double* Gf = a.mats + L.gfrag_off;
-  for (int g0 = 16 * wave; g0 < orows; g0 += 16 * (kStageThreads / 64))
-    gram_output_group<SCHEME, STAGE, UB>(a, c, Gf, g0, lane);
 }
 
 }  // namespace teuk
