@@ -35,7 +35,26 @@ class OracleParams(ctypes.Structure):
         ("n_targets", ctypes.c_int), ("targets", ctypes.c_int * 64),
         ("reduction_damping", ctypes.c_double), ("dissipation", ctypes.c_double),
         ("reduction", ctypes.c_int), ("scheme", ctypes.c_int), ("threads", ctypes.c_int),
+        ("source_mode", ctypes.c_int), ("source_start_time", ctypes.c_double),
+        ("constraint_tolerance", ctypes.c_double), ("required_passes", ctypes.c_int),
     ]
+
+
+class OracleActivation(ctypes.Structure):
+    _fields_ = [("active", ctypes.c_int), ("activation_time", ctypes.c_double),
+                ("consecutive_passes", ctypes.c_int),
+                ("last_eligibility_time", ctypes.c_double)]
+
+
+class OracleConstraintNorms(ctypes.Structure):
+    _fields_ = [("absolute_maximum", ctypes.c_double * 3),
+                ("normalized_maximum", ctypes.c_double * 3),
+                ("weighted_rms", ctypes.c_double * 3),
+                ("normalized_weighted", ctypes.c_double * 3),
+                ("controlling_normalized", ctypes.c_double), ("all_finite", ctypes.c_int)]
+
+
+SOURCE_MODES = {"disabled": 0, "constraint_aware": 1, "unrestricted": 2}
 
 
 def build() -> None:
@@ -64,8 +83,25 @@ def lib():
         L.oracle_init_gaussian.restype = d
         L.oracle_init_gaussian.argtypes = [p, d, d, i, i, ip, ip, dp, dp]
         L.oracle_evaluate_rhs.argtypes = [p, dp, dp]
+        L.oracle_step.restype = i
         L.oracle_step.argtypes = [p, d, d]
+        L.oracle_advance.restype = i
         L.oracle_advance.argtypes = [p, i, d, d]
+        L.oracle_get_activation.argtypes = [p, ctypes.POINTER(OracleActivation)]
+        L.oracle_set_activation.argtypes = [p, ctypes.POINTER(OracleActivation)]
+        L.oracle_constraint_norms.argtypes = [p, ctypes.POINTER(OracleConstraintNorms)]
+        L.oracle_consider_activation.restype = i
+        L.oracle_consider_activation.argtypes = [p, d]
+        L.oracle_last_stage_sources.restype = i
+        L.oracle_last_stage_sources.argtypes = [p, ip]
+        L.oracle_intermediate.restype = ctypes.c_longlong
+        L.oracle_intermediate.argtypes = [p, i, dp]
+        L.oracle_background.argtypes = [d, d, d, d, d, d, dp]
+        L.oracle_pair_source.argtypes = [d, dp, dp, dp, dp]
+        L.oracle_ghp_row.restype = i
+        L.oracle_ghp_row.argtypes = [i, i, i, i, i, i, d, d, d, dp, dp, dp]
+        L.oracle_delta_n.argtypes = [dp, dp, dp, i, d, d, d, dp]
+        L.oracle_recon_dt.argtypes = [dp, dp, dp, i, d, d, d, dp]
         L.oracle_endpoint_modal.restype = i
         L.oracle_endpoint_modal.argtypes = [p, i, dp, dp]
         L.oracle_gauss_legendre.argtypes = [i, dp, dp]
@@ -177,6 +213,55 @@ def dense_solve(matrix: np.ndarray, rhs: np.ndarray):
     return x.view(np.complex128), res
 
 
+# ---------------------------------------------------- second-order primitives
+BACKGROUND_NAMES = ("mu0", "tau0", "pi0", "psi20", "rho0", "epsilon0")
+
+
+def background(mass, spin, length, radius, cos_theta, sin_theta) -> dict:
+    out = np.zeros(12)
+    lib().oracle_background(mass, spin, length, radius, cos_theta, sin_theta, _dp(out))
+    v = out.view(np.complex128)
+    return dict(zip(BACKGROUND_NAMES, v))
+
+
+def pair_source(radius, bg: dict, fields, derivs):
+    """fields: 19 (value, tangent) complex pairs, derivs: 11 pairs.
+    Returns ((D, D_t), (T, T_t))."""
+    b = np.array([bg["mu0"], bg["tau0"], bg["pi0"]], dtype=np.complex128).view(np.float64).copy()
+    f = np.array(fields, dtype=np.complex128).reshape(19, 2).view(np.float64).copy()
+    g = np.array(derivs, dtype=np.complex128).reshape(11, 2).view(np.float64).copy()
+    out = np.zeros(8)
+    lib().oracle_pair_source(radius, _dp(b), _dp(f.reshape(-1)), _dp(g.reshape(-1)), _dp(out))
+    v = out.view(np.complex128)
+    return (v[0], v[1]), (v[2], v[3])
+
+
+def ghp_row(spin, boost, m, ell_max, x, dt, *, prime=False, kerr_spin=0.0, length=1.0,
+            radius=0.5):
+    x = _cview(x)
+    dt = _cview(dt)
+    out = np.zeros_like(x)
+    rc = lib().oracle_ghp_row(spin, boost, m, ell_max, len(x) // 2, int(prime), kerr_spin,
+                              length, radius, _dp(x), _dp(dt), _dp(out))
+    if rc < 0:
+        raise ValueError(lib().oracle_last_error().decode())
+    return out.view(np.complex128)
+
+
+def delta_n(value, dt, dr, n, radius, mass, length) -> complex:
+    a, b, c = (_cview(np.array([z])) for z in (value, dt, dr))
+    out = np.zeros(2)
+    lib().oracle_delta_n(_dp(a), _dp(b), _dp(c), n, radius, mass, length, _dp(out))
+    return complex(out[0], out[1])
+
+
+def recon_dt(value, dr, rhs, n, radius, mass, length) -> complex:
+    a, b, c = (_cview(np.array([z])) for z in (value, dr, rhs))
+    out = np.zeros(2)
+    lib().oracle_recon_dt(_dp(a), _dp(b), _dp(c), n, radius, mass, length, _dp(out))
+    return complex(out[0], out[1])
+
+
 # ------------------------------------------------------------------- pipeline
 @dataclass
 class Setup:
@@ -195,6 +280,10 @@ class Setup:
     dissipation: float = 0.0
     reduction: str = "free_damped"
     scheme: str = "d4-2"
+    source_mode: str = "disabled"
+    source_start_time: float = 0.0
+    constraint_tolerance: float = 1.0e-10
+    required_passes: int = 1
 
     @property
     def horizon(self) -> float:
@@ -234,6 +323,10 @@ class Pipeline:
         p.reduction = REDUCTIONS[setup.reduction]
         p.scheme = SCHEMES[setup.scheme]
         p.threads = threads
+        p.source_mode = SOURCE_MODES[setup.source_mode]
+        p.source_start_time = setup.source_start_time
+        p.constraint_tolerance = setup.constraint_tolerance
+        p.required_passes = setup.required_passes
         self._h = lib().oracle_create(ctypes.byref(p))
         if not self._h:
             raise ValueError(lib().oracle_last_error().decode())
@@ -274,11 +367,62 @@ class Pipeline:
         lib().oracle_evaluate_rhs(self._h, _dp(v), _dp(out))
         return out.view(np.complex128).reshape(self.shape)
 
+    def _ok(self, rc: int) -> None:
+        if rc < 0:
+            raise RuntimeError(lib().oracle_last_error().decode())
+
     def step(self, time: float, dt: float) -> None:
-        lib().oracle_step(self._h, time, dt)
+        self._ok(lib().oracle_step(self._h, time, dt))
 
     def advance(self, steps: int, dt: float, time0: float = 0.0) -> None:
-        lib().oracle_advance(self._h, steps, time0, dt)
+        self._ok(lib().oracle_advance(self._h, steps, time0, dt))
+
+    def activation(self) -> dict:
+        a = OracleActivation()
+        lib().oracle_get_activation(self._h, ctypes.byref(a))
+        return {"active": bool(a.active), "activation_time": a.activation_time,
+                "consecutive_passes": a.consecutive_passes,
+                "last_eligibility_time": a.last_eligibility_time}
+
+    def set_activation(self, active, activation_time, consecutive_passes,
+                       last_eligibility_time) -> None:
+        a = OracleActivation(int(active), activation_time, consecutive_passes,
+                             last_eligibility_time)
+        lib().oracle_set_activation(self._h, ctypes.byref(a))
+
+    def consider_activation(self, time: float) -> bool:
+        rc = lib().oracle_consider_activation(self._h, time)
+        self._ok(rc)
+        return bool(rc)
+
+    def constraint_norms(self) -> dict:
+        n = OracleConstraintNorms()
+        lib().oracle_constraint_norms(self._h, ctypes.byref(n))
+        return {"absolute_maximum": list(n.absolute_maximum),
+                "normalized_maximum": list(n.normalized_maximum),
+                "weighted_rms": list(n.weighted_rms),
+                "normalized_weighted": list(n.normalized_weighted),
+                "controlling_normalized": n.controlling_normalized,
+                "all_finite": bool(n.all_finite)}
+
+    def last_stage_sources(self):
+        flags = (ctypes.c_int * 8)()
+        n = lib().oracle_last_stage_sources(self._h, flags)
+        return [int(flags[k]) for k in range(n)]
+
+    INTERMEDIATES = {"forcing": (0, 1), "source_over_r3": (1, 1), "inner": (2, 2),
+                     "inner_tangent": (3, 2), "constraints": (4, 3), "constraint_terms": (5, 6)}
+
+    def intermediate(self, name: str) -> np.ndarray:
+        which, slots = self.INTERMEDIATES[name]
+        n = lib().oracle_intermediate(self._h, which, None)
+        if n < 0:
+            raise ValueError("intermediates exist only with the source policy enabled")
+        out = np.zeros(2 * n)
+        lib().oracle_intermediate(self._h, which, _dp(out))
+        M, _, N, T = self.shape
+        shape = (M, N, T) if slots == 1 else (M, slots, N, T)
+        return out.view(np.complex128).reshape(shape)
 
     def endpoint_modal(self, mode_index: int):
         nm = self.setup.ell_max_first + 1

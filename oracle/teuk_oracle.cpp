@@ -372,23 +372,80 @@ bool contains(const std::vector<int>& v, int x) {
 }  // namespace
 
 // ------------------------------------------------------------- pipeline
+namespace {
+
+// One angular coordinator (angular_coordinator.hpp:20-108): a (spin, boost,
+// ell_max) band over the stored modes it is active for, with the spin +- 1
+// transforms its eth / eth' operators synthesize into (ghp_device.hpp:
+// 174-196).
+struct Coord {
+  int spin = 0, boost = 0, ell_max = 0;
+  std::vector<bool> active;
+  std::vector<Transform> own, raised, lowered;
+};
+
+}  // namespace
+
 struct OraclePipeline {
   OracleParams p;
   int M = 0, N = 0, T = 0;
   double h = 0, inv_h = 0, horizon = 0;
-  std::vector<int> modes, parents, targets;
-  std::vector<double> theta;              // acos(x_j)
-  std::vector<Transform> first;           // per stored mode (empty if inactive)
+  std::vector<int> modes, parents, targets, sharp;
+  std::vector<double> theta, cos_t, sin_t;  // acos(x_j), x_j, sqrt(1 - x_j^2)
+  std::vector<Transform> first;            // per stored mode (empty if inactive)
   std::vector<bool> parent_active;
   std::vector<cplx> state, k1, k2, k3, k4, stage;
-  std::vector<cplx> lap, scratch;         // lap: M*N*T, scratch: M*5*N*T
+  std::vector<cplx> lap, scratch;          // lap: M*N*T, scratch: M*5*N*T
+  // second order (allocated when the source policy is not disabled)
+  bool second = false;
+  Coord c_first, c_second, c_g, c_b, c_pi, c_c, c_scalar, c_bsharp, c_csharp, c_t;
+  std::vector<cplx> tangent, scratch_t, rdr, rdr_t, rang, rang_t, extra, sharpw;
+  std::vector<cplx> constraints, terms, inner, inner_t, proj, proj_t, ethprime_t;
+  std::vector<cplx> source_over_r3, forcing, zero_plane;
+  std::vector<std::pair<int, int>> pair_list;   // (index1, index2) per ordered pair
+  std::vector<int> target_slots, pair_offsets;  // target mode index, pair range
+  // activation latch (spatial_pipeline.hpp:386-424, 1225-1289)
+  OracleActivation act{};
+  OracleConstraintNorms norms{};
+  int stage_sources[8] = {0};
+  int stage_source_count = 0;
 
   size_t idx(int mode, int field, int r, int t) const {
     return ((static_cast<size_t>(mode) * kFields + field) * N + r) * T + t;
   }
+  size_t plane(int mode, int r, int t) const {
+    return (static_cast<size_t>(mode) * N + r) * T + t;
+  }
 };
 
 namespace {
+
+// A (mode, radial, theta) plane inside a larger array: element (mode, r, t)
+// at base[mode * ms + r * T + t].
+struct PlaneW {
+  cplx* b;
+  size_t ms;
+};
+struct Plane {
+  const cplx* b;
+  size_t ms;
+  Plane(const cplx* base, size_t stride) : b(base), ms(stride) {}
+  Plane(PlaneW w) : b(w.b), ms(w.ms) {}
+};
+
+Plane field_of(const OraclePipeline* p, const cplx* arr, int field) {
+  return {arr + static_cast<size_t>(field) * p->N * p->T, static_cast<size_t>(kFields) * p->N * p->T};
+}
+PlaneW field_of(const OraclePipeline* p, cplx* arr, int field) {
+  return {arr + static_cast<size_t>(field) * p->N * p->T, static_cast<size_t>(kFields) * p->N * p->T};
+}
+Plane slot_of(const OraclePipeline* p, const cplx* arr, int slots, int slot) {
+  return {arr + static_cast<size_t>(slot) * p->N * p->T, static_cast<size_t>(slots) * p->N * p->T};
+}
+PlaneW slot_of(const OraclePipeline* p, cplx* arr, int slots, int slot) {
+  return {arr + static_cast<size_t>(slot) * p->N * p->T, static_cast<size_t>(slots) * p->N * p->T};
+}
+Plane ro(PlaneW w) { return {w.b, w.ms}; }
 
 void set_threads(const OraclePipeline* p) {
 #ifdef _OPENMP
@@ -398,170 +455,358 @@ void set_threads(const OraclePipeline* p) {
 #endif
 }
 
-// Angular Laplacian S*Lambda*A of FirstPsi per active mode, zero otherwise
-// (angular_coordinator.hpp:151-174, angular_device.hpp:177-243).
-void angular_laplacian(OraclePipeline* p, const cplx* stage) {
+Coord make_coord(const OraclePipeline* p, int spin, int boost, int E,
+                 const std::vector<int>& active_modes, bool ghp) {
+  Coord c;
+  c.spin = spin;
+  c.boost = boost;
+  c.ell_max = E;
+  c.active.assign(p->M, false);
+  c.own.resize(p->M);
+  c.raised.resize(p->M);
+  c.lowered.resize(p->M);
+  for (int i = 0; i < p->M; ++i) {
+    if (!contains(active_modes, p->modes[i])) continue;
+    c.active[i] = true;
+    c.own[i] = make_transform(spin, p->modes[i], E, p->T);
+    if (ghp) {
+      c.raised[i] = make_transform(spin + 1, p->modes[i], E, p->T);
+      c.lowered[i] = make_transform(spin - 1, p->modes[i], E, p->T);
+    }
+  }
+  return c;
+}
+
+void analyze(const Transform& tr, const cplx* in, int T, std::vector<cplx>& modal) {
+  const int nm = tr.modes();
+  modal.assign(nm, 0.0);
+  for (int k = 0; k < nm; ++k) {
+    cplx v(0.0, 0.0);
+    for (int t = 0; t < T; ++t) v += tr.A[k * T + t] * in[t];
+    modal[k] = v;
+  }
+}
+
+void synthesize(const Transform& tr, const std::vector<cplx>& modal, int T, cplx* out) {
+  const int nm = tr.modes();
+  for (int t = 0; t < T; ++t) {
+    cplx v(0.0, 0.0);
+    for (int k = 0; k < nm; ++k) v += tr.S[t * nm + k] * modal[k];
+    out[t] = v;
+  }
+}
+
+// Band projection S A (angular_coordinator.hpp:125-146); inactive modes zero.
+void project(const OraclePipeline* p, const Coord& c, Plane in, PlaneW out) {
   const int N = p->N, T = p->T;
 #pragma omp parallel for collapse(2) schedule(static)
-  for (int mode = 0; mode < p->M; ++mode) {
+  for (int mode = 0; mode < p->M; ++mode)
     for (int r = 0; r < N; ++r) {
-      cplx* out = &p->lap[(static_cast<size_t>(mode) * N + r) * T];
-      if (!p->parent_active[mode]) {
-        for (int t = 0; t < T; ++t) out[t] = 0.0;
+      cplx* o = out.b + mode * out.ms + static_cast<size_t>(r) * T;
+      if (!c.active[mode]) {
+        for (int t = 0; t < T; ++t) o[t] = 0.0;
         continue;
       }
-      const Transform& tr = p->first[mode];
-      const int nm = tr.modes();
-      const cplx* in = &stage[p->idx(mode, 2, r, 0)];
-      std::vector<cplx> modal(nm), op(nm);
-      for (int k = 0; k < nm; ++k) {
-        cplx v(0.0, 0.0);
-        for (int t = 0; t < T; ++t) v += tr.A[k * T + t] * in[t];
-        modal[k] = v;
+      std::vector<cplx> modal;
+      analyze(c.own[mode], in.b + mode * in.ms + static_cast<size_t>(r) * T, T, modal);
+      std::vector<cplx> tmp(T);
+      synthesize(c.own[mode], modal, T, tmp.data());
+      for (int t = 0; t < T; ++t) o[t] = tmp[t];
+    }
+}
+
+// S Lambda A (angular_coordinator.hpp:151-174, angular.hpp:139-165).
+void laplacian(const OraclePipeline* p, const Coord& c, Plane in, PlaneW out) {
+  const int N = p->N, T = p->T;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int mode = 0; mode < p->M; ++mode)
+    for (int r = 0; r < N; ++r) {
+      cplx* o = out.b + mode * out.ms + static_cast<size_t>(r) * T;
+      if (!c.active[mode]) {
+        for (int t = 0; t < T; ++t) o[t] = 0.0;
+        continue;
       }
-      for (int k = 0; k < nm; ++k) {
+      const Transform& tr = c.own[mode];
+      std::vector<cplx> modal;
+      analyze(tr, in.b + mode * in.ms + static_cast<size_t>(r) * T, T, modal);
+      for (int k = 0; k < tr.modes(); ++k) {
         const int l = tr.ell_min + k;
-        const double lam = -static_cast<double>((l - tr.spin) * (l + tr.spin + 1));
-        op[k] = lam * modal[k];
+        modal[k] *= -static_cast<double>((l - tr.spin) * (l + tr.spin + 1));
       }
-      for (int t = 0; t < T; ++t) {
-        cplx v(0.0, 0.0);
-        for (int k = 0; k < nm; ++k) v += tr.S[t * nm + k] * op[k];
-        out[t] = v;
-      }
+      synthesize(tr, modal, T, o);
     }
-  }
 }
 
-// Band projection S*A of one output field per active mode
-// (angular_coordinator.hpp:125-146).
-void project_field(OraclePipeline* p, cplx* out, int field) {
+// eth_n X and eth'_n X (ghp.hpp:60-116) with the angular part from the
+// spin-raised (lowered) band: analyze, multiply by the raising factor
+// sqrt((l - s)(l + s + 1)) (lowering factor -sqrt((l + s)(l - s + 1))),
+// align ell to the shifted band, synthesize (ghp_device.hpp:174-196,
+// angular.hpp:129-137).
+void ghp(const OraclePipeline* p, const Coord& c, bool prime, Plane x, Plane dt, PlaneW out) {
   const int N = p->N, T = p->T;
+  const double a = p->p.spin, L = p->p.length, L2 = L * L;
+  const double sqrt_two = std::sqrt(2.0);
+  const cplx I(0.0, 1.0);
 #pragma omp parallel for collapse(2) schedule(static)
-  for (int mode = 0; mode < p->M; ++mode) {
+  for (int mode = 0; mode < p->M; ++mode)
     for (int r = 0; r < N; ++r) {
-      cplx* v = &out[p->idx(mode, field, r, 0)];
-      if (!p->parent_active[mode]) {
-        for (int t = 0; t < T; ++t) v[t] = 0.0;
+      cplx* o = out.b + mode * out.ms + static_cast<size_t>(r) * T;
+      if (!c.active[mode]) {
+        for (int t = 0; t < T; ++t) o[t] = 0.0;
         continue;
       }
-      const Transform& tr = p->first[mode];
-      const int nm = tr.modes();
-      std::vector<cplx> modal(nm);
-      for (int k = 0; k < nm; ++k) {
-        cplx s(0.0, 0.0);
-        for (int t = 0; t < T; ++t) s += tr.A[k * T + t] * v[t];
-        modal[k] = s;
+      const Transform& src = c.own[mode];
+      const Transform& dst = prime ? c.lowered[mode] : c.raised[mode];
+      const cplx* xv = x.b + mode * x.ms + static_cast<size_t>(r) * T;
+      const cplx* dv = dt.b + mode * dt.ms + static_cast<size_t>(r) * T;
+      std::vector<cplx> modal;
+      analyze(src, xv, T, modal);
+      std::vector<cplx> shifted(dst.modes(), cplx(0.0, 0.0));
+      for (int j = 0; j < dst.modes(); ++j) {
+        const int l = dst.ell_min + j;
+        const int k = l - src.ell_min;
+        if (k < 0 || k >= src.modes()) continue;
+        const double f =
+            prime ? -std::sqrt(static_cast<double>((l + c.spin) * (l - c.spin + 1)))
+                  : std::sqrt(static_cast<double>((l - c.spin) * (l + c.spin + 1)));
+        shifted[j] = f * modal[k];
       }
+      std::vector<cplx> ang(T);
+      synthesize(dst, shifted, T, ang.data());
+      const double R = p->h * r;
       for (int t = 0; t < T; ++t) {
-        cplx s(0.0, 0.0);
-        for (int k = 0; k < nm; ++k) s += tr.S[t * nm + k] * modal[k];
-        v[t] = s;
+        const double st = p->sin_t[t], ct = p->cos_t[t];
+        if (!prime) {
+          const int pw = c.spin + c.boost;
+          const cplx den(L2, -a * R * ct);
+          const cplx deriv = (-I * a * st * dv[t] + ang[t]) / (sqrt_two * den);
+          const cplx conn = -I * static_cast<double>(pw) * a * R * st * xv[t] /
+                            (sqrt_two * den * den);
+          o[t] = deriv + conn;
+        } else {
+          const int qw = -c.spin + c.boost;
+          const cplx den(L2, a * R * ct);
+          const cplx deriv = (I * a * st * dv[t] + ang[t]) / (sqrt_two * den);
+          const cplx conn = I * static_cast<double>(qw) * a * R * st * xv[t] /
+                            (sqrt_two * den * den);
+          o[t] = deriv + conn;
+        }
       }
     }
-  }
 }
 
-// Three-pass full-grid Teukolsky RHS for fields 0..2
-// (full_spatial.hpp:99-280, 303-398), then projection and zeroing of fields
-// 3..12 (spatial_pipeline.hpp:483-515, 623-649).
-void evaluate_rhs(OraclePipeline* p, const cplx* stage, cplx* out) {
+// SBP radial derivative of one plane (radial_discretization.hpp:130-144).
+void radial_d(const OraclePipeline* p, Plane in, PlaneW out) {
+  const int N = p->N, T = p->T;
+#pragma omp parallel for collapse(3) schedule(static)
+  for (int mode = 0; mode < p->M; ++mode)
+    for (int r = 0; r < N; ++r)
+      for (int t = 0; t < T; ++t)
+        out.b[mode * out.ms + static_cast<size_t>(r) * T + t] =
+            derivative_at(p->p.scheme, in.b + mode * in.ms + t, N, r, p->inv_h, T);
+}
+
+// Rescaled rotated-Kinnersley background (background.hpp:29-78).
+struct Background {
+  cplx mu0, tau0, pi0, psi20, rho0, epsilon0;
+};
+
+Background kerr_background(double M, double a, double L, double R, double ct, double st) {
+  const double L2 = L * L, L4 = L2 * L2;
+  const cplx I(0.0, 1.0);
+  const double sqrt_two = std::sqrt(2.0);
+  const cplx minus_radial(-L2, a * R * ct);
+  const cplx angular(L2, -a * R * ct);
+  const cplx angular2 = angular * angular;
+  const double real_pi = L4 + a * a * R * R * ct * ct;
+  Background b;
+  b.mu0 = cplx(1.0, 0.0) / minus_radial;
+  b.tau0 = I * a * st / (sqrt_two * angular2);
+  b.pi0 = -I * a * st / (sqrt_two * real_pi);
+  const cplx ingoing(L2, a * R * ct);
+  const cplx optical = angular2 * ingoing;
+  b.rho0 = -0.5 * (L4 - 2.0 * L2 * M * R + a * a * R * R) / optical;
+  b.epsilon0 = 0.5 * cplx(L2 * M - a * a * R, -a * (L2 - M * R) * ct) / optical;
+  b.psi20 = -M / (angular2 * angular);
+  return b;
+}
+
+Background background_at(const OraclePipeline* p, int r, int t) {
+  return kerr_background(p->p.mass, p->p.spin, p->p.length, p->h * r, p->cos_t[t], p->sin_t[t]);
+}
+
+// Background at the theta coordinate itself (the reconstruction passes call
+// kerr_background_point(parameters, radius, theta), full_spatial.hpp:518).
+Background background_theta(const OraclePipeline* p, int r, int t) {
+  return kerr_background(p->p.mass, p->p.spin, p->p.length, p->h * r, std::cos(p->theta[t]),
+                         std::sin(p->theta[t]));
+}
+
+// Delta_n from point values (ghp.hpp:11-22).
+cplx delta_n(cplx v, cplx dt, cplx dr, int n, double R, double M, double L) {
+  const double L2 = L * L;
+  const double tc = 2.0 + 4.0 * M * R / L2;
+  const double rc = R / L2;
+  return tc * dt + rc * (R * dr + static_cast<double>(n) * v);
+}
+
+// Solve Delta_n X = R_X for the time derivative (reconstruction.hpp:108-121).
+cplx recon_dt(cplx v, cplx dr, cplx rhs, int n, double R, double M, double L) {
+  const double L2 = L * L;
+  const double den = 2.0 + 4.0 * M * R / L2;
+  const cplx radial = (R * R / L2) * dr + (static_cast<double>(n) * R / L2) * v;
+  return (rhs - radial) / den;
+}
+
+// thorn_n (ghp.hpp:24-58).
+cplx thorn_n(cplx v, cplx dt, cplx dr, int n, int sw, int bw, int m, double R, double ct,
+             double M, double a, double L, cplx eps0) {
+  const cplx I(0.0, 1.0);
+  const double L2 = L * L, L4 = L2 * L2;
+  const double den = L4 + a * a * R * R * ct * ct;
+  const double tc = R * 2.0 * M * (2.0 * M - a * a * R / L2) / den;
+  const double rc = -0.5 * (L2 - 2.0 * M * R + a * a * R * R / L2) / den;
+  const int pw = sw + bw, qw = -sw + bw;
+  return tc * dt + rc * (R * dr + static_cast<double>(n) * v) +
+         R * I * a * static_cast<double>(m) * v / den -
+         R * (static_cast<double>(pw) * eps0 + static_cast<double>(qw) * std::conj(eps0)) * v;
+}
+
+// Three-pass full-grid Teukolsky RHS of one (P, Q, Psi) triple
+// (full_spatial.hpp:99-280, 303-398): stage fields s0..s0+2 -> out fields
+// o0..o0+2, Laplacian through `coord`, optional forcing, scratch slots
+// (D Psi, Q_eff, D Q_eff, psi_t, D psi_t).  The caller projects.
+void linear_rhs(OraclePipeline* p, const cplx* stage, int s0, cplx* out, int o0, const Coord& coord,
+                const cplx* forcing, std::vector<cplx>& scratch) {
   const int M = p->M, N = p->N, T = p->T;
   const int scheme = p->p.scheme;
-  const size_t sstride = T;  // radial stride within one (mode, field)
+  const size_t sstride = T;
   enum { DPSI = 0, QEFF = 1, DQ = 2, PSIT = 3, DPSIT = 4 };
   auto sc = [&](int mode, int slot, int r, int t) -> cplx& {
-    return p->scratch[((static_cast<size_t>(mode) * 5 + slot) * N + r) * T + t];
+    return scratch[((static_cast<size_t>(mode) * 5 + slot) * N + r) * T + t];
   };
-  angular_laplacian(p, stage);
+  laplacian(p, coord, field_of(p, stage, s0 + 2), PlaneW{p->lap.data(), static_cast<size_t>(N) * T});
   const bool constrained = p->p.reduction == 1;
-  // pass 1: D psi and effective Q
 #pragma omp parallel for collapse(3) schedule(static)
   for (int mode = 0; mode < M; ++mode)
     for (int r = 0; r < N; ++r)
       for (int t = 0; t < T; ++t) {
-        const cplx d = derivative_at(scheme, &stage[p->idx(mode, 2, 0, t)], N, r,
-                                     p->inv_h, sstride);
+        const cplx d = derivative_at(scheme, &stage[p->idx(mode, s0 + 2, 0, t)], N, r, p->inv_h,
+                                     sstride);
         sc(mode, DPSI, r, t) = d;
-        sc(mode, QEFF, r, t) = constrained ? d : stage[p->idx(mode, 1, r, t)];
+        sc(mode, QEFF, r, t) = constrained ? d : stage[p->idx(mode, s0 + 1, r, t)];
       }
-  // pass 2: D Q_eff and psi_t
 #pragma omp parallel for collapse(3) schedule(static)
   for (int mode = 0; mode < M; ++mode)
     for (int r = 0; r < N; ++r)
       for (int t = 0; t < T; ++t) {
-        sc(mode, DQ, r, t) = derivative_at(scheme, &sc(mode, QEFF, 0, t), N, r,
-                                           p->inv_h, sstride);
-        const Coeffs c = teukolsky_coefficients(
-            p->p.mass, p->p.spin, p->p.length, -2, p->modes[mode], p->h * r,
-            p->theta[t]);
-        const cplx P = stage[p->idx(mode, 0, r, t)];
+        sc(mode, DQ, r, t) = derivative_at(scheme, &sc(mode, QEFF, 0, t), N, r, p->inv_h, sstride);
+        const Coeffs c = teukolsky_coefficients(p->p.mass, p->p.spin, p->p.length, -2,
+                                                p->modes[mode], p->h * r, p->theta[t]);
+        const cplx P = stage[p->idx(mode, s0, r, t)];
         const cplx Q = sc(mode, QEFF, r, t);
-        const cplx Psi = stage[p->idx(mode, 2, r, t)];
+        const cplx Psi = stage[p->idx(mode, s0 + 2, r, t)];
         sc(mode, PSIT, r, t) = (P + 2.0 * c.adv * Q - c.definition * Psi) / c.time;
       }
-  // pass 3: D psi_t and the three outputs
   const double damping = constrained ? 0.0 : p->p.reduction_damping;
 #pragma omp parallel for collapse(3) schedule(static)
   for (int mode = 0; mode < M; ++mode)
     for (int r = 0; r < N; ++r)
       for (int t = 0; t < T; ++t) {
-        const cplx dpt = derivative_at(scheme, &sc(mode, PSIT, 0, t), N, r,
-                                       p->inv_h, sstride);
+        const cplx dpt = derivative_at(scheme, &sc(mode, PSIT, 0, t), N, r, p->inv_h, sstride);
         sc(mode, DPSIT, r, t) = dpt;
-        const Coeffs c = teukolsky_coefficients(
-            p->p.mass, p->p.spin, p->p.length, -2, p->modes[mode], p->h * r,
-            p->theta[t]);
-        const cplx P = stage[p->idx(mode, 0, r, t)];
+        const Coeffs c = teukolsky_coefficients(p->p.mass, p->p.spin, p->p.length, -2,
+                                                p->modes[mode], p->h * r, p->theta[t]);
         const cplx Qe = sc(mode, QEFF, r, t);
-        const cplx Psi = stage[p->idx(mode, 2, r, t)];
-        const cplx constraint = stage[p->idx(mode, 1, r, t)] - sc(mode, DPSI, r, t);
-        const cplx lapv = p->lap[(static_cast<size_t>(mode) * N + r) * T + t];
-        const cplx zero(0.0, 0.0);
-        cplx Pt = c.principal * sc(mode, DQ, r, t) + c.q * Qe + c.psi * Psi + lapv + zero;
-        (void)P;
+        const cplx Psi = stage[p->idx(mode, s0 + 2, r, t)];
+        const cplx constraint = stage[p->idx(mode, s0 + 1, r, t)] - sc(mode, DPSI, r, t);
+        const cplx lapv = p->lap[p->plane(mode, r, t)];
+        const cplx frc = forcing ? forcing[p->plane(mode, r, t)] : cplx(0.0, 0.0);
+        cplx Pt = c.principal * sc(mode, DQ, r, t) + c.q * Qe + c.psi * Psi + lapv + frc;
         cplx Qt = dpt - damping * constraint;
         cplx St = sc(mode, PSIT, r, t);
         if (p->p.dissipation > 0.0) {
-          Pt += dissipation_at(scheme, &stage[p->idx(mode, 0, 0, t)], N, r, p->h,
+          Pt += dissipation_at(scheme, &stage[p->idx(mode, s0, 0, t)], N, r, p->h,
                                p->p.dissipation, sstride);
-          Qt += dissipation_at(scheme, &stage[p->idx(mode, 1, 0, t)], N, r, p->h,
+          Qt += dissipation_at(scheme, &stage[p->idx(mode, s0 + 1, 0, t)], N, r, p->h,
                                p->p.dissipation, sstride);
-          St += dissipation_at(scheme, &stage[p->idx(mode, 2, 0, t)], N, r, p->h,
+          St += dissipation_at(scheme, &stage[p->idx(mode, s0 + 2, 0, t)], N, r, p->h,
                                p->p.dissipation, sstride);
         }
-        out[p->idx(mode, 0, r, t)] = Pt;
-        out[p->idx(mode, 1, r, t)] = Qt;
-        out[p->idx(mode, 2, r, t)] = St;
+        out[p->idx(mode, o0, r, t)] = Pt;
+        out[p->idx(mode, o0 + 1, r, t)] = Qt;
+        out[p->idx(mode, o0 + 2, r, t)] = St;
       }
-  project_field(p, out, 0);
-  project_field(p, out, 1);
-  project_field(p, out, 2);
-#pragma omp parallel for collapse(2) schedule(static)
-  for (int mode = 0; mode < M; ++mode)
-    for (int f = 3; f < kFields; ++f)
-      std::fill(&out[p->idx(mode, f, 0, 0)], &out[p->idx(mode, f, 0, 0)] + static_cast<size_t>(N) * T,
-                cplx(0.0, 0.0));
 }
 
-// Classical RK4 (device_rk4.hpp:84-128).
+void project_triple(OraclePipeline* p, cplx* out, int o0, const Coord& c) {
+  for (int f = o0; f < o0 + 3; ++f) project(p, c, field_of(p, out, f), field_of(p, out, f));
+}
+
+void evaluate_rhs(OraclePipeline* p, const cplx* stage, cplx* out);
+
+#include "second_order.inc"
+
+// Full RHS (spatial_pipeline.hpp:458-588): the first-order triple; with the
+// source policy disabled, fields 3..12 are zero (623-649); otherwise the
+// reconstruction chain, its tangent, the quadratic source and the
+// second-order triple.
+void evaluate_rhs(OraclePipeline* p, const cplx* stage, cplx* out) {
+  linear_rhs(p, stage, 0, out, 0, p->c_first, nullptr, p->scratch);
+  project_triple(p, out, 0, p->c_first);
+  if (!p->second) {
+    const int M = p->M, N = p->N, T = p->T;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int mode = 0; mode < M; ++mode)
+      for (int f = 3; f < kFields; ++f)
+        std::fill(&out[p->idx(mode, f, 0, 0)],
+                  &out[p->idx(mode, f, 0, 0)] + static_cast<size_t>(N) * T, cplx(0.0, 0.0));
+    return;
+  }
+  second_order_rhs(p, stage, out);
+}
+
+// Classical RK4 over the whole 13-field state (device_rk4.hpp:84-128); the
+// stage source flags record the latch seen by each stage.
 void rk4_step(OraclePipeline* p, double dt) {
   const size_t n = p->state.size();
   cplx* y = p->state.data();
-  evaluate_rhs(p, y, p->k1.data());
+  auto rhs = [&](const cplx* s, cplx* k) {
+    if (p->stage_source_count < 8) p->stage_sources[p->stage_source_count++] = p->act.active;
+    evaluate_rhs(p, s, k);
+  };
+  rhs(y, p->k1.data());
 #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < n; ++i) p->stage[i] = y[i] + (0.5 * dt) * p->k1[i];
-  evaluate_rhs(p, p->stage.data(), p->k2.data());
+  rhs(p->stage.data(), p->k2.data());
 #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < n; ++i) p->stage[i] = y[i] + (0.5 * dt) * p->k2[i];
-  evaluate_rhs(p, p->stage.data(), p->k3.data());
+  rhs(p->stage.data(), p->k3.data());
 #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < n; ++i) p->stage[i] = y[i] + dt * p->k3[i];
-  evaluate_rhs(p, p->stage.data(), p->k4.data());
+  rhs(p->stage.data(), p->k4.data());
   const double scale = dt / 6.0;
 #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < n; ++i)
     y[i] += scale * (p->k1[i] + 2.0 * p->k2[i] + 2.0 * p->k3[i] + p->k4[i]);
+}
+
+// SpatialPipeline::step (spatial_pipeline.hpp:590-620): the RK4 step split
+// exactly at the source start time, with the latch considered on accepted
+// states only.
+void pipeline_step(OraclePipeline* p, double time, double dt) {
+  p->stage_source_count = 0;
+  const double end = time + dt;
+  const double start = p->p.source_start_time;
+  if (!p->act.active && time >= start) consider_activation(p, time);
+  if (!p->act.active && time < start && start < end) {
+    if (start - time > 0.0) rk4_step(p, start - time);
+    consider_activation(p, start);
+    if (end - start > 0.0) rk4_step(p, end - start);
+  } else {
+    rk4_step(p, dt);
+  }
+  if (!p->act.active && end >= start) consider_activation(p, end);
 }
 
 }  // namespace
@@ -596,6 +841,9 @@ OraclePipeline* oracle_create(const OracleParams* params) {
     gauss_legendre(q.ntheta, x, w);
     p->theta.resize(q.ntheta);
     for (int t = 0; t < q.ntheta; ++t) p->theta[t] = std::acos(x[t]);
+    p->cos_t = x;
+    p->sin_t.resize(q.ntheta);
+    for (int t = 0; t < q.ntheta; ++t) p->sin_t[t] = std::sqrt(1.0 - x[t] * x[t]);
     p->first.resize(p->M);
     p->parent_active.assign(p->M, false);
     for (int i = 0; i < p->M; ++i) {
@@ -604,6 +852,75 @@ OraclePipeline* oracle_create(const OracleParams* params) {
         p->first[i] = make_transform(-2, p->modes[i], q.ell_max_first, q.ntheta);
       }
     }
+    if (q.source_mode < 0 || q.source_mode > 2 || !std::isfinite(q.source_start_time) ||
+        q.source_start_time < 0.0 || !std::isfinite(q.constraint_tolerance) ||
+        q.constraint_tolerance < 0.0 || q.required_passes < 1)
+      throw std::invalid_argument("source policy is invalid");
+    p->second = q.source_mode != 0;
+    p->c_first = make_coord(p, -2, -2, q.ell_max_first, p->parents, p->second);
+    p->sharp.assign(p->M, -1);
+    for (int i = 0; i < p->M; ++i) {
+      const auto it = std::find(p->modes.begin(), p->modes.end(), -p->modes[i]);
+      if (it != p->modes.end()) p->sharp[i] = static_cast<int>(it - p->modes.begin());
+    }
+    if (p->second) {
+      for (int i = 0; i < p->M; ++i)
+        if (p->sharp[i] < 0) throw std::invalid_argument("modes must be closed under m -> -m");
+      const int E1 = q.ell_max_first, E2 = q.ell_max_second;
+      if (E1 < 3 || E2 < 3 || q.ntheta < std::max({E1 + 1, E2 + 1, (2 * E1 + E2 + 2) / 2}))
+        throw std::invalid_argument("angular bands require exact-product padding");
+      // Coordinators of spatial_pipeline.hpp:185-218.
+      p->c_second = make_coord(p, -2, -2, E2, p->targets, false);
+      p->c_g = make_coord(p, -1, -1, E1, p->parents, true);
+      p->c_b = make_coord(p, -2, 0, E1, p->parents, true);
+      p->c_pi = make_coord(p, -1, 0, E1, p->parents, true);
+      p->c_c = make_coord(p, -1, 1, E1, p->parents, true);
+      p->c_scalar = make_coord(p, 0, 0, E1, p->parents, true);
+      p->c_bsharp = make_coord(p, 2, 0, E1, p->parents, true);
+      p->c_csharp = make_coord(p, 1, 1, E1, p->parents, true);
+      p->c_t = make_coord(p, -1, -2, E2, p->targets, true);
+      // Ordered pairs (m1, m2) -> m1 + m2, target-major then m1 ascending,
+      // both factors parents (modes.hpp:32-213).
+      std::vector<int> sorted_targets = p->targets;
+      std::sort(sorted_targets.begin(), sorted_targets.end());
+      p->pair_offsets.assign(1, 0);
+      for (int m : sorted_targets) {
+        const int ti = static_cast<int>(std::find(p->modes.begin(), p->modes.end(), m) - p->modes.begin());
+        p->target_slots.push_back(ti);
+        std::vector<int> sp = p->parents;
+        std::sort(sp.begin(), sp.end());
+        for (int m1 : sp) {
+          const int m2 = m - m1;
+          if (!contains(p->parents, m2)) continue;
+          const int i1 = static_cast<int>(std::find(p->modes.begin(), p->modes.end(), m1) - p->modes.begin());
+          const int i2 = static_cast<int>(std::find(p->modes.begin(), p->modes.end(), m2) - p->modes.begin());
+          p->pair_list.push_back({i1, i2});
+        }
+        p->pair_offsets.push_back(static_cast<int>(p->pair_list.size()));
+      }
+      const size_t plane = static_cast<size_t>(p->M) * p->N * p->T;
+      const size_t full = plane * kFields;
+      p->tangent.assign(full, 0.0);
+      p->scratch_t.assign(plane * 5, 0.0);
+      p->rdr.assign(plane * 7, 0.0);
+      p->rdr_t.assign(plane * 7, 0.0);
+      p->rang.assign(plane * 6, 0.0);
+      p->rang_t.assign(plane * 6, 0.0);
+      p->extra.assign(plane * 4, 0.0);
+      p->sharpw.assign(plane * 4, 0.0);
+      p->constraints.assign(plane * 3, 0.0);
+      p->terms.assign(plane * 6, 0.0);
+      p->inner.assign(plane * 2, 0.0);
+      p->inner_t.assign(plane * 2, 0.0);
+      p->proj.assign(plane * 2, 0.0);
+      p->proj_t.assign(plane * 2, 0.0);
+      p->ethprime_t.assign(plane, 0.0);
+      p->source_over_r3.assign(plane, 0.0);
+      p->forcing.assign(plane, 0.0);
+    }
+    p->act = OracleActivation{0, -1.0, 0, -1.0};
+    if (q.source_mode == 2 && q.source_start_time == 0.0)
+      p->act = OracleActivation{1, 0.0, q.required_passes, -1.0};
     const size_t count = static_cast<size_t>(p->M) * kFields * p->N * p->T;
     p->state.assign(count, 0.0);
     p->k1.assign(count, 0.0);
@@ -751,16 +1068,65 @@ void oracle_evaluate_rhs(OraclePipeline* p, const double* stage, double* out) {
   evaluate_rhs(p, reinterpret_cast<const cplx*>(stage), reinterpret_cast<cplx*>(out));
 }
 
-void oracle_step(OraclePipeline* p, double time, double dt) {
-  (void)time;
-  set_threads(p);
-  rk4_step(p, dt);
+int oracle_step(OraclePipeline* p, double time, double dt) {
+  try {
+    set_threads(p);
+    pipeline_step(p, time, dt);
+    return 0;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return -1;
+  }
 }
 
-void oracle_advance(OraclePipeline* p, int steps, double time0, double dt) {
-  (void)time0;
-  set_threads(p);
-  for (int s = 0; s < steps; ++s) rk4_step(p, dt);
+int oracle_advance(OraclePipeline* p, int steps, double time0, double dt) {
+  try {
+    set_threads(p);
+    for (int s = 0; s < steps; ++s) pipeline_step(p, time0 + s * dt, dt);
+    return 0;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return -1;
+  }
+}
+
+void oracle_get_activation(const OraclePipeline* p, OracleActivation* out) { *out = p->act; }
+
+void oracle_set_activation(OraclePipeline* p, const OracleActivation* in) { p->act = *in; }
+
+void oracle_constraint_norms(const OraclePipeline* p, OracleConstraintNorms* out) {
+  *out = p->norms;
+}
+
+int oracle_consider_activation(OraclePipeline* p, double time) {
+  try {
+    set_threads(p);
+    return consider_activation(p, time) ? 1 : 0;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return -1;
+  }
+}
+
+int oracle_last_stage_sources(const OraclePipeline* p, int* flags) {
+  for (int i = 0; i < p->stage_source_count; ++i) flags[i] = p->stage_sources[i];
+  return p->stage_source_count;
+}
+
+long long oracle_intermediate(const OraclePipeline* p, int which, double* out) {
+  if (!p->second) return -1;
+  const std::vector<cplx>* v = nullptr;
+  switch (which) {
+    case 0: v = &p->forcing; break;
+    case 1: v = &p->source_over_r3; break;
+    case 2: v = &p->proj; break;
+    case 3: v = &p->proj_t; break;
+    case 4: v = &p->constraints; break;
+    case 5: v = &p->terms; break;
+    default: return -1;
+  }
+  if (out) std::memcpy(static_cast<void*>(out), static_cast<const void*>(v->data()), v->size() * sizeof(cplx));
+  return static_cast<long long>(v->size());
 }
 
 int oracle_endpoint_modal(const OraclePipeline* p, int mode_index, double* scri_out,
@@ -885,6 +1251,94 @@ double oracle_dense_solve(int n, const double* matrix, const double* rhs, double
   if (!dense_solve(n, m, b, x)) return -1.0;
   std::copy(x.begin(), x.end(), reinterpret_cast<cplx*>(solution));
   return dense_residual(n, m, x, b);
+}
+
+void oracle_background(double mass, double spin, double length, double radius,
+                       double cos_theta, double sin_theta, double* out) {
+  const Background b = kerr_background(mass, spin, length, radius, cos_theta, sin_theta);
+  const cplx v[6] = {b.mu0, b.tau0, b.pi0, b.psi20, b.rho0, b.epsilon0};
+  for (int k = 0; k < 6; ++k) {
+    out[2 * k] = v[k].real();
+    out[2 * k + 1] = v[k].imag();
+  }
+}
+
+void oracle_pair_source(double radius, const double* bg, const double* fields,
+                        const double* derivs, double* out) {
+  Background b{};
+  b.mu0 = cplx(bg[0], bg[1]);
+  b.tau0 = cplx(bg[2], bg[3]);
+  b.pi0 = cplx(bg[4], bg[5]);
+  auto jet = [](const double* v, int k) {
+    return Jet{cplx(v[4 * k], v[4 * k + 1]), cplx(v[4 * k + 2], v[4 * k + 3])};
+  };
+  PairFields f;
+  Jet* fp = &f.F1;
+  for (int k = 0; k < 19; ++k) fp[k] = jet(fields, k);
+  PairDerivs d;
+  Jet* dp = &d.d1F2;
+  for (int k = 0; k < 11; ++k) dp[k] = jet(derivs, k);
+  Jet D, T;
+  pair_source(radius, b, f, d, D, T);
+  const cplx o[4] = {D.v, D.d, T.v, T.d};
+  for (int k = 0; k < 4; ++k) {
+    out[2 * k] = o[k].real();
+    out[2 * k + 1] = o[k].imag();
+  }
+}
+
+int oracle_ghp_row(int spin, int boost, int m, int ell_max, int ntheta, int prime,
+                   double kerr_spin, double length, double radius, const double* x,
+                   const double* dt, double* out) {
+  try {
+    OraclePipeline p;
+    p.p.spin = kerr_spin;
+    p.p.length = length;
+    p.M = 1;
+    p.N = 1;
+    p.T = ntheta;
+    p.h = radius;
+    p.modes = {m};
+    std::vector<double> xs, ws;
+    gauss_legendre(ntheta, xs, ws);
+    p.cos_t = xs;
+    p.sin_t.resize(ntheta);
+    for (int t = 0; t < ntheta; ++t) p.sin_t[t] = std::sqrt(1.0 - xs[t] * xs[t]);
+    const Coord c = make_coord(&p, spin, boost, ell_max, {m}, true);
+    // radius enters as h * r with r = 0 on a one-point grid: use h = radius, r = 1
+    p.N = 2;
+    std::vector<cplx> xv(2 * ntheta, 0.0), dv(2 * ntheta, 0.0), ov(2 * ntheta, 0.0);
+    for (int t = 0; t < ntheta; ++t) {
+      xv[ntheta + t] = cplx(x[2 * t], x[2 * t + 1]);
+      dv[ntheta + t] = cplx(dt[2 * t], dt[2 * t + 1]);
+    }
+    ghp(&p, c, prime != 0, Plane{xv.data(), 0}, Plane{dv.data(), 0},
+        PlaneW{ov.data(), 0});
+    for (int t = 0; t < ntheta; ++t) {
+      out[2 * t] = ov[ntheta + t].real();
+      out[2 * t + 1] = ov[ntheta + t].imag();
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return -1;
+  }
+}
+
+void oracle_delta_n(const double* value, const double* dt, const double* dr, int n,
+                    double radius, double mass, double length, double* out) {
+  const cplx r = delta_n(cplx(value[0], value[1]), cplx(dt[0], dt[1]), cplx(dr[0], dr[1]), n,
+                         radius, mass, length);
+  out[0] = r.real();
+  out[1] = r.imag();
+}
+
+void oracle_recon_dt(const double* value, const double* dr, const double* rhs, int n,
+                     double radius, double mass, double length, double* out) {
+  const cplx r = recon_dt(cplx(value[0], value[1]), cplx(dr[0], dr[1]), cplx(rhs[0], rhs[1]), n,
+                          radius, mass, length);
+  out[0] = r.real();
+  out[1] = r.imag();
 }
 
 }  // extern "C"

@@ -45,7 +45,32 @@ typedef struct OracleParams {
   int reduction;                  /* 0 free_damped, 1 stage_constrained   */
   int scheme;                     /* 0 d4-2, 1 d8-4, 2 d10-5              */
   int threads;                    /* OpenMP threads (<=0: runtime default)*/
+  /* Second-order source policy (spatial_pipeline.hpp:35-62): 0 disabled
+   * (linear only), 1 constraint-aware, 2 unrestricted. */
+  int source_mode;
+  double source_start_time;
+  double constraint_tolerance;
+  int required_passes;
 } OracleParams;
+
+/* SourceActivationState (source_activation.hpp:17-22). */
+typedef struct OracleActivation {
+  int active;
+  double activation_time;
+  int consecutive_passes;
+  double last_eligibility_time;
+} OracleActivation;
+
+/* SourceConstraintNorms (source_activation.hpp:24-37); families in the
+ * order psi3 Bianchi, psi2 Bianchi, h_ll reality. */
+typedef struct OracleConstraintNorms {
+  double absolute_maximum[3];
+  double normalized_maximum[3];
+  double weighted_rms[3];
+  double normalized_weighted[3];
+  double controlling_normalized;
+  int all_finite;
+} OracleConstraintNorms;
 
 typedef struct OraclePipeline OraclePipeline;
 
@@ -71,9 +96,26 @@ double oracle_init_gaussian(OraclePipeline* p, double center, double width,
 
 /* One spatial RHS (spatial_pipeline.hpp:458-515, Disabled policy). */
 void oracle_evaluate_rhs(OraclePipeline* p, const double* stage, double* out);
-/* Classical RK4 steps of the owned state (device_rk4.hpp:84-128). */
-void oracle_step(OraclePipeline* p, double time, double dt);
-void oracle_advance(OraclePipeline* p, int steps, double time0, double dt);
+/* Classical RK4 steps of the owned state (device_rk4.hpp:84-128), split at
+ * the source start time with the activation latch considered on accepted
+ * states (spatial_pipeline.hpp:590-620).  Returns 0, or -1 on error. */
+int oracle_step(OraclePipeline* p, double time, double dt);
+int oracle_advance(OraclePipeline* p, int steps, double time0, double dt);
+
+/* Activation latch and the last constraint sample. */
+void oracle_get_activation(const OraclePipeline* p, OracleActivation* out);
+void oracle_set_activation(OraclePipeline* p, const OracleActivation* in);
+void oracle_constraint_norms(const OraclePipeline* p, OracleConstraintNorms* out);
+/* Consider activation on the owned state at `time`; returns the latch. */
+int oracle_consider_activation(OraclePipeline* p, double time);
+/* Latch seen by each RK stage of the last step (count returned). */
+int oracle_last_stage_sources(const OraclePipeline* p, int* flags);
+/* Intermediates of the last RHS evaluation (M x nr x ntheta complex, or
+ * M x 2 x nr x ntheta for the D, T pairs): 0 forcing, 1 source_over_r3,
+ * 2 projected inner (D, T), 3 projected inner tangent, 4 constraint
+ * residuals (M x 3 x ...), 5 constraint terms (M x 6 x ...).  Returns the
+ * complex value count, or -1. */
+long long oracle_intermediate(const OraclePipeline* p, int which, double* out);
 
 /* Modal endpoint coefficients of FirstPsi (waveform_output.hpp:40-160) for
  * one stored mode index: out[2*n_ell] at scri (radial 0) and horizon
@@ -109,6 +151,28 @@ double oracle_maximum_coordinate_speed(double mass, double spin,
  * relative residual or -1 on a singular matrix. */
 double oracle_dense_solve(int n, const double* matrix, const double* rhs,
                           double* solution);
+
+/* ---- second-order point functions used by the known-answer tests ---- */
+/* out: mu0, tau0, pi0, psi20, rho0, epsilon0 as (re, im) pairs (12). */
+void oracle_background(double mass, double spin, double length, double radius,
+                       double cos_theta, double sin_theta, double* out);
+/* Ordered-pair source with Jet1 tangents.  bg: mu0, tau0, pi0 (re, im);
+ * fields: 19 jets (value re, im, tangent re, im) in OrderedPairFields
+ * order; derivs: 11 jets in OrderedPairDerivatives order; out: D then T
+ * jets (8 doubles). */
+void oracle_pair_source(double radius, const double* bg, const double* fields,
+                        const double* derivs, double* out);
+/* eth_n (prime = 0) or eth'_n (prime = 1) of one theta row of a (spin,
+ * boost, m, ell_max) band on ntheta nodes at radius R: x, dt, out are
+ * complex rows of length ntheta.  Returns 0 or -1. */
+int oracle_ghp_row(int spin, int boost, int m, int ell_max, int ntheta, int prime,
+                   double kerr_spin, double length, double radius, const double* x,
+                   const double* dt, double* out);
+/* Delta_n and its inverse for the time derivative (complex in/out). */
+void oracle_delta_n(const double* value, const double* dt, const double* dr, int n,
+                    double radius, double mass, double length, double* out);
+void oracle_recon_dt(const double* value, const double* dr, const double* rhs, int n,
+                     double radius, double mass, double length, double* out);
 
 #ifdef __cplusplus
 }

@@ -18,7 +18,12 @@ def oracle_pipeline(params: dict, threads: int = 0) -> oracle.Pipeline:
         second_order_modes=list(p.get("second_order_modes", [-2, 2])),
         reduction_damping=p.get("reduction_damping", 0.1),
         dissipation=p.get("dissipation", 0.01), reduction=p.get("reduction_mode", "free_damped"),
-        scheme=p.get("radial_discretization", "d4-2"))
+        scheme=p.get("radial_discretization", "d4-2"),
+        source_mode=(p.get("second_order__source_mode", "constraint_aware")
+                     if p.get("second_order__enabled", False) else "disabled"),
+        source_start_time=p.get("second_order__source_start_time", 0.0),
+        constraint_tolerance=p.get("second_order__constraint_tolerance", 1.0e-10),
+        required_passes=p.get("second_order__required_consecutive_passes", 1))
     return oracle.Pipeline(st, threads=threads)
 
 
