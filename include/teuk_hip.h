@@ -148,6 +148,13 @@ int teuk_get_source_activation(const teuk_plan* plan, int instance,
 /* restore_source_activation (spatial_pipeline.hpp:395-400). */
 int teuk_set_source_activation(teuk_plan* plan, int instance,
                                const teuk_source_activation* state);
+/* source_constraint_norms (spatial_pipeline.hpp:378-380): the norms of the
+ * last constraint-aware activation check of a second-order instance,
+ * SourceConstraintNorms (source_activation.hpp:24-37) as 14 doubles: for
+ * psi3 Bianchi, psi2 Bianchi and h_ll reality in turn the absolute maximum,
+ * normalized maximum, weighted RMS and normalized weighted norm, then the
+ * controlling normalized value and 1.0 / 0.0 for all_finite. */
+int teuk_get_source_constraint_norms(const teuk_plan* plan, int instance, double* norms14);
 
 /* ---------------------------------------------------------------------
  * Execution tuning and measurement (no reference counterpart: they replace

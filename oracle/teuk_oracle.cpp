@@ -1082,7 +1082,13 @@ int oracle_step(OraclePipeline* p, double time, double dt) {
 int oracle_advance(OraclePipeline* p, int steps, double time0, double dt) {
   try {
     set_threads(p);
-    for (int s = 0; s < steps; ++s) pipeline_step(p, time0 + s * dt, dt);
+    // time accumulates by addition, as the driver's loop does
+    // (solver_driver.hpp:289-293)
+    double t = time0;
+    for (int s = 0; s < steps; ++s) {
+      pipeline_step(p, t, dt);
+      t += dt;
+    }
     return 0;
   } catch (const std::exception& e) {
     g_error = e.what();

@@ -72,6 +72,7 @@ SIGNATURES = {
     "teuk_sample_waveform": (_I, [_P, _I, ctypes.POINTER(WaveformRecord), _I, _IP]),
     "teuk_get_source_activation": (_I, [_P, _I, ctypes.POINTER(SourceActivation)]),
     "teuk_set_source_activation": (_I, [_P, _I, ctypes.POINTER(SourceActivation)]),
+    "teuk_get_source_constraint_norms": (_I, [_P, _I, ctypes.POINTER(ctypes.c_double)]),
     "teuk_plan_set_graph": (_I, [_P, _I]),
     "teuk_plan_set_lds_budget": (_I, [_P, _I]),
     "teuk_plan_set_tile_rows": (_I, [_P, _I]),
@@ -320,6 +321,17 @@ class Plan:
 
     def restore_source_activation(self, state: SourceActivation, instance: int = 0):
         self._check(lib().teuk_set_source_activation(self._h, instance, ctypes.byref(state)))
+
+    def source_constraint_norms(self, instance: int = 0) -> dict:
+        """Norms of the last constraint-aware activation check (second-order
+        plans): per family the absolute maximum, normalized maximum, weighted
+        RMS and normalized weighted norm."""
+        v = (ctypes.c_double * 14)()
+        self._check(lib().teuk_get_source_constraint_norms(self._h, instance, v))
+        fam = lambda k: [v[k], v[4 + k], v[8 + k]]
+        return {"absolute_maximum": fam(0), "normalized_maximum": fam(1),
+                "weighted_rms": fam(2), "normalized_weighted": fam(3),
+                "controlling_normalized": v[12], "all_finite": v[13] > 0.5}
 
     # ---------------------------------------------------------------- tuning
     def set_graph(self, enable: bool):

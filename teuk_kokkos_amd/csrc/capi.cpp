@@ -276,6 +276,17 @@ int teuk_set_source_activation(teuk_plan* plan, int instance,
   });
 }
 
+int teuk_get_source_constraint_norms(const teuk_plan* plan, int instance, double* norms14) {
+  auto* p = const_cast<teuk_plan*>(plan);
+  std::string dummy;
+  return guarded(p ? p->error : dummy, [&] {
+    const teuk::Plan& pl = plan_of(p);
+    check_instance(pl, instance);
+    if (!norms14) throw std::invalid_argument("null norms output");
+    pl.constraint_norms(instance, norms14);
+  });
+}
+
 int teuk_plan_set_graph(teuk_plan* plan, int enable) {
   std::string dummy;
   return guarded(plan ? plan->error : dummy, [&] { plan_of(plan).set_graph(enable != 0); });

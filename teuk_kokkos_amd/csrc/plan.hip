@@ -1,6 +1,8 @@
 // Plan: device memory, line/tile tables, initial data, import/export,
 // RK4 sequencing, hipGraph capture and waveform sampling.
 #include "plan.hpp"
+#include "plan_detail.hpp"
+#include "so_host.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -21,80 +23,7 @@ hipError_t launch_modal_to_nodal(const NodalArgs& a, const cdouble* modal, cdoub
                                  hipStream_t s);
 hipError_t launch_fill_zero(cdouble* p, long long count, hipStream_t s);
 
-namespace {
-
-void check(hipError_t e, const char* what) {
-  if (e != hipSuccess)
-    throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
-}
-
-template <class T>
-T* device_alloc(size_t count) {
-  if (count == 0) count = 1;
-  void* p = nullptr;
-  check(hipMalloc(&p, count * sizeof(T)), "hipMalloc");
-  return static_cast<T*>(p);
-}
-
-template <class F>
-void parallel_for(int n, F&& f) {
-  const int hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-  const int nt = std::min(hw, std::max(1, n));
-  if (nt <= 1) {
-    for (int i = 0; i < n; ++i) f(i);
-    return;
-  }
-  std::vector<std::thread> pool;
-  std::vector<std::exception_ptr> errors(nt);
-  for (int t = 0; t < nt; ++t)
-    pool.emplace_back([&, t] {
-      try {
-        for (int i = t; i < n; i += nt) f(i);
-      } catch (...) {
-        errors[t] = std::current_exception();
-      }
-    });
-  for (auto& th : pool) th.join();
-  for (auto& e : errors)
-    if (e) std::rethrow_exception(e);
-}
-
-SchemeTable make_table(RadialScheme s) {
-  const SbpScheme& sc = sbp_scheme(s);
-  SchemeTable t{};
-  t.W = sc.closure_rows;
-  t.C = sc.closure_width;
-  t.B = sc.half_width;
-  t.order = sc.diss_order;
-  for (int r = 0; r < sc.closure_rows; ++r)
-    for (int c = 0; c < sc.closure_width; ++c)
-      t.left[r * kMaxClosureWidth + c] = sc.left[r * sc.closure_width + c];
-  for (size_t k = 0; k < sc.interior.size(); ++k) t.interior[k] = sc.interior[k];
-  for (size_t k = 0; k < sc.norm.size(); ++k) t.norm[k] = sc.norm[k];
-  for (int d = 0; d < sc.diss_order; ++d)
-    for (size_t c = 0; c < sc.diss_left[d].size(); ++c) t.diss_left[d][c] = sc.diss_left[d][c];
-  for (size_t k = 0; k < sc.diss_interior.size(); ++k) t.diss_interior[k] = sc.diss_interior[k];
-  return t;
-}
-
-// Expanded Gaussian seeds with sharp partners, sorted by (m, ell)
-// (initial_data_factory.hpp:45-74).
-std::vector<SeedMode> expanded_seeds(const RunParameters& p) {
-  std::vector<SeedMode> out;
-  for (const auto& s : p.seeds) {
-    out.push_back(s);
-    if (p.add_sharp_partner && s.m != 0) out.push_back({s.ell, -s.m, std::conj(s.amplitude)});
-  }
-  std::sort(out.begin(), out.end(), [](const SeedMode& a, const SeedMode& b) {
-    return a.m != b.m ? a.m < b.m : a.ell < b.ell;
-  });
-  for (size_t i = 1; i < out.size(); ++i)
-    if (out[i].m == out[i - 1].m && out[i].ell == out[i - 1].ell)
-      throw std::invalid_argument("expanded Gaussian modes contain duplicate (ell,m) seeds");
-  return out;
-}
-
-}  // namespace
+using namespace detail;
 
 Plan::Instance::Instance(RunParameters p)
     : params(std::move(p)), registry(make_run_registry(params)) {
@@ -113,12 +42,20 @@ Plan::Instance::Instance(RunParameters p)
     sin_theta[j] = std::sin(th);
     cos_theta[j] = std::cos(th);
   }
-  if (params.source_start_time == 0.0 && params.second_order_enabled &&
-      params.source_mode == SourceMode::Unrestricted) {
-    activation.active = 1;
-    activation.activation_time = 0.0;
-    activation.consecutive_passes = params.required_consecutive_passes;
+  activation = Plan::initial_activation(params);
+}
+
+// The latch a pipeline starts with: active at t = 0 only for the
+// unrestricted policy with a zero start time (spatial_pipeline.hpp:342-348).
+SourceActivation Plan::initial_activation(const RunParameters& p) {
+  SourceActivation a;
+  if (p.source_start_time == 0.0 && p.second_order_enabled &&
+      p.source_mode == SourceMode::Unrestricted) {
+    a.active = 1;
+    a.activation_time = 0.0;
+    a.consecutive_passes = p.required_consecutive_passes;
   }
+  return a;
 }
 
 Plan::Plan(const std::vector<RunParameters>& params) {
@@ -128,9 +65,6 @@ Plan::Plan(const std::vector<RunParameters>& params) {
     if (p.plus2_enabled)
       throw std::runtime_error("plus2 mode is not production integrated; use plus2.enabled=false "
                                "and plus2.mode=disabled");
-    if (p.second_order_enabled)
-      throw std::invalid_argument(
-          "second_order.enabled=true: the quadratic-source path is not part of this build");
     const RunParameters& q = p;
     ModeRegistry reg = make_run_registry(q);
     if (!reg.closed_under_sharp())
@@ -153,7 +87,19 @@ Plan::Plan(const std::vector<RunParameters>& params) {
     if (in.dt > in.params.cfl * in.h / vmax)
       throw std::invalid_argument("steps are too small for the requested radial CFL limit");
   }
+  // Instances with the quadratic source run the 13-field second-order graph;
+  // a plan holds either kind, not both.
+  so_mode_ = params.front().second_order_enabled;
+  for (const auto& p : params)
+    if (p.second_order_enabled != so_mode_)
+      throw std::invalid_argument(
+          "a plan cannot mix first-order-only and second_order.enabled instances");
   check(hipStreamCreate(&stream_), "hipStreamCreate");
+  if (so_mode_) {
+    so_build();
+    check(hipStreamSynchronize(stream_), "plan setup");
+    return;
+  }
   build_lines();
   build_tiles();
   upload_tables();
@@ -183,6 +129,7 @@ Plan::~Plan() {
                   static_cast<void*>(d_y_), static_cast<void*>(d_acc_), static_cast<void*>(d_ya_),
                   static_cast<void*>(d_yb_), static_cast<void*>(d_k_)})
     if (p) hipFree(p);
+  so_free();
   if (stream_) hipStreamDestroy(stream_);
 }
 
@@ -498,111 +445,128 @@ double Plan::modal_points_per_step() const {
   return s;
 }
 
-// Gaussian pulse in Psi, Q = D Psi with the selected scheme, Galerkin P,
-// then analysis into the retained band (pipeline_initial_data.hpp:211-340).
+// Gaussian pulse in Psi, Q = D Psi with the selected scheme and the Galerkin
+// P of one first-order mode, nodal (N x T each)
+// (pipeline_initial_data.hpp:142-203, 211-340).  Returns the largest
+// relative residual of the per-radius Galerkin solves.
+double Plan::first_order_initial(const Instance& in, int mi, std::vector<cplx>& pn,
+                                 std::vector<cplx>& q, std::vector<cplx>& psi) {
+  const RunParameters& p = in.params;
+  const int m = in.registry.modes()[mi];
+  const int N = in.N, T = in.T;
+  const double center = p.center_fraction * in.horizon;
+  const double width = p.width_fraction * in.horizon;
+  if (!(width > 0.0) || center < 0.0 || center > in.horizon)
+    throw std::invalid_argument("Gaussian center must be on-grid and width must be positive");
+  const auto seeds = expanded_seeds(p);
+  for (const auto& s : seeds) {
+    if (!in.registry.is_parent(s.m))
+      throw std::invalid_argument("Gaussian seed m is not a first-order mode");
+    if (s.ell < std::max(2, std::abs(s.m)) || s.ell > p.ell_max_first)
+      throw std::invalid_argument("Gaussian seed ell is outside the band");
+  }
+  const BandTransform bt = make_band_transform(-2, m, p.ell_max_first, T);
+  const int nl = bt.modes();
+  std::vector<cplx> seed(nl, cplx(0.0, 0.0));
+  for (const auto& s : seeds)
+    if (s.m == m) seed[s.ell - bt.ell_min] += s.amplitude;
+  const std::vector<cplx> ang = bt.synthesize(seed);
+  psi.assign(static_cast<size_t>(N) * T, cplx(0.0, 0.0));
+  q.assign(psi.size(), cplx(0.0, 0.0));
+  pn.assign(psi.size(), cplx(0.0, 0.0));
+  for (int r = 0; r < N; ++r) {
+    const double z = (0.0 + in.h * r - center) / width;
+    double prof;
+    if (!p.compact_support) prof = std::exp(-z * z);
+    else if (std::abs(0.0 + in.h * r - center) >= width) prof = 0.0;
+    else prof = std::exp(1.0 - 1.0 / (1.0 - z * z));
+    for (int j = 0; j < T; ++j) psi[r * T + j] = prof * ang[j];
+  }
+  for (int j = 0; j < T; ++j)
+    sbp_derivative_line(p.scheme, N, 1.0 / in.h, &psi[j], T, &q[j], T);
+  double maxres = 0.0;
+  std::vector<cplx> mat(static_cast<size_t>(nl) * nl), rhs(nl), sol;
+  for (int r = 0; r < N; ++r) {
+    std::fill(mat.begin(), mat.end(), cplx(0.0, 0.0));
+    std::fill(rhs.begin(), rhs.end(), cplx(0.0, 0.0));
+    const RadialCoefficients rc =
+        radial_coefficients(p.mass, p.spin, p.compactification_length, -2, m, 0.0 + in.h * r);
+    for (int j = 0; j < T; ++j) {
+      const double ct = time_coefficient(rc, p.spin, in.sin_theta[j]);
+      const cdouble g = definition_coefficient(rc, p.spin, -2, in.cos_theta[j]);
+      const cplx inv_time = 1.0 / ct;
+      const cplx nodal =
+          inv_time * (-2.0 * rc.adv * q[r * T + j] + cplx(g.re, g.im) * psi[r * T + j]);
+      for (int a = 0; a < nl; ++a) {
+        const double aa = bt.A[static_cast<size_t>(a) * T + j];
+        rhs[a] += aa * nodal;
+        for (int b = 0; b < nl; ++b)
+          mat[a * nl + b] += aa * inv_time * bt.S[static_cast<size_t>(j) * nl + b];
+      }
+    }
+    if (!dense_complex_solve(nl, mat, rhs, sol))
+      throw std::runtime_error("singular Galerkin initialization matrix");
+    maxres = std::max(maxres, dense_complex_residual(nl, mat, sol, rhs));
+    const std::vector<cplx> pnod = bt.synthesize(sol);
+    for (int j = 0; j < T; ++j) pn[r * T + j] = pnod[j];
+  }
+  return maxres;
+}
+
 void Plan::initialize_gaussian() {
-  std::vector<cdouble> host(static_cast<size_t>(modal_count_));
   for (auto& in : inst_) {
     if (in.params.initial_type != InitialDataKind::Gaussian)
       throw std::invalid_argument("checkpoint initial data are not supported by this build");
   }
-  std::vector<std::pair<int, int>> work;  // (instance, mode)
-  for (int i = 0; i < instance_count(); ++i)
-    for (int mi = 0; mi < inst_[i].M; ++mi)
-      if (inst_[i].line_of_mode[mi] >= 0) work.push_back({i, mi});
-  std::vector<double> residual(work.size(), 0.0);
-  parallel_for(static_cast<int>(work.size()), [&](int w) {
-    const Instance& in = inst_[work[w].first];
-    const int mi = work[w].second;
-    const RunParameters& p = in.params;
-    const int m = in.registry.modes()[mi];
-    const LineDesc& L = lines_[in.line_of_mode[mi]];
-    const int N = in.N, T = in.T, nl = L.n_ell;
-    const double center = p.center_fraction * in.horizon;
-    const double width = p.width_fraction * in.horizon;
-    if (!(width > 0.0) || center < 0.0 || center > in.horizon)
-      throw std::invalid_argument("Gaussian center must be on-grid and width must be positive");
-    const auto seeds = expanded_seeds(p);
-    for (const auto& s : seeds) {
-      if (!in.registry.is_parent(s.m))
-        throw std::invalid_argument("Gaussian seed m is not a first-order mode");
-      if (s.ell < std::max(2, std::abs(s.m)) || s.ell > p.ell_max_first)
-        throw std::invalid_argument("Gaussian seed ell is outside the band");
-    }
-    const BandTransform bt = make_band_transform(-2, m, p.ell_max_first, T);
-    std::vector<cplx> seed(nl, cplx(0.0, 0.0));
-    for (const auto& s : seeds)
-      if (s.m == m) seed[s.ell - bt.ell_min] += s.amplitude;
-    const std::vector<cplx> ang = bt.synthesize(seed);
-    std::vector<cplx> psi(static_cast<size_t>(N) * T), q(static_cast<size_t>(N) * T),
-        pn(static_cast<size_t>(N) * T);
-    for (int r = 0; r < N; ++r) {
-      const double z = (0.0 + in.h * r - center) / width;
-      double prof;
-      if (!p.compact_support) prof = std::exp(-z * z);
-      else if (std::abs(0.0 + in.h * r - center) >= width) prof = 0.0;
-      else prof = std::exp(1.0 - 1.0 / (1.0 - z * z));
-      for (int j = 0; j < T; ++j) psi[r * T + j] = prof * ang[j];
-    }
-    for (int j = 0; j < T; ++j)
-      sbp_derivative_line(p.scheme, N, 1.0 / in.h, &psi[j], T, &q[j], T);
-    double maxres = 0.0;
-    std::vector<cplx> mat(static_cast<size_t>(nl) * nl), rhs(nl), sol;
-    for (int r = 0; r < N; ++r) {
-      std::fill(mat.begin(), mat.end(), cplx(0.0, 0.0));
-      std::fill(rhs.begin(), rhs.end(), cplx(0.0, 0.0));
-      const RadialCoefficients rc =
-          radial_coefficients(p.mass, p.spin, p.compactification_length, -2, m, 0.0 + in.h * r);
-      for (int j = 0; j < T; ++j) {
-        const double ct = time_coefficient(rc, p.spin, in.sin_theta[j]);
-        const cdouble g = definition_coefficient(rc, p.spin, -2, in.cos_theta[j]);
-        const cplx inv_time = 1.0 / ct;
-        const cplx nodal =
-            inv_time * (-2.0 * rc.adv * q[r * T + j] + cplx(g.re, g.im) * psi[r * T + j]);
-        for (int a = 0; a < nl; ++a) {
-          const double aa = bt.A[static_cast<size_t>(a) * T + j];
-          rhs[a] += aa * nodal;
-          for (int b = 0; b < nl; ++b)
-            mat[a * nl + b] += aa * inv_time * bt.S[static_cast<size_t>(j) * nl + b];
+  if (so_mode_) {
+    so_initialize();
+  } else {
+    std::vector<cdouble> host(static_cast<size_t>(modal_count_));
+    std::vector<std::pair<int, int>> work;  // (instance, mode)
+    for (int i = 0; i < instance_count(); ++i)
+      for (int mi = 0; mi < inst_[i].M; ++mi)
+        if (inst_[i].line_of_mode[mi] >= 0) work.push_back({i, mi});
+    std::vector<double> residual(work.size(), 0.0);
+    parallel_for(static_cast<int>(work.size()), [&](int w) {
+      const Instance& in = inst_[work[w].first];
+      const int mi = work[w].second;
+      const LineDesc& L = lines_[in.line_of_mode[mi]];
+      const int N = in.N, T = in.T, nl = L.n_ell;
+      std::vector<cplx> pn, q, psi;
+      residual[w] = first_order_initial(in, mi, pn, q, psi);
+      // Projection onto the retained band: the modal coefficients are A x.
+      const BandTransform bt = make_band_transform(-2, L.m, in.params.ell_max_first, T);
+      cdouble* out = host.data() + L.base;
+      for (int r = 0; r < N; ++r) {
+        const cplx* fields[3] = {&pn[r * T], &q[r * T], &psi[r * T]};
+        for (int f = 0; f < 3; ++f) {
+          const std::vector<cplx> c = bt.analyze(fields[f]);
+          for (int l = 0; l < nl; ++l)
+            out[static_cast<size_t>(r) * 3 * nl + f * nl + l] = {c[l].real(), c[l].imag()};
         }
       }
-      if (!dense_complex_solve(nl, mat, rhs, sol))
-        throw std::runtime_error("singular Galerkin initialization matrix");
-      maxres = std::max(maxres, dense_complex_residual(nl, mat, sol, rhs));
-      const std::vector<cplx> pnod = bt.synthesize(sol);
-      for (int j = 0; j < T; ++j) pn[r * T + j] = pnod[j];
-    }
-    residual[w] = maxres;
-    // Projection onto the retained band: the modal coefficients are A x.
-    cdouble* out = host.data() + L.base;
-    for (int r = 0; r < N; ++r) {
-      const cplx* fields[3] = {&pn[r * T], &q[r * T], &psi[r * T]};
-      for (int f = 0; f < 3; ++f) {
-        const std::vector<cplx> c = bt.analyze(fields[f]);
-        for (int l = 0; l < nl; ++l)
-          out[static_cast<size_t>(r) * 3 * nl + f * nl + l] = {c[l].real(), c[l].imag()};
-      }
-    }
-  });
-  for (double r : residual)
-    if (!std::isfinite(r) || r > 2.0e-12)
-      throw std::runtime_error("Galerkin initial-time-derivative solve failed its residual check");
-  check(hipMemcpyAsync(d_y_, host.data(), host.size() * sizeof(cdouble), hipMemcpyHostToDevice,
-                       stream_),
-        "upload initial state");
-  check(hipStreamSynchronize(stream_), "initialize");
+    });
+    for (double r : residual)
+      if (!std::isfinite(r) || r > 2.0e-12)
+        throw std::runtime_error(
+            "Galerkin initial-time-derivative solve failed its residual check");
+    check(hipMemcpyAsync(d_y_, host.data(), host.size() * sizeof(cdouble), hipMemcpyHostToDevice,
+                         stream_),
+          "upload initial state");
+    check(hipStreamSynchronize(stream_), "initialize");
+  }
   for (auto& in : inst_) {
     in.passive.clear();
     in.residual.clear();
     in.time = 0.0;
     in.steps_done = 0;
-    SourceActivation fresh;
-    in.activation = fresh;
+    in.activation = initial_activation(in.params);
   }
 }
 
 // Strict band-validated import (pipeline_bands.hpp:108-232).
 double Plan::import_state(int i, const cplx* host, double tol) {
+  if (so_mode_) return so_import(i, host, tol);
   Instance& in = inst_.at(i);
   const int M = in.M, N = in.N, T = in.T;
   static const int spins[13] = {-2, -2, -2, -1, -2, 0, -2, -1, -1, 0, -2, -2, -2};
@@ -670,6 +634,7 @@ double Plan::import_state(int i, const cplx* host, double tol) {
 }
 
 void Plan::export_state(int i, cplx* host) {
+  if (so_mode_) return so_export(i, host);
   Instance& in = inst_.at(i);
   const int M = in.M, N = in.N, T = in.T;
   check(hipStreamSynchronize(stream_), "export sync");
@@ -787,7 +752,22 @@ void Plan::step(double t, double dt) {
   if (!std::isfinite(t) || !std::isfinite(dt) || t < 0.0 || !(dt > 0.0))
     throw std::invalid_argument("pipeline step requires finite positive time data");
   if (!std::isfinite(t + dt)) throw std::invalid_argument("pipeline step end time is not finite");
-  launch_step(dt);
+  if (so_mode_) {
+    const bool timed = timing_;
+    hipEvent_t e0 = nullptr;
+    if (timed) {
+      e0 = timing_event();
+      check(hipEventRecord(e0, stream_), "event");
+    }
+    so_step(std::vector<double>(inst_.size(), t), std::vector<double>(inst_.size(), dt));
+    if (timed) {
+      hipEvent_t e1 = timing_event();
+      check(hipEventRecord(e1, stream_), "event");
+      step_events_.push_back({e0, e1});
+    }
+  } else {
+    launch_step(dt);
+  }
   for (auto& in : inst_) {
     in.time = t + dt;
     ++in.steps_done;
@@ -796,6 +776,28 @@ void Plan::step(double t, double dt) {
 
 void Plan::advance(int steps) {
   if (steps < 0) throw std::invalid_argument("step count must be nonnegative");
+  if (so_mode_) {
+    for (int s = 0; s < steps; ++s) {
+      std::vector<double> t, dt;
+      for (const auto& in : inst_) {
+        t.push_back(in.time);
+        dt.push_back(in.dt);
+      }
+      hipEvent_t e0 = nullptr;
+      if (timing_) {
+        e0 = timing_event();
+        check(hipEventRecord(e0, stream_), "event");
+      }
+      so_step(t, dt);
+      if (timing_) {
+        hipEvent_t e1 = timing_event();
+        check(hipEventRecord(e1, stream_), "event");
+        step_events_.push_back({e0, e1});
+      }
+      record_time_step(0.0);
+    }
+    return;
+  }
   if (use_graph_) {
     if (!graph_exec_) {
       hipGraph_t graph;
@@ -818,7 +820,8 @@ void Plan::advance(int steps) {
 }
 
 void Plan::evaluate_rhs(int i, const cplx* d_stage, cplx* d_out, double t) {
-  (void)t;  // the linear first-order operator is autonomous
+  (void)t;  // the stage operator is autonomous; the latch is host state
+  if (so_mode_) return so_evaluate_rhs(i, d_stage, d_out);
   Instance& in = inst_.at(i);
   if (in.tile_count == 0) {
     check(launch_fill_zero(reinterpret_cast<cdouble*>(d_out), nodal_value_count(i), stream_), "zero");
@@ -851,6 +854,7 @@ void Plan::evaluate_rhs(int i, const cplx* d_stage, cplx* d_out, double t) {
 
 // EndpointWaveformSampler::sample (waveform_output.hpp:79-160).
 std::vector<WaveformRecordC> Plan::sample_waveform(int i) {
+  if (so_mode_) return so_sample_waveform(i);
   Instance& in = inst_.at(i);
   check(hipStreamSynchronize(stream_), "waveform sync");
   std::vector<WaveformRecordC> out;
@@ -971,6 +975,7 @@ void Plan::group_info(int g, int* scheme, int* ub, int* tiles, int* lds_bytes,
 void Plan::set_lds_budget(int bytes) {
   if (bytes < 16 * 1024 || bytes > 160 * 1024)
     throw std::invalid_argument("LDS budget must be in [16 KiB, 160 KiB]");
+  if (so_mode_) return;  // the second-order kernels size their own tiles
   lds_budget_ = bytes;
   build_tiles();
   upload_tables();
@@ -983,6 +988,7 @@ void Plan::set_lds_budget(int bytes) {
 
 void Plan::set_segment_rows(int rows) {
   if (rows < 0) throw std::invalid_argument("segment rows must be nonnegative");
+  if (so_mode_) return;
   segment_rows_ = rows;
   build_tiles();
   upload_tables();
@@ -994,6 +1000,7 @@ void Plan::set_segment_rows(int rows) {
 
 void Plan::set_tile_rows(int rows) {
   if (rows < 0) throw std::invalid_argument("tile rows must be nonnegative");
+  if (so_mode_) return;
   tile_rows_ = rows;
   build_tiles();
   upload_tables();
@@ -1009,6 +1016,7 @@ void Plan::set_activation(int i, const SourceActivation& a) {
       (!a.active && a.activation_time != -1.0) || a.last_eligibility_time < -1.0)
     throw std::invalid_argument("invalid second-order source activation state");
   inst_.at(i).activation = a;
+  if (so_mode_) so_upload_active();
 }
 
 }  // namespace teuk

@@ -92,8 +92,30 @@ class Plan {
   void set_segment_rows(int rows);  // marching-kernel segment length, 0 = off
   SourceActivation activation(int i) const { return inst_.at(i).activation; }
   void set_activation(int i, const SourceActivation& a);
+  bool second_order() const { return so_mode_; }
+  // Constraint norms of the last activation check of instance i
+  // (source_activation.hpp:24-37): per family psi3, psi2, h_ll the absolute
+  // maximum, normalized maximum, weighted RMS and normalized weighted norm,
+  // then the controlling value and the finiteness flag.
+  void constraint_norms(int i, double* out14) const;
+  static SourceActivation initial_activation(const RunParameters& p);
 
  private:
+  static double first_order_initial(const Instance& in, int mi, std::vector<cplx>& pn,
+                                    std::vector<cplx>& q, std::vector<cplx>& psi);
+  // second-order graph (so_plan.hip)
+  void so_build();
+  void so_free();
+  void so_initialize();
+  double so_import(int i, const cplx* host, double tol);
+  void so_export(int i, cplx* host);
+  void so_rhs(int first, int n, const cdouble* in, cdouble* out, bool constraints);
+  void so_rk_pass(const std::vector<double>& dts);
+  void so_upload_active();
+  bool so_consider(int i, double time);
+  void so_step(const std::vector<double>& t, const std::vector<double>& dt);
+  std::vector<WaveformRecordC> so_sample_waveform(int i);
+  void so_evaluate_rhs(int i, const cplx* d_stage, cplx* d_out);
   void build_lines();
   void build_tiles();
   void upload_tables();
@@ -141,6 +163,11 @@ class Plan {
   std::vector<hipStream_t> side_;  // streams of execution groups 1..n-1
   std::vector<hipEvent_t> joins_;
   hipEvent_t fork_ = nullptr;
+
+  // second-order graph state
+  bool so_mode_ = false;
+  struct SoHost;
+  std::unique_ptr<SoHost> so_;
 };
 
 }  // namespace teuk

@@ -1,0 +1,840 @@
+// Second-order stage graph on gfx950: the reference's 13-field nodal state
+// advanced with its reconstruction chain, stage tangent, quadratic
+// ordered-pair source and second-order Teukolsky triple
+// (spatial_pipeline.hpp:458-588).
+//
+// One RHS evaluation is seven launches:
+//   so_linear_kernel<0>   first-order triple of the stage -> k[0..2], D Psi
+//   so_drad_kernel        D of the stage reconstruction fields
+//   so_recon_kernel<0>    reconstruction chain (stage, k) -> k[3..9], and the
+//                         independent constraints when asked
+//   so_linear_kernel<1>   first-order triple of k -> tangent, D Psi_t
+//   so_drad_kernel        D of k's reconstruction fields
+//   so_recon_kernel<1>    tangent chain (k, tangent) kept in LDS, then the
+//                         ordered-pair source with Jet1 tangents, its band
+//                         projection and eth'_3 T
+//   so_linear_kernel<2>   second-order triple of the stage with the forcing
+//                         (outer operator evaluated in the kernel) -> k[10..12]
+// Angular operators are T x T nodal matrices built on the host (projection,
+// Laplacian, eth and eth' angular parts), applied to rows staged in LDS.
+#include <hip/hip_runtime.h>
+
+#include "sbp_device.hpp"
+#include "so_layout.hpp"
+
+namespace teuk {
+
+namespace {
+
+__device__ __forceinline__ cdouble cconj(cdouble a) { return {a.re, -a.im}; }
+__device__ __forceinline__ cdouble cneg(cdouble a) { return {-a.re, -a.im}; }
+__device__ __forceinline__ cdouble cmulr(cdouble a, double s) { return {a.re * s, a.im * s}; }
+
+// Complex quotient by Smith's algorithm (Smith 1962, CACM 5:435).
+__device__ __forceinline__ cdouble cdiv(cdouble a, cdouble b) {
+  if (fabs(b.re) >= fabs(b.im)) {
+    const double r = b.im / b.re, d = b.re + b.im * r;
+    return {(a.re + a.im * r) / d, (a.im - a.re * r) / d};
+  }
+  const double r = b.re / b.im, d = b.re * r + b.im;
+  return {(a.re * r + a.im) / d, (a.im * r - a.re) / d};
+}
+
+// Rescaled rotated-Kinnersley background (background.hpp:29-78).
+struct Bg {
+  cdouble mu0, tau0, pi0, psi20, rho0, eps0;
+};
+
+__device__ Bg kerr_bg(double M, double a, double L, double R, double ct, double st) {
+  const double L2 = L * L, L4 = L2 * L2;
+  const double sqrt_two = 1.4142135623730951;
+  const cdouble minus_radial = cmake(-L2, a * R * ct);
+  const cdouble angular = cmake(L2, -a * R * ct);
+  const cdouble angular2 = cmul(angular, angular);
+  const double real_pi = L4 + a * a * R * R * ct * ct;
+  Bg b;
+  b.mu0 = cdiv(cmake(1.0, 0.0), minus_radial);
+  // I a st / (sqrt2 angular2)
+  b.tau0 = cdiv(cmake(0.0, a * st), cscale(sqrt_two, angular2));
+  b.pi0 = cmake(0.0, -(a * st) / (sqrt_two * real_pi));
+  const cdouble ingoing = cmake(L2, a * R * ct);
+  const cdouble optical = cmul(angular2, ingoing);
+  b.rho0 = cdiv(cmake(-0.5 * (L4 - 2.0 * L2 * M * R + a * a * R * R), 0.0), optical);
+  b.eps0 = cdiv(cscale(0.5, cmake(L2 * M - a * a * R, -a * (L2 - M * R) * ct)), optical);
+  b.psi20 = cdiv(cmake(-M, 0.0), cmul(angular2, angular));
+  return b;
+}
+
+// Delta_n (ghp.hpp:11-22) and its inverse for the time derivative
+// (reconstruction.hpp:108-121).
+__device__ __forceinline__ cdouble delta_n(cdouble v, cdouble dt, cdouble dr, int n, double R,
+                                           double M, double L) {
+  const double L2 = L * L;
+  const double tc = 2.0 + 4.0 * M * R / L2;
+  const double rc = R / L2;
+  return cadd(cscale(tc, dt), cscale(rc, cadd(cscale(R, dr), cscale(static_cast<double>(n), v))));
+}
+
+__device__ __forceinline__ cdouble recon_dt(cdouble v, cdouble dr, cdouble rhs, int n, double R,
+                                            double M, double L) {
+  const double L2 = L * L;
+  const double den = 2.0 + 4.0 * M * R / L2;
+  const cdouble radial =
+      cadd(cscale(R * R / L2, dr), cscale(static_cast<double>(n) * R / L2, v));
+  return cdivr(csub(rhs, radial), den);
+}
+
+// eth_n / eth'_n point formulas (ghp.hpp:60-116) given the angular part.
+__device__ __forceinline__ cdouble eth_point(bool prime, cdouble v, cdouble dt, cdouble ang,
+                                             int spin, int boost, double R, double st,
+                                             double ct, double a, double L) {
+  const double sqrt_two = 1.4142135623730951;
+  const double L2 = L * L;
+  if (!prime) {
+    const cdouble den = cmake(L2, -a * R * ct);
+    const cdouble num = cadd(cmul(cmake(0.0, -a * st), dt), ang);
+    const cdouble deriv = cdiv(num, cscale(sqrt_two, den));
+    const cdouble conn = cdiv(cmul(cmake(0.0, -static_cast<double>(spin + boost) * a * R * st), v),
+                              cscale(sqrt_two, cmul(den, den)));
+    return cadd(deriv, conn);
+  }
+  const cdouble den = cmake(L2, a * R * ct);
+  const cdouble num = cadd(cmul(cmake(0.0, a * st), dt), ang);
+  const cdouble deriv = cdiv(num, cscale(sqrt_two, den));
+  const cdouble conn = cdiv(cmul(cmake(0.0, static_cast<double>(-spin + boost) * a * R * st), v),
+                            cscale(sqrt_two, cmul(den, den)));
+  return cadd(deriv, conn);
+}
+
+// thorn_n (ghp.hpp:24-58).
+__device__ __forceinline__ cdouble thorn_n(cdouble v, cdouble dt, cdouble dr, int n, int sw,
+                                           int bw, int m, double R, double ct, double M,
+                                           double a, double L, cdouble eps0) {
+  const double L2 = L * L, L4 = L2 * L2;
+  const double den = L4 + a * a * R * R * ct * ct;
+  const double tc = R * 2.0 * M * (2.0 * M - a * a * R / L2) / den;
+  const double rc = -0.5 * (L2 - 2.0 * M * R + a * a * R * R / L2) / den;
+  const int pw = sw + bw, qw = -sw + bw;
+  cdouble r = cadd(cscale(tc, dt), cscale(rc, cadd(cscale(R, dr), cscale(static_cast<double>(n), v))));
+  r = cadd(r, cdivr(cmul(cmake(0.0, R * a * static_cast<double>(m)), v), den));
+  const cdouble w = cadd(cscale(static_cast<double>(pw), eps0), cscale(static_cast<double>(qw), cconj(eps0)));
+  return csub(r, cscale(R, cmul(w, v)));
+}
+
+// Jet1 (jet.hpp:9-136): value and coordinate-time derivative.
+struct Jet {
+  cdouble v, d;
+};
+__device__ __forceinline__ Jet jadd(Jet a, Jet b) { return {cadd(a.v, b.v), cadd(a.d, b.d)}; }
+__device__ __forceinline__ Jet jneg(Jet a) { return {cneg(a.v), cneg(a.d)}; }
+__device__ __forceinline__ Jet jmul(Jet a, Jet b) {
+  return {cmul(a.v, b.v), cadd(cmul(a.d, b.v), cmul(a.v, b.d))};
+}
+__device__ __forceinline__ Jet jscale(Jet a, cdouble s) { return {cmul(a.v, s), cmul(a.d, s)}; }
+__device__ __forceinline__ Jet jscaler(Jet a, double s) { return {cmulr(a.v, s), cmulr(a.d, s)}; }
+__device__ __forceinline__ Jet jdiv(Jet a, cdouble s) { return {cdiv(a.v, s), cdiv(a.d, s)}; }
+__device__ __forceinline__ Jet jconj(Jet a) { return {cconj(a.v), cconj(a.d)}; }
+
+// corrected_ordered_pair_source (second_order.hpp:74-142) on jets.
+// f: F1 G1 Lambda1 Pi1 B1 C1 U1 F2 G2 H2 B2 C2 U2 U2# C2# C1# B1# Pi2# B2#
+// d: d1F2 d3U2 eth2C2 eth'2C2# eth1B2 d2C2 d2G2 eth2G2 eth'1F2 d2C2# eth'1B2#
+__device__ void pair_source(double R, const Bg& bg, const Jet* f, const Jet* d, Jet& D, Jet& T) {
+  enum { F1, G1, La1, Pi1, B1, C1, U1, F2, G2, H2, B2, C2, U2, U2s, C2s, C1s, B1s, Pi2s, B2s };
+  enum { d1F2, d3U2, e2C2, ep2C2s, e1B2, d2C2, d2G2, e2G2, ep1F2, d2C2s, ep1B2s };
+  const cdouble mu0b = cconj(bg.mu0), pi0b = cconj(bg.pi0), tau0b = cconj(bg.tau0);
+  const double R2 = R * R;
+  Jet acc;
+  // sd01-sd02
+  acc = jmul(jscaler(f[U1], 0.5), jadd(jdiv(d[d1F2], bg.mu0), jscaler(f[F2], R)));
+  // sd03
+  acc = jadd(acc, jmul(f[F1], jscaler(d[e2C2], 0.5 * R)));
+  // sd04
+  acc = jadd(acc, jmul(f[F1], jscale(f[C2], cscale(0.5 * R2, cadd(pi0b, cscale(2.0, bg.tau0))))));
+  // sd05
+  acc = jadd(acc, jmul(f[F1], jadd(jdiv(d[d3U2], bg.mu0), jscaler(f[U2s], R))));
+  // sd06
+  acc = jadd(acc, jmul(f[F1], jscaler(d[ep2C2s], -0.5 * R)));
+  // sd07
+  acc = jadd(acc, jmul(f[F1], jscale(f[C2s], cscale(0.5 * R2, cadd(cscale(5.0, bg.pi0),
+                                                                  cscale(4.0, tau0b))))));
+  // sd08 - sd11: -0.5 G1 * (...)
+  const Jet g1 = jscaler(f[G1], -0.5);
+  acc = jadd(acc, jmul(g1, jscaler(d[e1B2], R)));
+  acc = jadd(acc, jmul(g1, jscale(f[B2], cscale(R2, cadd(pi0b, bg.tau0)))));
+  acc = jadd(acc, jmul(g1, jscaler(d[d2C2], R)));
+  acc = jadd(acc, jmul(g1, jscale(f[C2], cscale(R2, cadd(cscale(-2.0, bg.mu0), mu0b)))));
+  // sd12 - sd15
+  acc = jadd(acc, jmul(jscaler(f[C1], -R), d[d2G2]));
+  acc = jadd(acc, jmul(jscaler(f[B1], 0.5 * R), d[e2G2]));
+  acc = jadd(acc, jmul(jscaler(f[Pi1], 4.0 * R), f[G2]));
+  acc = jadd(acc, jmul(jscaler(f[La1], -3.0 * R), f[H2]));
+  D = acc;
+  // st01-st02
+  Jet t = jmul(jneg(f[C1s]),
+               jadd(d[d1F2], jscale(f[F2], cscale(R, cadd(bg.mu0, cscale(2.0, mu0b))))));
+  t = jadd(t, jmul(jscaler(f[B1s], 0.5), d[ep1F2]));
+  t = jadd(t, jmul(f[F1], f[Pi2s]));
+  t = jadd(t, jmul(jneg(f[F1]), d[d2C2s]));
+  t = jadd(t, jmul(f[F1], d[ep1B2s]));
+  t = jadd(t, jmul(f[F1], jscale(f[B2s], cscale(-0.5 * R, cadd(bg.pi0, tau0b)))));
+  T = t;
+}
+
+__device__ __forceinline__ const double* node_ptr(const SoArgs& a, const SoInstance& I) {
+  return a.nodes + I.node_off;
+}
+
+// element (mode, field, r, t) of a 13-field buffer
+__device__ __forceinline__ long long fidx(const SoInstance& I, int mode, int f, int r, int t) {
+  return I.base + ((static_cast<long long>(mode) * 13 + f) * I.N + r) * I.T + t;
+}
+// element (mode, slot, r, t) of a scratch array with S slots
+__device__ __forceinline__ long long sidx(const SoInstance& I, int S, int mode, int s, int r,
+                                          int t) {
+  return I.pbase * S + ((static_cast<long long>(mode) * S + s) * I.N + r) * I.T + t;
+}
+
+// out[t] = sum_t' Op[t][t'] in[t'] (in: one theta row), zero where inactive.
+__device__ __forceinline__ cdouble row_op(const double* ops, int off, const cdouble* in, int t,
+                                          int T) {
+  if (off < 0) return cmake(0.0, 0.0);
+  const double* o = ops + off + static_cast<long long>(t) * T;
+  double re = 0.0, im = 0.0;
+  for (int k = 0; k < T; ++k) {
+    re += o[k] * in[k].re;
+    im += o[k] * in[k].im;
+  }
+  return cmake(re, im);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ linear triple
+// KIND 0: stage fields 0..2 -> out 0..2, D Psi -> dpsi      (first order)
+// KIND 1: out fields 0..2 -> tan 0..2, D Psi -> dpsi_t      (its tangent)
+// KIND 2: stage fields 10..12 -> out 10..12 with the forcing (second order)
+// full_spatial.hpp:99-398 with the band projection of spatial_pipeline.hpp:
+// 756-766 and, for KIND 2, the outer operator and coordinate forcing of
+// source_spatial.hpp:357-425, second_order.hpp:152-182.
+template <int SCHEME, int KIND>
+__global__ __launch_bounds__(kSoThreads) void so_linear_kernel(SoArgs a) {
+  extern __shared__ __align__(16) double smem[];
+  const SoTile tile = a.tiles[blockIdx.x];
+  const SoInstance& I = a.inst[tile.inst];
+  const int N = I.N, T = I.T, mode = tile.mode, tid = threadIdx.x;
+  const int rows = tile.hi - tile.lo, mrows = tile.mhi - tile.mlo, orows = tile.t1 - tile.t0;
+  const int us = 3 * T;
+  const int coord = KIND == 2 ? kCoSecond : kCoFirst;
+  SchemeTable* TS = reinterpret_cast<SchemeTable*>(smem);
+  cdouble* U = reinterpret_cast<cdouble*>(smem + kSchemeTableDoubles);
+  cdouble* QE = U + rows * us;
+  cdouble* PT = QE + mrows * T;
+  cdouble* OUT = PT + mrows * T;
+  const double* nodes = node_ptr(a, I);
+  const double* cos_th = nodes + 2 * T;
+  const double* sin_th = nodes + 3 * T;
+
+  for (int j = tid; j < static_cast<int>(sizeof(SchemeTable) / 8); j += kSoThreads)
+    smem[j] = reinterpret_cast<const double*>(a.tables + I.scheme)[j];
+  auto in_at = [&](int f, int r, int t) -> cdouble {
+    if constexpr (KIND == 0) return a.in[fidx(I, mode, f, r, t)];
+    if constexpr (KIND == 1) return a.out[fidx(I, mode, f, r, t)];
+    return a.in[fidx(I, mode, 10 + f, r, t)];
+  };
+  for (int e = tid; e < rows * us; e += kSoThreads) {
+    const int row = e / us, rem = e - row * us, f = rem / T, t = rem - f * T;
+    U[e] = in_at(f, tile.lo + row, t);
+  }
+  __syncthreads();
+  const SchemeTable& TB = *TS;
+  const bool constrained = I.constrained != 0;
+  const int m = I.modes[mode];
+  for (int e = tid; e < mrows * T; e += kSoThreads) {
+    const int jj = e / T, t = e - jj * T, j = tile.mlo + jj;
+    const cdouble* u = U + (j - tile.lo) * us;
+    const cdouble qe = constrained ? apply_d<SCHEME>(TB, U + 2 * T + t, us, tile.lo, j, N, I.inv_h)
+                                   : u[T + t];
+    QE[e] = qe;
+    const RadialCoefficients rc = radial_coefficients(I.mass, I.spin, I.length, -2, m, I.h * j);
+    const double ct = time_coefficient(rc, I.spin, sin_th[t]);
+    const cdouble g = definition_coefficient(rc, I.spin, -2, cos_th[t]);
+    const cdouble num = csub(cadd(u[t], cscale(2.0 * rc.adv, qe)), cmul(g, u[2 * T + t]));
+    PT[e] = cdivr(num, ct);
+  }
+  __syncthreads();
+  const int lap_off = I.op[coord][mode][kOpLaplacian];
+  const double damping = constrained ? 0.0 : I.gamma;
+  const bool forced = KIND == 2 && I.op[kCoSecond][mode][kOpProject] >= 0;
+  const double a_spin = I.spin, L = I.length, L2 = L * L, L4 = L2 * L2;
+  const double* cos_x = nodes;
+  const double* sin_x = nodes + T;
+  for (int e = tid; e < orows * T; e += kSoThreads) {
+    const int rr = e / T, t = e - rr * T, r = tile.t0 + rr;
+    const cdouble* u = U + (r - tile.lo) * us;
+    const cdouble dpsi = apply_d<SCHEME>(TB, U + 2 * T + t, us, tile.lo, r, N, I.inv_h);
+    const cdouble dq = apply_d<SCHEME>(TB, QE + t, T, tile.mlo, r, N, I.inv_h);
+    const cdouble dpt = apply_d<SCHEME>(TB, PT + t, T, tile.mlo, r, N, I.inv_h);
+    const cdouble qe = QE[(r - tile.mlo) * T + t];
+    const cdouble lap = row_op(a.ops, lap_off, u + 2 * T, t, T);
+    const RadialCoefficients rc = radial_coefficients(I.mass, I.spin, I.length, -2, m, I.h * r);
+    cdouble pt = cadd(cadd(cadd(cscale(rc.principal, dq), cmul(rc.q, qe)), cmul(rc.psi, u[2 * T + t])),
+                      lap);
+    if constexpr (KIND == 2) {
+      cdouble frc = cmake(0.0, 0.0);
+      if (forced && a.active[tile.inst]) {
+        const double R = I.h * r;
+        const cdouble* dcol = a.src + sidx(I, kSlotsSrc, mode, 0, 0, t);
+        const cdouble Dv = dcol[static_cast<long long>(r) * T];
+        const cdouble Dt = a.src[sidx(I, kSlotsSrc, mode, 1, r, t)];
+        const cdouble Tv = a.src[sidx(I, kSlotsSrc, mode, 2, r, t)];
+        const cdouble ethT = a.src[sidx(I, kSlotsSrc, mode, 3, r, t)];
+        const cdouble Dr = apply_d<SCHEME>(TB, dcol, T, 0, r, N, I.inv_h);
+        const cdouble d3D = delta_n(Dv, Dt, Dr, 3, R, I.mass, L);
+        const Bg bg = kerr_bg(I.mass, a_spin, L, R, cos_x[t], sin_x[t]);
+        cdouble s = d3D;
+        s = cadd(s, cmul(cscale(R, cadd(cscale(4.0, bg.mu0), cconj(bg.mu0))), Dv));
+        s = cadd(s, cscale(R, ethT));
+        s = cadd(s, cmul(cscale(R * R, csub(cscale(4.0, bg.pi0), cconj(bg.tau0))), Tv));
+        const double norm = 2.0 * (L4 + a_spin * a_spin * R * R * cos_x[t] * cos_x[t]);
+        frc = cscale(norm, s);
+      }
+      pt = cadd(pt, frc);
+    }
+    cdouble qt = csub(dpt, cscale(damping, csub(u[T + t], dpsi)));
+    cdouble st = PT[(r - tile.mlo) * T + t];
+    if (I.dissipate) {
+      pt = cadd(pt, apply_diss<SCHEME>(TB, U + t, us, tile.lo, r, N, I.h, I.eps));
+      qt = cadd(qt, apply_diss<SCHEME>(TB, U + T + t, us, tile.lo, r, N, I.h, I.eps));
+      st = cadd(st, apply_diss<SCHEME>(TB, U + 2 * T + t, us, tile.lo, r, N, I.h, I.eps));
+    }
+    cdouble* o = OUT + rr * us;
+    o[t] = pt;
+    o[T + t] = qt;
+    o[2 * T + t] = st;
+    if constexpr (KIND == 0) a.dpsi[sidx(I, 1, mode, 0, r, t)] = dpsi;
+    if constexpr (KIND == 1) a.dpsi_t[sidx(I, 1, mode, 0, r, t)] = dpsi;
+  }
+  __syncthreads();
+  const int proj_off = I.op[coord][mode][kOpProject];
+  for (int e = tid; e < orows * us; e += kSoThreads) {
+    const int rr = e / us, rem = e - rr * us, f = rem / T, t = rem - f * T, r = tile.t0 + rr;
+    const cdouble v = row_op(a.ops, proj_off, OUT + rr * us + f * T, t, T);
+    if constexpr (KIND == 0) a.out[fidx(I, mode, f, r, t)] = v;
+    if constexpr (KIND == 1) a.tan[sidx(I, kSlotsTan, mode, f, r, t)] = v;
+    if constexpr (KIND == 2) a.out[fidx(I, mode, 10 + f, r, t)] = v;
+  }
+}
+
+// ------------------------------------------------------------ radial D
+// D of the seven reconstruction fields of the stage (which 0) or of the RHS
+// (which 1) (full_spatial.hpp:400-455).  grid.y = instance.
+template <int SCHEME>
+__global__ __launch_bounds__(kSoThreads) void so_drad_kernel(SoArgs a, int which) {
+  const SoInstance& I = a.inst[a.first_inst + blockIdx.y];
+  if (I.scheme != SCHEME) return;
+  const cdouble* src = which == 0 ? a.in : a.out;
+  cdouble* dst = which == 0 ? a.rdr : a.rdr_t;
+  const SchemeTable& TB = a.tables[SCHEME];
+  const long long total = static_cast<long long>(I.M) * 7 * I.N * I.T;
+  for (long long e = static_cast<long long>(blockIdx.x) * kSoThreads + threadIdx.x; e < total;
+       e += static_cast<long long>(gridDim.x) * kSoThreads) {
+    const int t = static_cast<int>(e % I.T);
+    const long long q = e / I.T;
+    const int r = static_cast<int>(q % I.N);
+    const int mc = static_cast<int>(q / I.N);
+    const int c = mc % 7, mode = mc / 7;
+    const cdouble* col = src + fidx(I, mode, 3 + c, 0, t);
+    dst[sidx(I, kSlotsRdr, mode, c, r, t)] = apply_d<SCHEME>(TB, col, I.T, 0, r, I.N, I.inv_h);
+  }
+}
+
+// ------------------------------------------------------------ reconstruction
+// One workgroup per (instance, radial row), all modes and nodes in LDS.
+// KIND 0: chain of (stage, k) -> k[3..9] and the stage angular inputs, plus
+//         the independent constraints when asked.
+// KIND 1: chain of (k, tangent) in LDS, then the quadratic source.
+// Reconstruction passes: full_spatial.hpp:486-693, reconstruction.hpp:48-105;
+// chain order: spatial_pipeline.hpp:664-754; constraints: 768-893,
+// reconstruction_constraints.hpp:24-56; source inputs: 934-1181,
+// source_tangent_spatial.hpp:65-224; projection: 1183-1209.
+template <int KIND>
+__global__ __launch_bounds__(kSoThreads) void so_recon_kernel(SoArgs a) {
+  extern __shared__ __align__(16) double smem[];
+  const int inst = a.first_inst + blockIdx.y;
+  const SoInstance& I = a.inst[inst];
+  const int r = blockIdx.x;
+  if (r >= I.N) return;
+  const int M = I.M, T = I.T, MT = M * T, tid = threadIdx.x;
+  const double R = I.h * r, R2 = R * R, R3 = R2 * R;
+  const double mass = I.mass, aspin = I.spin, L = I.length;
+  const double* nodes = node_ptr(a, I);
+  const double* cos_x = nodes;
+  const double* sin_x = nodes + T;
+  const double* cos_th = nodes + 2 * T;
+  const double* sin_th = nodes + 3 * T;
+  cdouble* X = reinterpret_cast<cdouble*>(smem);  // [8][MT] F G La H B Pi C U of `in`
+  cdouble* XT = X + 8 * MT;                       // [8][MT] their time derivatives
+  cdouble* ANG = XT + 8 * MT;                     // [6][MT]
+  cdouble* RAW = ANG + 6 * MT;                    // [4][MT]
+  enum { xF = 0, xG, xLa, xH, xB, xPi, xC, xU };
+  enum { aEth1F = 0, aEth2G, aEth2C, aEth2Pi, aEthP1BS, aEthP2CS };
+  // state field of LDS slot k: F is field 2, G..U are fields 3..9
+  auto slot_of = [](int k) { return 2 + k; };
+  auto at = [&](cdouble* base, int k, int mode) { return base + k * MT + mode * T; };
+  auto op = [&](int coord, int mode, int kind) { return I.op[coord][mode][kind]; };
+  // eth / eth' of rows (x, dt) of coordinator `coord` for every (mode, t)
+  auto ghp = [&](int coord, bool prime, const cdouble* xb, const cdouble* db, cdouble* ob) {
+    for (int e = tid; e < MT; e += kSoThreads) {
+      const int mode = e / T, t = e - mode * T;
+      const int off = op(coord, mode, prime ? kOpLower : kOpRaise);
+      if (off < 0) {
+        ob[e] = cmake(0.0, 0.0);
+        continue;
+      }
+      const cdouble ang = row_op(a.ops, off, xb + mode * T, t, T);
+      ob[e] = eth_point(prime, xb[e], db[e], ang, I.coord_spin[coord], I.coord_boost[coord], R,
+                        sin_x[t], cos_x[t], aspin, L);
+    }
+  };
+  auto project = [&](int coord, const cdouble* in, cdouble* out) {
+    for (int e = tid; e < MT; e += kSoThreads) {
+      const int mode = e / T, t = e - mode * T;
+      out[e] = row_op(a.ops, op(coord, mode, kOpProject), in + mode * T, t, T);
+    }
+  };
+
+  // stage the row: `in` fields and the first-order time derivative
+  for (int e = tid; e < 8 * MT; e += kSoThreads) {
+    const int k = e / MT, rem = e - k * MT, mode = rem / T, t = rem - mode * T;
+    X[e] = KIND == 0 ? a.in[fidx(I, mode, slot_of(k), r, t)] : a.out[fidx(I, mode, slot_of(k), r, t)];
+  }
+  for (int e = tid; e < MT; e += kSoThreads) {
+    const int mode = e / T, t = e - mode * T;
+    XT[e] = KIND == 0 ? a.out[fidx(I, mode, 2, r, t)] : a.tan[sidx(I, kSlotsTan, mode, 2, r, t)];
+  }
+  __syncthreads();
+  const cdouble* rdr = KIND == 0 ? a.rdr : a.rdr_t;
+  auto dr = [&](int mode, int c, int t) { return rdr[sidx(I, kSlotsRdr, mode, c, r, t)]; };
+  ghp(kCoFirst, false, at(X, xF, 0), at(XT, xF, 0), at(ANG, aEth1F, 0));
+  __syncthreads();
+  // pass 1: G, Lambda
+  for (int e = tid; e < MT; e += kSoThreads) {
+    const int mode = e / T, t = e - mode * T;
+    const Bg bg = kerr_bg(mass, aspin, L, R, cos_th[t], sin_th[t]);
+    const cdouble g = X[xG * MT + e], lam = X[xLa * MT + e], f = X[xF * MT + e];
+    const cdouble dg = csub(cadd(cmul(cscale(-4.0 * R, bg.mu0), g), ANG[aEth1F * MT + e]),
+                            cmul(cscale(R, bg.tau0), f));
+    const cdouble dl = csub(cmul(cscale(-R, cadd(bg.mu0, cconj(bg.mu0))), lam), f);
+    RAW[0 * MT + e] = recon_dt(g, dr(mode, 0, t), dg, 2, R, mass, L);
+    RAW[1 * MT + e] = recon_dt(lam, dr(mode, 1, t), dl, 1, R, mass, L);
+  }
+  __syncthreads();
+  project(kCoG, RAW, at(XT, xG, 0));
+  project(kCoFirst, RAW + MT, at(XT, xLa, 0));
+  __syncthreads();
+  ghp(kCoG, false, at(X, xG, 0), at(XT, xG, 0), at(ANG, aEth2G, 0));
+  __syncthreads();
+  // pass 2: H, B, Pi, C
+  for (int e = tid; e < MT; e += kSoThreads) {
+    const int mode = e / T, t = e - mode * T;
+    const Bg bg = kerr_bg(mass, aspin, L, R, cos_th[t], sin_th[t]);
+    const cdouble mub = cconj(bg.mu0), pib = cconj(bg.pi0);
+    const cdouble g = X[xG * MT + e], lam = X[xLa * MT + e], hh = X[xH * MT + e];
+    const cdouble b = X[xB * MT + e], pi = X[xPi * MT + e], c = X[xC * MT + e];
+    const cdouble dh = csub(cadd(cmul(cscale(-3.0 * R, bg.mu0), hh), ANG[aEth2G * MT + e]),
+                            cmul(cscale(2.0 * R, bg.tau0), g));
+    const cdouble db = csub(cmul(cscale(R, csub(bg.mu0, mub)), b), cscale(2.0, lam));
+    const cdouble pt = cadd(pib, bg.tau0);
+    const cdouble dpi = cadd(csub(cneg(g), cmul(cscale(R, pt), lam)),
+                             cmul(cmul(cscale(0.5 * R2, bg.mu0), pt), b));
+    const cdouble dc = csub(csub(cmul(cscale(-R, mub), c), cscale(2.0, pi)),
+                            cmul(cscale(R, bg.tau0), b));
+    RAW[0 * MT + e] = recon_dt(hh, dr(mode, 2, t), dh, 3, R, mass, L);
+    RAW[1 * MT + e] = recon_dt(b, dr(mode, 3, t), db, 1, R, mass, L);
+    RAW[2 * MT + e] = recon_dt(pi, dr(mode, 4, t), dpi, 2, R, mass, L);
+    RAW[3 * MT + e] = recon_dt(c, dr(mode, 5, t), dc, 2, R, mass, L);
+  }
+  __syncthreads();
+  project(kCoScalar, RAW, at(XT, xH, 0));
+  project(kCoB, RAW + MT, at(XT, xB, 0));
+  project(kCoPi, RAW + 2 * MT, at(XT, xPi, 0));
+  project(kCoC, RAW + 3 * MT, at(XT, xC, 0));
+  __syncthreads();
+  ghp(kCoC, false, at(X, xC, 0), at(XT, xC, 0), at(ANG, aEth2C, 0));
+  ghp(kCoPi, false, at(X, xPi, 0), at(XT, xPi, 0), at(ANG, aEth2Pi, 0));
+  // sharp fields conj(X[-m]) (spatial_pipeline.hpp:895-932)
+  for (int e = tid; e < MT; e += kSoThreads) {
+    const int mode = e / T, t = e - mode * T, se = I.sharp[mode] * T + t;
+    RAW[0 * MT + e] = cconj(X[xB * MT + se]);
+    RAW[1 * MT + e] = cconj(X[xC * MT + se]);
+    RAW[2 * MT + e] = cconj(XT[xB * MT + se]);
+    RAW[3 * MT + e] = cconj(XT[xC * MT + se]);
+  }
+  __syncthreads();
+  ghp(kCoBSharp, true, RAW, RAW + 2 * MT, at(ANG, aEthP1BS, 0));
+  ghp(kCoCSharp, true, RAW + MT, RAW + 3 * MT, at(ANG, aEthP2CS, 0));
+  __syncthreads();
+  // pass 3: U
+  for (int e = tid; e < MT; e += kSoThreads) {
+    const int mode = e / T, t = e - mode * T, se = I.sharp[mode] * T + t;
+    const Bg bg = kerr_bg(mass, aspin, L, R, cos_th[t], sin_th[t]);
+    const cdouble mu0b = cconj(bg.mu0), pi0b = cconj(bg.pi0), tau0b = cconj(bg.tau0);
+    const cdouble hh = X[xH * MT + e], pi = X[xPi * MT + e], c = X[xC * MT + e];
+    const cdouble u = X[xU * MT + e];
+    const cdouble pi_s = cconj(X[xPi * MT + se]), b_s = cconj(X[xB * MT + se]),
+                  c_s = cconj(X[xC * MT + se]);
+    cdouble du = cmul(cscale(-R, mu0b), u);
+    du = cadd(du, cmul(cscale(-R, bg.mu0), ANG[aEth2C * MT + e]));
+    du = cadd(du, cmul(cmul(cscale(-R2, bg.mu0), cadd(pi0b, cscale(2.0, bg.tau0))), c));
+    du = cadd(du, cscale(-2.0, ANG[aEth2Pi * MT + e]));
+    du = cadd(du, cmul(cscale(-2.0 * R, pi0b), pi));
+    du = cadd(du, cscale(-2.0, hh));
+    du = cadd(du, cmul(cscale(-2.0 * R, bg.pi0), pi_s));
+    du = cadd(du, cmul(cscale(-R, bg.pi0), ANG[aEthP1BS * MT + e]));
+    du = cadd(du, cmul(cmul(cscale(R2, bg.pi0), bg.pi0), b_s));
+    du = cadd(du, cmul(cscale(R, bg.mu0), ANG[aEthP2CS * MT + e]));
+    const cdouble sc = cadd(csub(cscale(-3.0, cmul(bg.mu0, bg.pi0)), cscale(-2.0, cmul(mu0b, bg.pi0))),
+                            cscale(-2.0, cmul(bg.mu0, tau0b)));
+    du = cadd(du, cmul(cscale(R2, sc), c_s));
+    RAW[e] = recon_dt(u, dr(mode, 6, t), du, 3, R, mass, L);
+  }
+  __syncthreads();
+  project(kCoScalar, RAW, at(XT, xU, 0));
+  __syncthreads();
+
+  if constexpr (KIND == 0) {
+    for (int e = tid; e < 7 * MT; e += kSoThreads) {
+      const int k = 1 + e / MT, rem = e % MT, mode = rem / T, t = rem - mode * T;
+      a.out[fidx(I, mode, slot_of(k), r, t)] = XT[k * MT + rem];
+    }
+    for (int e = tid; e < 4 * MT; e += kSoThreads) {
+      const int k = e / MT, rem = e - k * MT, mode = rem / T, t = rem - mode * T;
+      const int src = k == kRangEth2C ? aEth2C
+                      : k == kRangEthP2CSharp ? aEthP2CS
+                      : k == kRangEth2G ? aEth2G
+                                        : aEthP1BS;
+      a.rang[sidx(I, kSlotsRang, mode, k, r, t)] = ANG[src * MT + rem];
+    }
+    if (!a.want_constraints) return;
+    ghp(kCoG, true, at(X, xG, 0), at(XT, xG, 0), RAW);
+    ghp(kCoScalar, true, at(X, xH, 0), at(XT, xH, 0), RAW + MT);
+    __syncthreads();
+    for (int e = tid; e < MT; e += kSoThreads) {
+      const int mode = e / T, t = e - mode * T, se = I.sharp[mode] * T + t;
+      const Bg bg = kerr_bg(mass, aspin, L, R, cos_x[t], sin_x[t]);
+      const cdouble F = X[xF * MT + e], G = X[xG * MT + e], H = X[xH * MT + e];
+      const cdouble La = X[xLa * MT + e], Pi = X[xPi * MT + e], B = X[xB * MT + e];
+      const cdouble C = X[xC * MT + e], Uv = X[xU * MT + e];
+      const cdouble Us = cconj(X[xU * MT + se]);
+      const int m = I.modes[mode];
+      const cdouble th1 = thorn_n(F, XT[xF * MT + e], a.dpsi[sidx(I, 1, mode, 0, r, t)], 1, -2, -2, m,
+                                  R, cos_x[t], mass, aspin, L, bg.eps0);
+      const cdouble th2 = thorn_n(G, XT[xG * MT + e], a.rdr[sidx(I, kSlotsRdr, mode, 0, r, t)], 2, -1,
+                                  -1, m, R, cos_x[t], mass, aspin, L, bg.eps0);
+      const cdouble l3 = cadd(cadd(cscale(R, RAW[e]), cmul(cscale(4.0 * R2, bg.pi0), G)),
+                              cmul(bg.rho0, F));
+      const cdouble r3 = cadd(th1, cmul(cscale(3.0 * R2, bg.psi20), La));
+      const cdouble bgt = cmul(bg.psi20, csub(csub(cmul(cscale(-3.0 * R3, bg.mu0), C),
+                                                   cmul(cscale(1.5 * R3, bg.tau0), B)),
+                                              cscale(3.0 * R2, Pi)));
+      const cdouble l2 = csub(th2, cmul(cscale(2.0, bg.rho0), G));
+      const cdouble r2 = csub(cadd(cscale(R, RAW[MT + e]), cmul(cscale(3.0 * R2, bg.pi0), H)), bgt);
+      const cdouble lh = cdiv(Uv, bg.mu0);
+      const cdouble rh = cdiv(Us, cconj(bg.mu0));
+      const cdouble tv[6] = {l3, r3, l2, r2, lh, rh};
+      for (int k = 0; k < 3; ++k)
+        a.cons[sidx(I, kSlotsCons, mode, k, r, t)] = csub(tv[2 * k], tv[2 * k + 1]);
+      for (int k = 0; k < 6; ++k) a.terms[sidx(I, kSlotsTerms, mode, k, r, t)] = tv[k];
+    }
+    return;
+  } else {
+    // ---- quadratic source: X = k, XT = tangent, ANG = tangent chain inputs
+    cdouble* EX = RAW + 4 * MT;          // [4][MT] eth1 B, eth'1 F and their tangents
+    Jet* DJ = reinterpret_cast<Jet*>(EX + 4 * MT);  // [11][MT]
+    cdouble* SUM = reinterpret_cast<cdouble*>(DJ + 11 * MT);  // [targets][4][T]
+    cdouble* PRJ = SUM + 4 * I.n_targets * T;                  // [targets][4][T]
+    // stage B and F rows for the eth of the stage
+    for (int e = tid; e < MT; e += kSoThreads) {
+      const int mode = e / T, t = e - mode * T;
+      RAW[e] = a.in[fidx(I, mode, 6, r, t)];
+      RAW[MT + e] = a.in[fidx(I, mode, 2, r, t)];
+    }
+    __syncthreads();
+    ghp(kCoB, false, RAW, at(X, xB, 0), EX);
+    ghp(kCoFirst, true, RAW + MT, at(X, xF, 0), EX + MT);
+    ghp(kCoB, false, at(X, xB, 0), at(XT, xB, 0), EX + 2 * MT);
+    ghp(kCoFirst, true, at(X, xF, 0), at(XT, xF, 0), EX + 3 * MT);
+    __syncthreads();
+    auto S = [&](int mode, int k, int t) { return a.in[fidx(I, mode, slot_of(k), r, t)]; };
+    auto rd = [&](int mode, int c, int t) { return a.rdr[sidx(I, kSlotsRdr, mode, c, r, t)]; };
+    auto rdt = [&](int mode, int c, int t) { return a.rdr_t[sidx(I, kSlotsRdr, mode, c, r, t)]; };
+    auto rg = [&](int mode, int k, int t) { return a.rang[sidx(I, kSlotsRang, mode, k, r, t)]; };
+    for (int e = tid; e < MT; e += kSoThreads) {
+      const int mode = e / T, t = e - mode * T, s = I.sharp[mode];
+      auto dl = [&](int k, int n, int c) {
+        return Jet{delta_n(S(mode, k, t), X[k * MT + e], rd(mode, c, t), n, R, mass, L),
+                   delta_n(X[k * MT + e], XT[k * MT + e], rdt(mode, c, t), n, R, mass, L)};
+      };
+      DJ[0 * MT + e] = {delta_n(S(mode, xF, t), X[xF * MT + e], a.dpsi[sidx(I, 1, mode, 0, r, t)], 1,
+                                R, mass, L),
+                        delta_n(X[xF * MT + e], XT[xF * MT + e], a.dpsi_t[sidx(I, 1, mode, 0, r, t)],
+                                1, R, mass, L)};
+      DJ[1 * MT + e] = dl(xU, 3, 6);
+      DJ[2 * MT + e] = {rg(mode, kRangEth2C, t), ANG[aEth2C * MT + e]};
+      DJ[3 * MT + e] = {rg(mode, kRangEthP2CSharp, t), ANG[aEthP2CS * MT + e]};
+      DJ[4 * MT + e] = {EX[e], EX[2 * MT + e]};
+      DJ[5 * MT + e] = dl(xC, 2, 5);
+      DJ[6 * MT + e] = dl(xG, 2, 0);
+      DJ[7 * MT + e] = {rg(mode, kRangEth2G, t), ANG[aEth2G * MT + e]};
+      DJ[8 * MT + e] = {EX[MT + e], EX[3 * MT + e]};
+      const int se = s * T + t;
+      DJ[9 * MT + e] = {delta_n(cconj(S(s, xC, t)), cconj(X[xC * MT + se]), cconj(rd(s, 5, t)), 2, R,
+                                mass, L),
+                        delta_n(cconj(X[xC * MT + se]), cconj(XT[xC * MT + se]), cconj(rdt(s, 5, t)),
+                                2, R, mass, L)};
+      DJ[10 * MT + e] = {rg(mode, kRangEthP1BSharp, t), ANG[aEthP1BS * MT + e]};
+    }
+    __syncthreads();
+    // ordered pairs, target-major (source_tangent_spatial.hpp:137-219)
+    const int NT = I.n_targets;
+    for (int e = tid; e < NT * T; e += kSoThreads) {
+      const int ts = e / T, t = e - ts * T;
+      const Bg bg = kerr_bg(mass, aspin, L, R, cos_x[t], sin_x[t]);
+      Jet totD{cmake(0.0, 0.0), cmake(0.0, 0.0)}, totT = totD;
+      for (int k = I.pair_begin[ts]; k < I.pair_begin[ts + 1]; ++k) {
+        const int m1 = I.pair_m1[k], m2 = I.pair_m2[k];
+        const int s1 = I.sharp[m1], s2 = I.sharp[m2];
+        auto fj = [&](int mode, int k2) {
+          return Jet{S(mode, k2, t), X[k2 * MT + mode * T + t]};
+        };
+        const Jet f[19] = {fj(m1, xF),  fj(m1, xG),  fj(m1, xLa), fj(m1, xPi), fj(m1, xB),
+                           fj(m1, xC),  fj(m1, xU),  fj(m2, xF),  fj(m2, xG),  fj(m2, xH),
+                           fj(m2, xB),  fj(m2, xC),  fj(m2, xU),  jconj(fj(s2, xU)),
+                           jconj(fj(s2, xC)), jconj(fj(s1, xC)), jconj(fj(s1, xB)),
+                           jconj(fj(s2, xPi)), jconj(fj(s2, xB))};
+        Jet d[11];
+        for (int q = 0; q < 11; ++q) d[q] = DJ[q * MT + m2 * T + t];
+        Jet Dp, Tp;
+        pair_source(R, bg, f, d, Dp, Tp);
+        totD = jadd(totD, Dp);
+        totT = jadd(totT, Tp);
+      }
+      SUM[(ts * 4 + 0) * T + t] = totD.v;
+      SUM[(ts * 4 + 1) * T + t] = totD.d;
+      SUM[(ts * 4 + 2) * T + t] = totT.v;
+      SUM[(ts * 4 + 3) * T + t] = totT.d;
+    }
+    __syncthreads();
+    for (int e = tid; e < NT * 4 * T; e += kSoThreads) {
+      const int ts = e / (4 * T), rem = e - ts * 4 * T, c = rem / T, t = rem - c * T;
+      const int mode = I.target_mode[ts];
+      const int coord = c < 2 ? kCoSecond : kCoT;
+      PRJ[e] = row_op(a.ops, op(coord, mode, kOpProject), SUM + (ts * 4 + c) * T, t, T);
+    }
+    __syncthreads();
+    for (int e = tid; e < NT * T; e += kSoThreads) {
+      const int ts = e / T, t = e - ts * T;
+      const int mode = I.target_mode[ts];
+      const int off = op(kCoT, mode, kOpLower);
+      const cdouble* tv = PRJ + (ts * 4 + 2) * T;
+      const cdouble* tt = PRJ + (ts * 4 + 3) * T;
+      cdouble ethT = cmake(0.0, 0.0);
+      if (off >= 0)
+        ethT = eth_point(true, tv[t], tt[t], row_op(a.ops, off, tv, t, T), I.coord_spin[kCoT],
+                         I.coord_boost[kCoT], R, sin_x[t], cos_x[t], aspin, L);
+      a.src[sidx(I, kSlotsSrc, mode, 0, r, t)] = PRJ[(ts * 4 + 0) * T + t];
+      a.src[sidx(I, kSlotsSrc, mode, 1, r, t)] = PRJ[(ts * 4 + 1) * T + t];
+      a.src[sidx(I, kSlotsSrc, mode, 2, r, t)] = tv[t];
+      a.src[sidx(I, kSlotsSrc, mode, 3, r, t)] = ethT;
+    }
+  }
+}
+
+// ------------------------------------------------------------ RK4 combination
+// Classical RK4 (device_rk4.hpp:84-128) on whole instance blocks:
+//   stage 1: acc = y + dt/6 k, Y2 = y + dt/2 k;   stage 2: acc += dt/3 k,
+//   Y3 = y + dt/2 k;   stage 3: acc += dt/3 k, Y4 = y + dt k;
+//   stage 4: y = acc + dt/6 k.   grid.y = instance.
+__global__ __launch_bounds__(kSoThreads) void so_rk_kernel(SoRkArgs a) {
+  const SoInstance& I = a.inst[blockIdx.y];
+  const double dt = a.dt[blockIdx.y];
+  const long long n = static_cast<long long>(I.M) * 13 * I.N * I.T;
+  for (long long e = static_cast<long long>(blockIdx.x) * kSoThreads + threadIdx.x; e < n;
+       e += static_cast<long long>(gridDim.x) * kSoThreads) {
+    const long long g = I.base + e;
+    const cdouble k = a.k[g];
+    if (a.stage == 1) {
+      const cdouble y = a.y[g];
+      a.acc[g] = cadd(y, cscale(dt / 6.0, k));
+      a.yout[g] = cadd(y, cscale(0.5 * dt, k));
+    } else if (a.stage == 2) {
+      a.acc[g] = cadd(a.acc[g], cscale(dt / 3.0, k));
+      a.yout[g] = cadd(a.y[g], cscale(0.5 * dt, k));
+    } else if (a.stage == 3) {
+      a.acc[g] = cadd(a.acc[g], cscale(dt / 3.0, k));
+      a.yout[g] = cadd(a.y[g], cscale(dt, k));
+    } else {
+      a.yout[g] = cadd(a.acc[g], cscale(dt / 6.0, k));
+    }
+  }
+}
+
+// ------------------------------------------------------------ constraint norms
+// One workgroup per instance; fixed-order tree reductions, so the sums do not
+// depend on scheduling (the reference reduces with atomics,
+// source_activation.hpp:120-170).  Output per instance: weighted sums of
+// c^2, l^2, r^2 (3 each), max c^2 (3), max natural (3), max normalized^2 (3),
+// finiteness flag.
+__global__ __launch_bounds__(kSoThreads) void so_reduce_kernel(SoReduceArgs a) {
+  __shared__ double red[kSoThreads];
+  const SoInstance& I = a.inst[blockIdx.x];
+  const int tid = threadIdx.x;
+  const SchemeTable& TB = a.tables[I.scheme];
+  const int W = TB.W;
+  const double* wts = a.nodes + I.node_off + 4 * I.T;
+  const long long plane = static_cast<long long>(I.M) * I.N * I.T;
+  double acc[15];
+  for (double& v : acc) v = 0.0;
+  int finite = 1;
+  const double two_pi = 6.283185307179586;
+  for (long long e = tid; e < plane; e += kSoThreads) {
+    const int t = static_cast<int>(e % I.T);
+    const int r = static_cast<int>((e / I.T) % I.N);
+    const int mode = static_cast<int>(e / (static_cast<long long>(I.T) * I.N));
+    const int dist = r < I.N - 1 - r ? r : I.N - 1 - r;
+    const double hw = dist < W ? TB.norm[dist] : 1.0;
+    const double weight = I.h * hw * two_pi * wts[t];
+    for (int f = 0; f < 3; ++f) {
+      const cdouble cv = a.cons[sidx(I, kSlotsCons, mode, f, r, t)];
+      const cdouble lv = a.terms[sidx(I, kSlotsTerms, mode, 2 * f, r, t)];
+      const cdouble rv = a.terms[sidx(I, kSlotsTerms, mode, 2 * f + 1, r, t)];
+      const double c = hypot(cv.re, cv.im), l = hypot(lv.re, lv.im), rr = hypot(rv.re, rv.im);
+      if (!isfinite(c) || !isfinite(l) || !isfinite(rr)) {
+        finite = 0;
+        continue;
+      }
+      acc[f] += weight * c * c;
+      acc[3 + f] += weight * l * l;
+      acc[6 + f] += weight * rr * rr;
+      acc[9 + f] = fmax(acc[9 + f], c * c);
+      acc[12 + f] = fmax(acc[12 + f], fmax(l * l, rr * rr));
+    }
+  }
+  double* out = a.out + static_cast<long long>(blockIdx.x) * kSoReduceDoubles;
+  auto reduce = [&](double v, bool is_max) {
+    red[tid] = v;
+    __syncthreads();
+    for (int s = kSoThreads / 2; s > 0; s >>= 1) {
+      if (tid < s) red[tid] = is_max ? fmax(red[tid], red[tid + s]) : red[tid] + red[tid + s];
+      __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+  };
+  double total[15];
+  for (int k = 0; k < 15; ++k) total[k] = reduce(acc[k], k >= 9);
+  const double fin = reduce(static_cast<double>(1 - finite), true) < 0.5 ? 1.0 : 0.0;
+  const double floor = 64.0 * 2.220446049250313e-16;
+  double mnorm[3] = {0.0, 0.0, 0.0};
+  int finite2 = 1;
+  for (long long e = tid; e < plane; e += kSoThreads) {
+    const int t = static_cast<int>(e % I.T);
+    const int r = static_cast<int>((e / I.T) % I.N);
+    const int mode = static_cast<int>(e / (static_cast<long long>(I.T) * I.N));
+    for (int f = 0; f < 3; ++f) {
+      const cdouble cv = a.cons[sidx(I, kSlotsCons, mode, f, r, t)];
+      const cdouble lv = a.terms[sidx(I, kSlotsTerms, mode, 2 * f, r, t)];
+      const cdouble rv = a.terms[sidx(I, kSlotsTerms, mode, 2 * f + 1, r, t)];
+      const double c = hypot(cv.re, cv.im), l = hypot(lv.re, lv.im), rr = hypot(rv.re, rv.im);
+      const double den = l + rr + floor * sqrt(total[12 + f]);
+      const double nv = den > 0.0 ? c / den : 0.0;
+      if (isfinite(nv)) mnorm[f] = fmax(mnorm[f], nv * nv);
+      else finite2 = 0;
+    }
+  }
+  double m2[3];
+  for (int f = 0; f < 3; ++f) m2[f] = reduce(mnorm[f], true);
+  const double fin2 = reduce(static_cast<double>(1 - finite2), true) < 0.5 ? 1.0 : 0.0;
+  if (tid == 0) {
+    for (int k = 0; k < 15; ++k) out[k] = total[k];
+    for (int f = 0; f < 3; ++f) out[15 + f] = m2[f];
+    out[18] = fin * fin2;
+  }
+}
+
+// ------------------------------------------------------------ launchers
+namespace {
+template <int K>
+hipError_t launch_linear_k(int scheme, const SoArgs& a, int n, int lds, hipStream_t s) {
+  const dim3 g(n), b(kSoThreads);
+  switch (scheme) {
+    case 0: hipLaunchKernelGGL((so_linear_kernel<0, K>), g, b, lds, s, a); break;
+    case 1: hipLaunchKernelGGL((so_linear_kernel<1, K>), g, b, lds, s, a); break;
+    default: hipLaunchKernelGGL((so_linear_kernel<2, K>), g, b, lds, s, a); break;
+  }
+  return hipGetLastError();
+}
+}  // namespace
+
+hipError_t so_launch_linear(int scheme, int kind, const SoArgs& a, int n_tiles, int lds,
+                            hipStream_t s) {
+  if (n_tiles <= 0) return hipSuccess;
+  if (kind == 0) return launch_linear_k<0>(scheme, a, n_tiles, lds, s);
+  if (kind == 1) return launch_linear_k<1>(scheme, a, n_tiles, lds, s);
+  return launch_linear_k<2>(scheme, a, n_tiles, lds, s);
+}
+
+hipError_t so_launch_drad(int which, const SoArgs& a, long long points, hipStream_t s) {
+  if (a.n_inst <= 0) return hipSuccess;
+  const long long want = (points + kSoThreads - 1) / kSoThreads;
+  const dim3 g(static_cast<unsigned>(want < 1 ? 1 : (want > 4096 ? 4096 : want)), a.n_inst);
+  hipLaunchKernelGGL(so_drad_kernel<0>, g, dim3(kSoThreads), 0, s, a, which);
+  hipLaunchKernelGGL(so_drad_kernel<1>, g, dim3(kSoThreads), 0, s, a, which);
+  hipLaunchKernelGGL(so_drad_kernel<2>, g, dim3(kSoThreads), 0, s, a, which);
+  return hipGetLastError();
+}
+
+hipError_t so_launch_recon(int kind, const SoArgs& a, int rows, int lds, hipStream_t s) {
+  if (a.n_inst <= 0) return hipSuccess;
+  const dim3 g(rows, a.n_inst), b(kSoThreads);
+  if (kind == 0) hipLaunchKernelGGL(so_recon_kernel<0>, g, b, lds, s, a);
+  else hipLaunchKernelGGL(so_recon_kernel<1>, g, b, lds, s, a);
+  return hipGetLastError();
+}
+
+hipError_t so_launch_rk(const SoRkArgs& a, hipStream_t s) {
+  if (a.n_inst <= 0) return hipSuccess;
+  const long long want = (a.max_values + kSoThreads - 1) / kSoThreads;
+  const dim3 g(static_cast<unsigned>(want < 1 ? 1 : (want > 8192 ? 8192 : want)), a.n_inst);
+  hipLaunchKernelGGL(so_rk_kernel, g, dim3(kSoThreads), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t so_launch_reduce(const SoReduceArgs& a, hipStream_t s) {
+  if (a.n_inst <= 0) return hipSuccess;
+  hipLaunchKernelGGL(so_reduce_kernel, dim3(a.n_inst), dim3(kSoThreads), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t so_set_lds_limit(int bytes) {
+  const void* fns[] = {
+      reinterpret_cast<const void*>(&so_linear_kernel<0, 0>),
+      reinterpret_cast<const void*>(&so_linear_kernel<0, 1>),
+      reinterpret_cast<const void*>(&so_linear_kernel<0, 2>),
+      reinterpret_cast<const void*>(&so_linear_kernel<1, 0>),
+      reinterpret_cast<const void*>(&so_linear_kernel<1, 1>),
+      reinterpret_cast<const void*>(&so_linear_kernel<1, 2>),
+      reinterpret_cast<const void*>(&so_linear_kernel<2, 0>),
+      reinterpret_cast<const void*>(&so_linear_kernel<2, 1>),
+      reinterpret_cast<const void*>(&so_linear_kernel<2, 2>),
+      reinterpret_cast<const void*>(&so_recon_kernel<0>),
+      reinterpret_cast<const void*>(&so_recon_kernel<1>)};
+  for (const void* f : fns) {
+    const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+}  // namespace teuk

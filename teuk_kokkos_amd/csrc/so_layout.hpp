@@ -1,0 +1,134 @@
+// Descriptors of the second-order path (plans whose instances enable the
+// quadratic source): the reference's 13-field nodal state
+// (mode, field, radial, theta) evolved with its full stage graph
+// (spatial_pipeline.hpp:458-588).
+//
+// HBM layout, per instance, all complex128:
+//   state-like buffers (y, acc, Ya, Yb, k):  (M, 13, N, T) at SoInstance::base
+//   scratch arrays with S slots:             (M, S, N, T) at pbase * S
+// where pbase counts M*N*T values of the instances before this one.
+#pragma once
+
+#include "device_layout.hpp"
+
+namespace teuk {
+
+inline constexpr int kSoMaxModes = 16;
+inline constexpr int kSoMaxTargets = 16;
+inline constexpr int kSoMaxPairs = 64;
+inline constexpr int kSoThreads = 256;
+
+// Angular coordinators of the stage graph (spatial_pipeline.hpp:185-218):
+// (spin, boost, band) over the parent or target modes.
+enum SoCoord {
+  kCoFirst = 0,   // spin -2, boost -2, ell_max_first, parents
+  kCoSecond,      // spin -2, boost -2, ell_max_second, targets
+  kCoG,           // spin -1, boost -1
+  kCoB,           // spin -2, boost 0
+  kCoPi,          // spin -1, boost 0
+  kCoC,           // spin -1, boost 1
+  kCoScalar,      // spin 0, boost 0
+  kCoBSharp,      // spin 2, boost 0
+  kCoCSharp,      // spin 1, boost 1
+  kCoT,           // spin -1, boost -2, ell_max_second, targets
+  kSoCoords
+};
+// Per (coordinator, mode) T x T nodal operators (row-major, out = Op * in):
+// band projection S A, Laplacian S Lambda A, and the angular parts of eth and
+// eth': S' diag(raising or lowering factor) A with the ell alignment of the
+// spin-shifted band (ghp_device.hpp:174-196).
+enum SoOp { kOpProject = 0, kOpLaplacian, kOpRaise, kOpLower, kSoOps };
+
+struct SoInstance {
+  long long base;   // offset of the (M, 13, N, T) block in the state buffers
+  long long pbase;  // M * N * T summed over the instances before this one
+  int M, N, T, scheme;
+  int constrained, dissipate;
+  int n_targets;
+  int node_off;     // x | sqrt(1 - x^2) | cos(acos x) | sin(acos x), T each
+  double mass, spin, length, h, inv_h, gamma, eps;
+  int modes[kSoMaxModes];
+  int sharp[kSoMaxModes];
+  int target_mode[kSoMaxTargets];
+  int pair_begin[kSoMaxTargets + 1];
+  int pair_m1[kSoMaxPairs], pair_m2[kSoMaxPairs];
+  int coord_spin[kSoCoords], coord_boost[kSoCoords];
+  int op[kSoCoords][kSoMaxModes][kSoOps];  // double offset, -1 where inactive
+};
+
+// One radial tile of one (instance, mode) for the linear triple kernel:
+// output rows [t0, t1), psi_t rows [mlo, mhi), staged rows [lo, hi).
+struct SoTile {
+  int inst, mode;
+  int t0, t1, lo, hi, mlo, mhi;
+};
+
+// Scratch slot counts (per instance, per mode).
+enum {
+  kSlotsTan = 3,     // tangent RHS of the first-order triple
+  kSlotsRdr = 7,     // D of the seven reconstruction fields
+  kSlotsRang = 4,    // eth2 C, eth'2 C#, eth2 G, eth'1 B# of the stage chain
+  kSlotsSrc = 4,     // projected D, projected D_t, projected T, eth'3 T
+  kSlotsCons = 3,    // constraint residuals
+  kSlotsTerms = 6    // constraint left/right terms
+};
+enum { kRangEth2C = 0, kRangEthP2CSharp = 1, kRangEth2G = 2, kRangEthP1BSharp = 3 };
+
+struct SoArgs {
+  const SoInstance* inst;
+  const double* ops;
+  const double* nodes;
+  const SchemeTable* tables;  // D4-2, D8-4, D10-5
+  const SoTile* tiles;
+  const int* active;          // per instance: forcing applied
+  const cdouble* in;          // stage, (M, 13, N, T)
+  cdouble* out;               // RHS, (M, 13, N, T)
+  cdouble* tan;
+  cdouble* dpsi;              // D Psi of the stage
+  cdouble* dpsi_t;            // D Psi of the RHS
+  cdouble* rdr;               // D of the stage reconstruction fields
+  cdouble* rdr_t;             // D of the RHS reconstruction fields
+  cdouble* rang;
+  cdouble* src;
+  cdouble* cons;
+  cdouble* terms;
+  int want_constraints;
+  int first_inst;             // instances [first_inst, first_inst + n_inst)
+  int n_inst;
+};
+
+// RK4 stage combination over whole state blocks (stage 1..4).
+struct SoRkArgs {
+  const SoInstance* inst;
+  const double* dt;           // per instance
+  const cdouble* k;
+  const cdouble* y;
+  cdouble* acc;
+  cdouble* yout;
+  long long max_values;       // largest instance block (M * 13 * N * T)
+  int stage;
+  int n_inst;
+};
+
+// Deterministic constraint reduction: per instance 18 sums and maxima plus
+// a finiteness flag (source_activation.hpp:120-170).
+struct SoReduceArgs {
+  const SoInstance* inst;
+  const double* nodes;        // Gauss-Legendre weights at node_off + 4 T
+  const SchemeTable* tables;
+  const cdouble* cons;
+  const cdouble* terms;
+  double* out;                // per instance: kSoReduceDoubles
+  int n_inst;
+};
+inline constexpr int kSoReduceDoubles = 19;
+
+hipError_t so_launch_linear(int scheme, int kind, const SoArgs& a, int n_tiles, int lds,
+                            hipStream_t s);
+hipError_t so_launch_drad(int which, const SoArgs& a, long long points, hipStream_t s);
+hipError_t so_launch_recon(int kind, const SoArgs& a, int rows, int lds, hipStream_t s);
+hipError_t so_launch_rk(const SoRkArgs& a, hipStream_t s);
+hipError_t so_launch_reduce(const SoReduceArgs& a, hipStream_t s);
+hipError_t so_set_lds_limit(int bytes);
+
+}  // namespace teuk

@@ -6,7 +6,8 @@ The five standard cases of this project:
   3. batched sweep of 512 independent Kerr evolutions (uneven nr, ellmax,
      spin, m), D10-5, CFL 0.5
   4. near-extremal a=0.999, nr=4097 D10-5, ell<=32
-  (5. second order: not part of this build)
+  5. second order: 64 instances, parents m=+-2 (ell<=8), targets {-4,0,4}
+     (ell<=12), nr=1025 D10-5, all 13 fields live, unrestricted source
 Values the case descriptions leave open come from the reference templates
 (configs/*.cfg: L, reduction damping, free-damped reduction).  Draws use a
 SplitMix64 stream with Box-Muller normals so every host produces identical
@@ -161,6 +162,32 @@ def sweep_instance(k: int, cfl=0.5) -> dict:
     seed_ell = max(2, abs(m))
     return _linear(1.0, spin, 2.0, nr, ell, "d10-5", modes, [(seed_ell, m, amp)], 20000, cfl, 0.0,
                    center, width, add_sharp=(m != 0))
+
+
+def second_order_instance(k: int, cfl=0.5, steps=200) -> dict:
+    """Instance k of the 64-instance second-order self-coupling batch:
+    a ~ U[0, 0.95], L = 1.5, parents m = +-2 (ell <= 8), targets
+    {-4, 0, 4} (ell <= 12), N_theta = 15, nr = 1025 D10-5, (2, +-2) pulses
+    with sharp partners, the unrestricted source switched on at 10 dt
+    (configs/second_order_22_self_coupling.cfg for L, damping, center and
+    width)."""
+    rng = SplitMix64(seed_for(5, k))
+    spin = 0.95 * rng.uniform()
+    nt = minimum_ntheta(8, 12)
+    nr = 1025
+    dt = cfl_time_step(1.0, spin, 1.5, nr, nt, cfl) * (1.0 - 1e-9)
+    return dict(mass=1.0, spin=spin, compactification_length=1.5, nr=nr, ntheta=nt,
+                ellmax_first=8, ellmax_second=12, first_order_modes=[-2, 2],
+                second_order_modes=[-4, 0, 4], final_time=dt * steps, steps=steps, cfl=cfl,
+                reduction_damping=0.1, dissipation=0.0, radial_discretization="d10-5",
+                reduction_mode="free_damped", initial_data__center=0.45,
+                initial_data__width=0.12, initial_data__add_sharp_partner=True,
+                second_order__enabled=True, second_order__source_mode="unrestricted",
+                second_order__source_start_time=10.0 * dt, seeds=[(2, 2, 1.0e-3)])
+
+
+def second_order_batch(n: int = 64, cfl=0.5):
+    return [second_order_instance(k, cfl) for k in range(n)]
 
 
 def sweep(n: int = 512, cfl=0.5):
