@@ -25,6 +25,9 @@ HBM_PEAK_GBS = 8000.0
 # complex128 fields, 16 state vectors (stage reads/writes of y, stage value
 # and accumulator); one step is four launches.
 BYTES_PER_MODAL_STEP = 3 * 16 * 16
+# Second-order workloads evolve all 13 nodal fields: 13 complex128 values
+# per grid point and 16 state vectors per RK4 step.
+BYTES_PER_GRID_STEP_13 = 13 * 16 * 16
 
 
 def workload(name: str):
@@ -38,6 +41,10 @@ def workload(name: str):
         return [W.case1()], "schwarzschild: nr=257 d8-4 ellmax=10 ntheta=16 cfl 0.02"
     if name == "case4":
         return [W.case4()], "near_extremal_a0.999: nr=4097 d10-5 ellmax=32 ntheta=49 cfl 0.5"
+    if name == "so64":
+        return W.second_order_batch(64), \
+            "second_order_64: a~U[0,0.95], L=1.5, parents m=+-2 ell<=8, targets {-4,0,4} " \
+            "ell<=12, ntheta=15, nr=1025 d10-5, 13 live fields, unrestricted source from 10 dt"
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -143,7 +150,17 @@ def main() -> int:
     modal = plan.modal_points
     value = grid * args.steps / seconds
     roofline = None
-    if timed_steps:
+    second_order = modal == 0
+    if timed_steps and second_order:
+        # The 13-field graph: seven RHS launches and one RK combination per
+        # stage; algorithmic bytes are the 13-field state vectors only.
+        stage_s = step_ms / (4 * timed_steps) * 1e-3
+        achieved = BYTES_PER_GRID_STEP_13 / 4.0 * grid / stage_s / 1e9
+        roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                    "kernel": "second-order stage graph (8 launches per RK stage)",
+                    "avg_stage_ms": stage_s * 1e3}
+    elif timed_steps:
         # One RK stage = one launch per execution group, the groups running
         # concurrently on streams of their own: a stage's algorithmic bytes
         # over a quarter of the GPU time of a step (HIP events on the plan's
