@@ -20,6 +20,8 @@ struct Plan::SoHost {
   std::vector<std::array<double, 14>> norms;
   std::vector<int> active_host;
 
+  long long bg_points = 0, max_nt = 0;
+  SoBackground* d_bg = nullptr;
   SoInstance* d_inst = nullptr;
   double* d_ops = nullptr;
   double* d_nodes = nullptr;

@@ -41,9 +41,7 @@ __device__ __forceinline__ cdouble cdiv(cdouble a, cdouble b) {
 }
 
 // Rescaled rotated-Kinnersley background (background.hpp:29-78).
-struct Bg {
-  cdouble mu0, tau0, pi0, psi20, rho0, eps0;
-};
+using Bg = SoBackground;
 
 __device__ Bg kerr_bg(double M, double a, double L, double R, double ct, double st) {
   const double L2 = L * L, L4 = L2 * L2;
@@ -195,16 +193,36 @@ __device__ __forceinline__ long long sidx(const SoInstance& I, int S, int mode, 
 }
 
 // out[t] = sum_t' Op[t][t'] in[t'] (in: one theta row), zero where inactive.
+// The operator row is read in chunks of 16 independent loads, so a chunk
+// costs one memory latency rather than one per coefficient.
 __device__ __forceinline__ cdouble row_op(const double* ops, int off, const cdouble* in, int t,
                                           int T) {
   if (off < 0) return cmake(0.0, 0.0);
   const double* o = ops + off + static_cast<long long>(t) * T;
   double re = 0.0, im = 0.0;
-  for (int k = 0; k < T; ++k) {
-    re += o[k] * in[k].re;
-    im += o[k] * in[k].im;
+  for (int k0 = 0; k0 < T; k0 += 16) {
+    double c[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) c[j] = k0 + j < T ? o[k0 + j] : 0.0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      if (k0 + j < T) {
+        re += c[j] * in[k0 + j].re;
+        im += c[j] * in[k0 + j].im;
+      }
+    }
   }
   return cmake(re, im);
+}
+
+// Background coefficients at (r, t) from the per-instance tables: `theta`
+// selects the theta-coordinate evaluation (cos, sin of acos x) used by the
+// reconstruction passes, otherwise the storage evaluation (x, sqrt(1-x^2)).
+__device__ __forceinline__ Bg bg_at(const SoArgs& a, const SoInstance& I, bool theta, int r,
+                                    int t) {
+  const long long i = I.bgoff * 2 + (theta ? static_cast<long long>(I.N) * I.T : 0) +
+                      static_cast<long long>(r) * I.T + t;
+  return a.bg[i];
 }
 
 }  // namespace
@@ -290,7 +308,7 @@ __global__ __launch_bounds__(kSoThreads) void so_linear_kernel(SoArgs a) {
         const cdouble ethT = a.src[sidx(I, kSlotsSrc, mode, 3, r, t)];
         const cdouble Dr = apply_d<SCHEME>(TB, dcol, T, 0, r, N, I.inv_h);
         const cdouble d3D = delta_n(Dv, Dt, Dr, 3, R, I.mass, L);
-        const Bg bg = kerr_bg(I.mass, a_spin, L, R, cos_x[t], sin_x[t]);
+        const Bg bg = bg_at(a, I, false, r, t);
         cdouble s = d3D;
         s = cadd(s, cmul(cscale(R, cadd(cscale(4.0, bg.mu0), cconj(bg.mu0))), Dv));
         s = cadd(s, cscale(R, ethT));
@@ -420,7 +438,7 @@ __global__ __launch_bounds__(kSoThreads) void so_recon_kernel(SoArgs a) {
   // pass 1: G, Lambda
   for (int e = tid; e < MT; e += kSoThreads) {
     const int mode = e / T, t = e - mode * T;
-    const Bg bg = kerr_bg(mass, aspin, L, R, cos_th[t], sin_th[t]);
+    const Bg bg = bg_at(a, I, true, r, t);
     const cdouble g = X[xG * MT + e], lam = X[xLa * MT + e], f = X[xF * MT + e];
     const cdouble dg = csub(cadd(cmul(cscale(-4.0 * R, bg.mu0), g), ANG[aEth1F * MT + e]),
                             cmul(cscale(R, bg.tau0), f));
@@ -437,7 +455,7 @@ __global__ __launch_bounds__(kSoThreads) void so_recon_kernel(SoArgs a) {
   // pass 2: H, B, Pi, C
   for (int e = tid; e < MT; e += kSoThreads) {
     const int mode = e / T, t = e - mode * T;
-    const Bg bg = kerr_bg(mass, aspin, L, R, cos_th[t], sin_th[t]);
+    const Bg bg = bg_at(a, I, true, r, t);
     const cdouble mub = cconj(bg.mu0), pib = cconj(bg.pi0);
     const cdouble g = X[xG * MT + e], lam = X[xLa * MT + e], hh = X[xH * MT + e];
     const cdouble b = X[xB * MT + e], pi = X[xPi * MT + e], c = X[xC * MT + e];
@@ -477,7 +495,7 @@ __global__ __launch_bounds__(kSoThreads) void so_recon_kernel(SoArgs a) {
   // pass 3: U
   for (int e = tid; e < MT; e += kSoThreads) {
     const int mode = e / T, t = e - mode * T, se = I.sharp[mode] * T + t;
-    const Bg bg = kerr_bg(mass, aspin, L, R, cos_th[t], sin_th[t]);
+    const Bg bg = bg_at(a, I, true, r, t);
     const cdouble mu0b = cconj(bg.mu0), pi0b = cconj(bg.pi0), tau0b = cconj(bg.tau0);
     const cdouble hh = X[xH * MT + e], pi = X[xPi * MT + e], c = X[xC * MT + e];
     const cdouble u = X[xU * MT + e];
@@ -521,7 +539,7 @@ __global__ __launch_bounds__(kSoThreads) void so_recon_kernel(SoArgs a) {
     __syncthreads();
     for (int e = tid; e < MT; e += kSoThreads) {
       const int mode = e / T, t = e - mode * T, se = I.sharp[mode] * T + t;
-      const Bg bg = kerr_bg(mass, aspin, L, R, cos_x[t], sin_x[t]);
+      const Bg bg = bg_at(a, I, false, r, t);
       const cdouble F = X[xF * MT + e], G = X[xG * MT + e], H = X[xH * MT + e];
       const cdouble La = X[xLa * MT + e], Pi = X[xPi * MT + e], B = X[xB * MT + e];
       const cdouble C = X[xC * MT + e], Uv = X[xU * MT + e];
@@ -550,8 +568,7 @@ __global__ __launch_bounds__(kSoThreads) void so_recon_kernel(SoArgs a) {
   } else {
     // ---- quadratic source: X = k, XT = tangent, ANG = tangent chain inputs
     cdouble* EX = RAW + 4 * MT;          // [4][MT] eth1 B, eth'1 F and their tangents
-    Jet* DJ = reinterpret_cast<Jet*>(EX + 4 * MT);  // [11][MT]
-    cdouble* SUM = reinterpret_cast<cdouble*>(DJ + 11 * MT);  // [targets][4][T]
+    cdouble* SUM = EX + 4 * MT;                // [targets][4][T]
     cdouble* PRJ = SUM + 4 * I.n_targets * T;                  // [targets][4][T]
     // stage B and F rows for the eth of the stage
     for (int e = tid; e < MT; e += kSoThreads) {
@@ -569,37 +586,36 @@ __global__ __launch_bounds__(kSoThreads) void so_recon_kernel(SoArgs a) {
     auto rd = [&](int mode, int c, int t) { return a.rdr[sidx(I, kSlotsRdr, mode, c, r, t)]; };
     auto rdt = [&](int mode, int c, int t) { return a.rdr_t[sidx(I, kSlotsRdr, mode, c, r, t)]; };
     auto rg = [&](int mode, int k, int t) { return a.rang[sidx(I, kSlotsRang, mode, k, r, t)]; };
-    for (int e = tid; e < MT; e += kSoThreads) {
-      const int mode = e / T, t = e - mode * T, s = I.sharp[mode];
+    // the eleven derivative jets of (mode, t) (spatial_pipeline.hpp:1019-1180)
+    auto derivs = [&](int mode, int t, Jet* d) {
+      const int e = mode * T + t, s = I.sharp[mode], se = s * T + t;
       auto dl = [&](int k, int n, int c) {
         return Jet{delta_n(S(mode, k, t), X[k * MT + e], rd(mode, c, t), n, R, mass, L),
                    delta_n(X[k * MT + e], XT[k * MT + e], rdt(mode, c, t), n, R, mass, L)};
       };
-      DJ[0 * MT + e] = {delta_n(S(mode, xF, t), X[xF * MT + e], a.dpsi[sidx(I, 1, mode, 0, r, t)], 1,
-                                R, mass, L),
-                        delta_n(X[xF * MT + e], XT[xF * MT + e], a.dpsi_t[sidx(I, 1, mode, 0, r, t)],
-                                1, R, mass, L)};
-      DJ[1 * MT + e] = dl(xU, 3, 6);
-      DJ[2 * MT + e] = {rg(mode, kRangEth2C, t), ANG[aEth2C * MT + e]};
-      DJ[3 * MT + e] = {rg(mode, kRangEthP2CSharp, t), ANG[aEthP2CS * MT + e]};
-      DJ[4 * MT + e] = {EX[e], EX[2 * MT + e]};
-      DJ[5 * MT + e] = dl(xC, 2, 5);
-      DJ[6 * MT + e] = dl(xG, 2, 0);
-      DJ[7 * MT + e] = {rg(mode, kRangEth2G, t), ANG[aEth2G * MT + e]};
-      DJ[8 * MT + e] = {EX[MT + e], EX[3 * MT + e]};
-      const int se = s * T + t;
-      DJ[9 * MT + e] = {delta_n(cconj(S(s, xC, t)), cconj(X[xC * MT + se]), cconj(rd(s, 5, t)), 2, R,
-                                mass, L),
-                        delta_n(cconj(X[xC * MT + se]), cconj(XT[xC * MT + se]), cconj(rdt(s, 5, t)),
-                                2, R, mass, L)};
-      DJ[10 * MT + e] = {rg(mode, kRangEthP1BSharp, t), ANG[aEthP1BS * MT + e]};
-    }
-    __syncthreads();
+      d[0] = {delta_n(S(mode, xF, t), X[xF * MT + e], a.dpsi[sidx(I, 1, mode, 0, r, t)], 1, R,
+                      mass, L),
+              delta_n(X[xF * MT + e], XT[xF * MT + e], a.dpsi_t[sidx(I, 1, mode, 0, r, t)], 1, R,
+                      mass, L)};
+      d[1] = dl(xU, 3, 6);
+      d[2] = {rg(mode, kRangEth2C, t), ANG[aEth2C * MT + e]};
+      d[3] = {rg(mode, kRangEthP2CSharp, t), ANG[aEthP2CS * MT + e]};
+      d[4] = {EX[e], EX[2 * MT + e]};
+      d[5] = dl(xC, 2, 5);
+      d[6] = dl(xG, 2, 0);
+      d[7] = {rg(mode, kRangEth2G, t), ANG[aEth2G * MT + e]};
+      d[8] = {EX[MT + e], EX[3 * MT + e]};
+      d[9] = {delta_n(cconj(S(s, xC, t)), cconj(X[xC * MT + se]), cconj(rd(s, 5, t)), 2, R, mass,
+                      L),
+              delta_n(cconj(X[xC * MT + se]), cconj(XT[xC * MT + se]), cconj(rdt(s, 5, t)), 2, R,
+                      mass, L)};
+      d[10] = {rg(mode, kRangEthP1BSharp, t), ANG[aEthP1BS * MT + e]};
+    };
     // ordered pairs, target-major (source_tangent_spatial.hpp:137-219)
     const int NT = I.n_targets;
     for (int e = tid; e < NT * T; e += kSoThreads) {
       const int ts = e / T, t = e - ts * T;
-      const Bg bg = kerr_bg(mass, aspin, L, R, cos_x[t], sin_x[t]);
+      const Bg bg = bg_at(a, I, false, r, t);
       Jet totD{cmake(0.0, 0.0), cmake(0.0, 0.0)}, totT = totD;
       for (int k = I.pair_begin[ts]; k < I.pair_begin[ts + 1]; ++k) {
         const int m1 = I.pair_m1[k], m2 = I.pair_m2[k];
@@ -613,7 +629,7 @@ __global__ __launch_bounds__(kSoThreads) void so_recon_kernel(SoArgs a) {
                            jconj(fj(s2, xC)), jconj(fj(s1, xC)), jconj(fj(s1, xB)),
                            jconj(fj(s2, xPi)), jconj(fj(s2, xB))};
         Jet d[11];
-        for (int q = 0; q < 11; ++q) d[q] = DJ[q * MT + m2 * T + t];
+        derivs(m2, t, d);
         Jet Dp, Tp;
         pair_source(R, bg, f, d, Dp, Tp);
         totD = jadd(totD, Dp);
@@ -648,6 +664,31 @@ __global__ __launch_bounds__(kSoThreads) void so_recon_kernel(SoArgs a) {
       a.src[sidx(I, kSlotsSrc, mode, 3, r, t)] = ethT;
     }
   }
+}
+
+// ------------------------------------------------------------ background tables
+// Kerr background (background.hpp:29-78) at every (r, theta node) of every
+// instance, once per plan: storage evaluation then theta evaluation.
+__global__ __launch_bounds__(kSoThreads) void so_background_kernel(SoArgs a) {
+  const SoInstance& I = a.inst[blockIdx.y];
+  const double* nodes = a.nodes + I.node_off;
+  const long long n = static_cast<long long>(I.N) * I.T;
+  for (long long e = static_cast<long long>(blockIdx.x) * kSoThreads + threadIdx.x; e < n;
+       e += static_cast<long long>(gridDim.x) * kSoThreads) {
+    const int r = static_cast<int>(e / I.T), t = static_cast<int>(e % I.T);
+    const double R = I.h * r;
+    a.bg[I.bgoff * 2 + e] = kerr_bg(I.mass, I.spin, I.length, R, nodes[t], nodes[I.T + t]);
+    a.bg[I.bgoff * 2 + n + e] =
+        kerr_bg(I.mass, I.spin, I.length, R, nodes[2 * I.T + t], nodes[3 * I.T + t]);
+  }
+}
+
+hipError_t so_launch_background(const SoArgs& a, long long max_points, hipStream_t s) {
+  if (a.n_inst <= 0) return hipSuccess;
+  const long long want = (max_points + kSoThreads - 1) / kSoThreads;
+  const dim3 g(static_cast<unsigned>(want < 1 ? 1 : (want > 4096 ? 4096 : want)), a.n_inst);
+  hipLaunchKernelGGL(so_background_kernel, g, dim3(kSoThreads), 0, s, a);
+  return hipGetLastError();
 }
 
 // ------------------------------------------------------------ RK4 combination

@@ -45,7 +45,8 @@ struct SoInstance {
   int M, N, T, scheme;
   int constrained, dissipate;
   int n_targets;
-  int node_off;     // x | sqrt(1 - x^2) | cos(acos x) | sin(acos x), T each
+  int node_off;     // x | sqrt(1 - x^2) | cos(acos x) | sin(acos x) | weights, T each
+  long long bgoff;  // N * T summed over the instances before this one
   double mass, spin, length, h, inv_h, gamma, eps;
   int modes[kSoMaxModes];
   int sharp[kSoMaxModes];
@@ -74,8 +75,15 @@ enum {
 };
 enum { kRangEth2C = 0, kRangEthP2CSharp = 1, kRangEth2G = 2, kRangEthP1BSharp = 3 };
 
+// Kerr background coefficients at one (r, theta) point, rescaled as in the
+// reference: mu = R mu0, tau = R^2 tau0, pi = R^2 pi0, Psi2 = R^3 psi20.
+struct SoBackground {
+  cdouble mu0, tau0, pi0, psi20, rho0, eps0;
+};
+
 struct SoArgs {
   const SoInstance* inst;
+  SoBackground* bg;           // per instance: storage table then theta table, N * T each
   const double* ops;
   const double* nodes;
   const SchemeTable* tables;  // D4-2, D8-4, D10-5
@@ -130,5 +138,6 @@ hipError_t so_launch_recon(int kind, const SoArgs& a, int rows, int lds, hipStre
 hipError_t so_launch_rk(const SoRkArgs& a, hipStream_t s);
 hipError_t so_launch_reduce(const SoReduceArgs& a, hipStream_t s);
 hipError_t so_set_lds_limit(int bytes);
+hipError_t so_launch_background(const SoArgs& a, long long max_points, hipStream_t s);
 
 }  // namespace teuk

@@ -83,7 +83,7 @@ void Plan::so_build() {
   so_ = std::make_unique<SoHost>();
   SoHost& H = *so_;
   std::map<std::tuple<int, int, int, int>, int> op_index;  // (spin, m, E, T) -> offset
-  long long base = 0, pbase = 0;
+  long long base = 0, pbase = 0, bgoff = 0;
   for (int i = 0; i < instance_count(); ++i) {
     const Instance& in = inst_[i];
     const RunParameters& p = in.params;
@@ -97,6 +97,7 @@ void Plan::so_build() {
     SoInstance I{};
     I.base = base;
     I.pbase = pbase;
+    I.bgoff = bgoff;
     I.M = M;
     I.N = N;
     I.T = T;
@@ -222,14 +223,17 @@ void Plan::so_build() {
     const long long MT = static_cast<long long>(M) * T;
     H.rec0_lds = std::max(H.rec0_lds, static_cast<int>(16 * 26 * MT));
     H.rec1_lds = std::max(H.rec1_lds,
-                          static_cast<int>(16 * (52 * MT + 8LL * I.n_targets * T)));
+                          static_cast<int>(16 * (30 * MT + 8LL * I.n_targets * T)));
     H.max_n = std::max(H.max_n, N);
     const long long block = static_cast<long long>(M) * 13 * N * T;
     H.max_block = std::max(H.max_block, block);
     H.max_plane = std::max(H.max_plane, static_cast<long long>(M) * N * T);
     base += block;
     pbase += static_cast<long long>(M) * N * T;
+    bgoff += static_cast<long long>(N) * T;
+    H.max_nt = std::max(H.max_nt, static_cast<long long>(N) * T);
   }
+  H.bg_points = bgoff;
   for (auto& v : H.inst_tiles) v.resize(instance_count(), {0, 0});
   if (H.rec1_lds > 160 * 1024 || H.rec0_lds > 160 * 1024)
     throw std::invalid_argument(
@@ -276,6 +280,15 @@ void Plan::so_build() {
     *ptr = device_alloc<cdouble>(H.planes * slots);
     check(launch_fill_zero(*ptr, H.planes * slots, stream_), "zero scratch");
   }
+  H.d_bg = device_alloc<SoBackground>(2 * H.bg_points);
+  {
+    SoArgs a{};
+    a.inst = H.d_inst;
+    a.nodes = H.d_nodes;
+    a.bg = H.d_bg;
+    a.n_inst = instance_count();
+    check(so_launch_background(a, H.max_nt, stream_), "background tables");
+  }
   const int lds = std::max({H.lin_lds, H.rec0_lds, H.rec1_lds});
   if (lds > 64 * 1024) check(so_set_lds_limit(lds), "raise LDS limit");
   for (int i = 0; i < instance_count(); ++i)
@@ -286,7 +299,7 @@ void Plan::so_build() {
 void Plan::so_free() {
   if (!so_) return;
   SoHost& H = *so_;
-  void* ptrs[] = {H.d_inst, H.d_ops,  H.d_nodes, H.d_tables, H.d_tiles[0], H.d_tiles[1],
+  void* ptrs[] = {H.d_bg, H.d_inst, H.d_ops,  H.d_nodes, H.d_tables, H.d_tiles[0], H.d_tiles[1],
                   H.d_tiles[2], H.d_active, H.d_dt, H.d_red, H.y, H.acc, H.ya, H.yb, H.k,
                   H.tan, H.dpsi, H.dpsi_t, H.rdr, H.rdr_t, H.rang, H.src, H.cons, H.terms};
   for (void* p : ptrs)
@@ -309,6 +322,7 @@ void Plan::so_rhs(int first, int n, const cdouble* in, cdouble* out, bool constr
   SoHost& H = *so_;
   SoArgs a{};
   a.inst = H.d_inst;
+  a.bg = H.d_bg;
   a.ops = H.d_ops;
   a.nodes = H.d_nodes;
   a.tables = H.d_tables;
