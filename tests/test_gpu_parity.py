@@ -40,6 +40,11 @@ CASES = {
     "kerr_ell36_wide": H.small_case(spin=0.8, nr=33, radial_discretization="d4-2",
                                     ellmax_first=36, ellmax_second=36,
                                     seeds=[(2, 2, 1e-3), (7, 2, 4e-4 + 1e-4j)]),
+    # compact-support bump (the Galerkin P of the GPU initial data at its
+    # edges, where the profile's derivatives are steep)
+    "kerr_d84_compact": H.small_case(spin=0.5, nr=57, radial_discretization="d8-4",
+                                     initial_data__compact_support=True, initial_data__width=0.3,
+                                     seeds=[(2, 2, 1e-3), (4, 2, -5e-4j)]),
     "targets_only_modes": H.small_case(
         spin=0.3, ellmax_first=4, ellmax_second=6, first_order_modes=[-2, 2],
         second_order_modes=[-4, 0, 4]),

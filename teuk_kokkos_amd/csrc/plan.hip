@@ -3,6 +3,7 @@
 #include "plan.hpp"
 #include "plan_detail.hpp"
 #include "so_host.hpp"
+#include "init_layout.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -521,39 +522,92 @@ void Plan::initialize_gaussian() {
   if (so_mode_) {
     so_initialize();
   } else {
-    std::vector<cdouble> host(static_cast<size_t>(modal_count_));
-    std::vector<std::pair<int, int>> work;  // (instance, mode)
-    for (int i = 0; i < instance_count(); ++i)
-      for (int mi = 0; mi < inst_[i].M; ++mi)
-        if (inst_[i].line_of_mode[mi] >= 0) work.push_back({i, mi});
-    std::vector<double> residual(work.size(), 0.0);
-    parallel_for(static_cast<int>(work.size()), [&](int w) {
-      const Instance& in = inst_[work[w].first];
-      const int mi = work[w].second;
-      const LineDesc& L = lines_[in.line_of_mode[mi]];
-      const int N = in.N, T = in.T, nl = L.n_ell;
-      std::vector<cplx> pn, q, psi;
-      residual[w] = first_order_initial(in, mi, pn, q, psi);
-      // Projection onto the retained band: the modal coefficients are A x.
-      const BandTransform bt = make_band_transform(-2, L.m, in.params.ell_max_first, T);
-      cdouble* out = host.data() + L.base;
-      for (int r = 0; r < N; ++r) {
-        const cplx* fields[3] = {&pn[r * T], &q[r * T], &psi[r * T]};
-        for (int f = 0; f < 3; ++f) {
-          const std::vector<cplx> c = bt.analyze(fields[f]);
-          for (int l = 0; l < nl; ++l)
-            out[static_cast<size_t>(r) * 3 * nl + f * nl + l] = {c[l].real(), c[l].imag()};
-        }
+    // One wavefront per (line, radius) on the GPU: Psi, Q = D Psi, the
+    // Galerkin solve for P and the modal coefficients of all three fields.
+    std::vector<InitLine> il(lines_.size());
+    std::vector<cdouble> seeds;
+    int lds = 0, max_nr = 0;
+    for (int i = 0; i < instance_count(); ++i) {
+      const Instance& in = inst_[i];
+      const RunParameters& p = in.params;
+      const double center = p.center_fraction * in.horizon;
+      const double width = p.width_fraction * in.horizon;
+      if (!(width > 0.0) || center < 0.0 || center > in.horizon)
+        throw std::invalid_argument("Gaussian center must be on-grid and width must be positive");
+      const auto expanded = expanded_seeds(p);
+      for (const auto& s : expanded) {
+        if (!in.registry.is_parent(s.m))
+          throw std::invalid_argument("Gaussian seed m is not a first-order mode");
+        if (s.ell < std::max(2, std::abs(s.m)) || s.ell > p.ell_max_first)
+          throw std::invalid_argument("Gaussian seed ell is outside the band");
       }
-    });
-    for (double r : residual)
+      for (int mi = 0; mi < in.M; ++mi) {
+        const int li = in.line_of_mode[mi];
+        if (li < 0) continue;
+        const LineDesc& L = lines_[li];
+        InitLine d{};
+        d.seed_off = static_cast<int>(seeds.size());
+        d.compact = p.compact_support ? 1 : 0;
+        d.scheme = static_cast<int>(p.scheme);
+        d.center = center;
+        d.width = width;
+        il[li] = d;
+        seeds.resize(seeds.size() + L.n_ell, cdouble{0.0, 0.0});
+        for (const auto& s : expanded)
+          if (s.m == L.m) {
+            cdouble& c = seeds[d.seed_off + s.ell - L.ell_min];
+            c.re += s.amplitude.real();
+            c.im += s.amplitude.imag();
+          }
+        lds = std::max(lds, galerkin_init_lds_bytes(L.n_ell, L.ntheta));
+        max_nr = std::max(max_nr, L.nr);
+      }
+    }
+    InitLine* d_il = device_alloc<InitLine>(il.size());
+    cdouble* d_seeds = device_alloc<cdouble>(seeds.size());
+    unsigned long long* d_res = device_alloc<unsigned long long>(il.size());
+    std::vector<unsigned long long> res(il.size(), 0ULL);
+    try {
+      check(hipMemcpy(d_il, il.data(), il.size() * sizeof(InitLine), hipMemcpyHostToDevice),
+            "upload initial data lines");
+      check(hipMemcpy(d_seeds, seeds.data(), seeds.size() * sizeof(cdouble),
+                      hipMemcpyHostToDevice),
+            "upload seeds");
+      check(hipMemcpy(d_res, res.data(), res.size() * sizeof(unsigned long long),
+                      hipMemcpyHostToDevice),
+            "clear residuals");
+      InitArgs a{};
+      a.lines = d_lines_;
+      a.init = d_il;
+      a.mats = d_mats_;
+      a.nodes = d_nodes_;
+      a.tables = d_tables_;
+      a.seeds = d_seeds;
+      a.y = d_y_;
+      a.residual = d_res;
+      a.n_lines = static_cast<int>(il.size());
+      check(launch_galerkin_init(a, max_nr, lds, stream_), "Galerkin initial data");
+      check(hipMemcpyAsync(res.data(), d_res, res.size() * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, stream_),
+            "read residuals");
+      check(hipStreamSynchronize(stream_), "initialize");
+    } catch (...) {
+      hipFree(d_il);
+      hipFree(d_seeds);
+      hipFree(d_res);
+      throw;
+    }
+    hipFree(d_il);
+    hipFree(d_seeds);
+    hipFree(d_res);
+    for (unsigned long long bits : res) {
+      double r;
+      std::memcpy(&r, &bits, sizeof(r));
+      if (std::isinf(r)) throw std::runtime_error("singular Galerkin initialization matrix");
       if (!std::isfinite(r) || r > 2.0e-12)
         throw std::runtime_error(
             "Galerkin initial-time-derivative solve failed its residual check");
-    check(hipMemcpyAsync(d_y_, host.data(), host.size() * sizeof(cdouble), hipMemcpyHostToDevice,
-                         stream_),
-          "upload initial state");
-    check(hipStreamSynchronize(stream_), "initialize");
+    }
   }
   for (auto& in : inst_) {
     in.passive.clear();
