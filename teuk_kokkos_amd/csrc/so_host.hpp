@@ -33,6 +33,9 @@ struct Plan::SoHost {
   cdouble *y = nullptr, *acc = nullptr, *ya = nullptr, *yb = nullptr, *k = nullptr;
   cdouble *tan = nullptr, *dpsi = nullptr, *dpsi_t = nullptr, *rdr = nullptr, *rdr_t = nullptr;
   cdouble *rang = nullptr, *src = nullptr, *cons = nullptr, *terms = nullptr;
+  cdouble *rang_t = nullptr, *ex = nullptr, *raw = nullptr;
+  int proj_lds = 0;
+  long long max_source = 0;
 };
 
 }  // namespace teuk

@@ -3,16 +3,18 @@
 // ordered-pair source and second-order Teukolsky triple
 // (spatial_pipeline.hpp:458-588).
 //
-// One RHS evaluation is seven launches:
+// One RHS evaluation is nine launches:
 //   so_linear_kernel<0>   first-order triple of the stage -> k[0..2], D Psi
 //   so_drad_kernel        D of the stage reconstruction fields
 //   so_recon_kernel<0>    reconstruction chain (stage, k) -> k[3..9], and the
 //                         independent constraints when asked
 //   so_linear_kernel<1>   first-order triple of k -> tangent, D Psi_t
 //   so_drad_kernel        D of k's reconstruction fields
-//   so_recon_kernel<1>    tangent chain (k, tangent) kept in LDS, then the
-//                         ordered-pair source with Jet1 tangents, its band
-//                         projection and eth'_3 T
+//   so_recon_kernel<1>    tangent chain (k, tangent) -> tangent[3..9], its
+//                         angular inputs, eth1 B and eth'1 F with tangents
+//   so_source_kernel      ordered-pair source with Jet1 tangents, one thread
+//                         per (target, r, theta)
+//   so_source_projection_kernel  band projection of D and T, eth'_3 T
 //   so_linear_kernel<2>   second-order triple of the stage with the forcing
 //                         (outer operator evaluated in the kernel) -> k[10..12]
 // Angular operators are T x T nodal matrices built on the host (projection,
@@ -367,16 +369,20 @@ __global__ __launch_bounds__(kSoThreads) void so_drad_kernel(SoArgs a, int which
 }
 
 // ------------------------------------------------------------ reconstruction
+// Two waves per row: a row's M * N_theta points (75 for the standard
+// second-order workload) fill one pass of 128 lanes, and the LDS of a row
+// (about 31 KiB) then admits five rows per CU instead of three.
 // One workgroup per (instance, radial row), all modes and nodes in LDS.
 // KIND 0: chain of (stage, k) -> k[3..9] and the stage angular inputs, plus
 //         the independent constraints when asked.
-// KIND 1: chain of (k, tangent) in LDS, then the quadratic source.
+// KIND 1: chain of (k, tangent) -> tangent[3..9], and the source's angular
+//         inputs of the tangent chain and of eth1 B, eth'1 F.
 // Reconstruction passes: full_spatial.hpp:486-693, reconstruction.hpp:48-105;
 // chain order: spatial_pipeline.hpp:664-754; constraints: 768-893,
 // reconstruction_constraints.hpp:24-56; source inputs: 934-1181,
 // source_tangent_spatial.hpp:65-224; projection: 1183-1209.
 template <int KIND>
-__global__ __launch_bounds__(kSoThreads) void so_recon_kernel(SoArgs a) {
+__global__ __launch_bounds__(kReconThreads) void so_recon_kernel(SoArgs a) {
   extern __shared__ __align__(16) double smem[];
   const int inst = a.first_inst + blockIdx.y;
   const SoInstance& I = a.inst[inst];
@@ -402,7 +408,7 @@ __global__ __launch_bounds__(kSoThreads) void so_recon_kernel(SoArgs a) {
   auto op = [&](int coord, int mode, int kind) { return I.op[coord][mode][kind]; };
   // eth / eth' of rows (x, dt) of coordinator `coord` for every (mode, t)
   auto ghp = [&](int coord, bool prime, const cdouble* xb, const cdouble* db, cdouble* ob) {
-    for (int e = tid; e < MT; e += kSoThreads) {
+    for (int e = tid; e < MT; e += kReconThreads) {
       const int mode = e / T, t = e - mode * T;
       const int off = op(coord, mode, prime ? kOpLower : kOpRaise);
       if (off < 0) {
@@ -415,18 +421,18 @@ __global__ __launch_bounds__(kSoThreads) void so_recon_kernel(SoArgs a) {
     }
   };
   auto project = [&](int coord, const cdouble* in, cdouble* out) {
-    for (int e = tid; e < MT; e += kSoThreads) {
+    for (int e = tid; e < MT; e += kReconThreads) {
       const int mode = e / T, t = e - mode * T;
       out[e] = row_op(a.ops, op(coord, mode, kOpProject), in + mode * T, t, T);
     }
   };
 
   // stage the row: `in` fields and the first-order time derivative
-  for (int e = tid; e < 8 * MT; e += kSoThreads) {
+  for (int e = tid; e < 8 * MT; e += kReconThreads) {
     const int k = e / MT, rem = e - k * MT, mode = rem / T, t = rem - mode * T;
     X[e] = KIND == 0 ? a.in[fidx(I, mode, slot_of(k), r, t)] : a.out[fidx(I, mode, slot_of(k), r, t)];
   }
-  for (int e = tid; e < MT; e += kSoThreads) {
+  for (int e = tid; e < MT; e += kReconThreads) {
     const int mode = e / T, t = e - mode * T;
     XT[e] = KIND == 0 ? a.out[fidx(I, mode, 2, r, t)] : a.tan[sidx(I, kSlotsTan, mode, 2, r, t)];
   }
@@ -436,7 +442,7 @@ __global__ __launch_bounds__(kSoThreads) void so_recon_kernel(SoArgs a) {
   ghp(kCoFirst, false, at(X, xF, 0), at(XT, xF, 0), at(ANG, aEth1F, 0));
   __syncthreads();
   // pass 1: G, Lambda
-  for (int e = tid; e < MT; e += kSoThreads) {
+  for (int e = tid; e < MT; e += kReconThreads) {
     const int mode = e / T, t = e - mode * T;
     const Bg bg = bg_at(a, I, true, r, t);
     const cdouble g = X[xG * MT + e], lam = X[xLa * MT + e], f = X[xF * MT + e];
@@ -453,7 +459,7 @@ __global__ __launch_bounds__(kSoThreads) void so_recon_kernel(SoArgs a) {
   ghp(kCoG, false, at(X, xG, 0), at(XT, xG, 0), at(ANG, aEth2G, 0));
   __syncthreads();
   // pass 2: H, B, Pi, C
-  for (int e = tid; e < MT; e += kSoThreads) {
+  for (int e = tid; e < MT; e += kReconThreads) {
     const int mode = e / T, t = e - mode * T;
     const Bg bg = bg_at(a, I, true, r, t);
     const cdouble mub = cconj(bg.mu0), pib = cconj(bg.pi0);
@@ -481,7 +487,7 @@ __global__ __launch_bounds__(kSoThreads) void so_recon_kernel(SoArgs a) {
   ghp(kCoC, false, at(X, xC, 0), at(XT, xC, 0), at(ANG, aEth2C, 0));
   ghp(kCoPi, false, at(X, xPi, 0), at(XT, xPi, 0), at(ANG, aEth2Pi, 0));
   // sharp fields conj(X[-m]) (spatial_pipeline.hpp:895-932)
-  for (int e = tid; e < MT; e += kSoThreads) {
+  for (int e = tid; e < MT; e += kReconThreads) {
     const int mode = e / T, t = e - mode * T, se = I.sharp[mode] * T + t;
     RAW[0 * MT + e] = cconj(X[xB * MT + se]);
     RAW[1 * MT + e] = cconj(X[xC * MT + se]);
@@ -493,7 +499,7 @@ __global__ __launch_bounds__(kSoThreads) void so_recon_kernel(SoArgs a) {
   ghp(kCoCSharp, true, RAW + MT, RAW + 3 * MT, at(ANG, aEthP2CS, 0));
   __syncthreads();
   // pass 3: U
-  for (int e = tid; e < MT; e += kSoThreads) {
+  for (int e = tid; e < MT; e += kReconThreads) {
     const int mode = e / T, t = e - mode * T, se = I.sharp[mode] * T + t;
     const Bg bg = bg_at(a, I, true, r, t);
     const cdouble mu0b = cconj(bg.mu0), pi0b = cconj(bg.pi0), tau0b = cconj(bg.tau0);
@@ -521,11 +527,11 @@ __global__ __launch_bounds__(kSoThreads) void so_recon_kernel(SoArgs a) {
   __syncthreads();
 
   if constexpr (KIND == 0) {
-    for (int e = tid; e < 7 * MT; e += kSoThreads) {
+    for (int e = tid; e < 7 * MT; e += kReconThreads) {
       const int k = 1 + e / MT, rem = e % MT, mode = rem / T, t = rem - mode * T;
       a.out[fidx(I, mode, slot_of(k), r, t)] = XT[k * MT + rem];
     }
-    for (int e = tid; e < 4 * MT; e += kSoThreads) {
+    for (int e = tid; e < 4 * MT; e += kReconThreads) {
       const int k = e / MT, rem = e - k * MT, mode = rem / T, t = rem - mode * T;
       const int src = k == kRangEth2C ? aEth2C
                       : k == kRangEthP2CSharp ? aEthP2CS
@@ -537,7 +543,7 @@ __global__ __launch_bounds__(kSoThreads) void so_recon_kernel(SoArgs a) {
     ghp(kCoG, true, at(X, xG, 0), at(XT, xG, 0), RAW);
     ghp(kCoScalar, true, at(X, xH, 0), at(XT, xH, 0), RAW + MT);
     __syncthreads();
-    for (int e = tid; e < MT; e += kSoThreads) {
+    for (int e = tid; e < MT; e += kReconThreads) {
       const int mode = e / T, t = e - mode * T, se = I.sharp[mode] * T + t;
       const Bg bg = bg_at(a, I, false, r, t);
       const cdouble F = X[xF * MT + e], G = X[xG * MT + e], H = X[xH * MT + e];
@@ -566,103 +572,156 @@ __global__ __launch_bounds__(kSoThreads) void so_recon_kernel(SoArgs a) {
     }
     return;
   } else {
-    // ---- quadratic source: X = k, XT = tangent, ANG = tangent chain inputs
-    cdouble* EX = RAW + 4 * MT;          // [4][MT] eth1 B, eth'1 F and their tangents
-    cdouble* SUM = EX + 4 * MT;                // [targets][4][T]
-    cdouble* PRJ = SUM + 4 * I.n_targets * T;                  // [targets][4][T]
-    // stage B and F rows for the eth of the stage
-    for (int e = tid; e < MT; e += kSoThreads) {
+    // tangent chain results and the source's angular inputs to HBM: the
+    // reconstruction tangents, the four angular inputs, eth1 B and eth'1 F
+    // with their tangents
+    // eth1 B, eth'1 F and their tangents reuse LDS rows the chain is done
+    // with: the eth1 F and eth2 Pi slots of ANG and the last two RAW rows
+    cdouble* EXr[4] = {at(ANG, aEth1F, 0), at(ANG, aEth2Pi, 0), RAW + 2 * MT, RAW + 3 * MT};
+    for (int e = tid; e < 7 * MT; e += kReconThreads) {
+      const int k = 1 + e / MT, rem = e % MT, mode = rem / T, t = rem - mode * T;
+      a.tan[sidx(I, kSlotsTan, mode, slot_of(k), r, t)] = XT[k * MT + rem];
+    }
+    for (int e = tid; e < 4 * MT; e += kReconThreads) {
+      const int k = e / MT, rem = e - k * MT, mode = rem / T, t = rem - mode * T;
+      const int src = k == kRangEth2C ? aEth2C
+                      : k == kRangEthP2CSharp ? aEthP2CS
+                      : k == kRangEth2G ? aEth2G
+                                        : aEthP1BS;
+      a.rang_t[sidx(I, kSlotsRang, mode, k, r, t)] = ANG[src * MT + rem];
+    }
+    for (int e = tid; e < MT; e += kReconThreads) {
       const int mode = e / T, t = e - mode * T;
       RAW[e] = a.in[fidx(I, mode, 6, r, t)];
       RAW[MT + e] = a.in[fidx(I, mode, 2, r, t)];
     }
     __syncthreads();
-    ghp(kCoB, false, RAW, at(X, xB, 0), EX);
-    ghp(kCoFirst, true, RAW + MT, at(X, xF, 0), EX + MT);
-    ghp(kCoB, false, at(X, xB, 0), at(XT, xB, 0), EX + 2 * MT);
-    ghp(kCoFirst, true, at(X, xF, 0), at(XT, xF, 0), EX + 3 * MT);
-    __syncthreads();
-    auto S = [&](int mode, int k, int t) { return a.in[fidx(I, mode, slot_of(k), r, t)]; };
-    auto rd = [&](int mode, int c, int t) { return a.rdr[sidx(I, kSlotsRdr, mode, c, r, t)]; };
-    auto rdt = [&](int mode, int c, int t) { return a.rdr_t[sidx(I, kSlotsRdr, mode, c, r, t)]; };
-    auto rg = [&](int mode, int k, int t) { return a.rang[sidx(I, kSlotsRang, mode, k, r, t)]; };
-    // the eleven derivative jets of (mode, t) (spatial_pipeline.hpp:1019-1180)
-    auto derivs = [&](int mode, int t, Jet* d) {
-      const int e = mode * T + t, s = I.sharp[mode], se = s * T + t;
+    ghp(kCoB, false, RAW, at(X, xB, 0), EXr[0]);
+    ghp(kCoFirst, true, RAW + MT, at(X, xF, 0), EXr[1]);
+    ghp(kCoB, false, at(X, xB, 0), at(XT, xB, 0), EXr[2]);
+    ghp(kCoFirst, true, at(X, xF, 0), at(XT, xF, 0), EXr[3]);
+    __syncthreads();  // the rows were written by other threads
+    for (int e = tid; e < 4 * MT; e += kReconThreads) {
+      const int k = e / MT, rem = e - k * MT, mode = rem / T, t = rem - mode * T;
+      a.ex[sidx(I, kSlotsEx, mode, k, r, t)] = EXr[k][rem];
+    }
+  }
+}
+
+// ------------------------------------------------------------ quadratic source
+// One thread per (instance, target, r, theta): the ordered-pair sum with Jet1
+// tangents (source_tangent_spatial.hpp:137-219, second_order.hpp:74-142) from
+// the stage, its RHS and tangent and the derivative inputs in HBM
+// (spatial_pipeline.hpp:934-1181).  grid.y = instance.
+__global__ __launch_bounds__(kSoThreads) void so_source_kernel(SoArgs a) {
+  const SoInstance& I = a.inst[a.first_inst + blockIdx.y];
+  const int N = I.N, T = I.T;
+  const long long total = static_cast<long long>(I.n_targets) * N * T;
+  const double mass = I.mass, L = I.length;
+  auto slot_of = [](int k) { return 2 + k; };
+  enum { xF = 0, xG, xLa, xH, xB, xPi, xC, xU };
+  for (long long e = static_cast<long long>(blockIdx.x) * kSoThreads + threadIdx.x; e < total;
+       e += static_cast<long long>(gridDim.x) * kSoThreads) {
+    const int t = static_cast<int>(e % T);
+    const int r = static_cast<int>((e / T) % N);
+    const int ts = static_cast<int>(e / (static_cast<long long>(T) * N));
+    const double R = I.h * r;
+    auto S = [&](int mode, int k) { return a.in[fidx(I, mode, slot_of(k), r, t)]; };
+    auto X = [&](int mode, int k) { return a.out[fidx(I, mode, slot_of(k), r, t)]; };
+    auto XT = [&](int mode, int k) { return a.tan[sidx(I, kSlotsTan, mode, slot_of(k), r, t)]; };
+    auto rd = [&](int mode, int c) { return a.rdr[sidx(I, kSlotsRdr, mode, c, r, t)]; };
+    auto rdt = [&](int mode, int c) { return a.rdr_t[sidx(I, kSlotsRdr, mode, c, r, t)]; };
+    auto rg = [&](int mode, int k) { return a.rang[sidx(I, kSlotsRang, mode, k, r, t)]; };
+    auto rgt = [&](int mode, int k) { return a.rang_t[sidx(I, kSlotsRang, mode, k, r, t)]; };
+    auto ex = [&](int mode, int k) { return a.ex[sidx(I, kSlotsEx, mode, k, r, t)]; };
+    // the eleven derivative jets of a mode (spatial_pipeline.hpp:1019-1180)
+    auto derivs = [&](int mode, Jet* d) {
+      const int s = I.sharp[mode];
       auto dl = [&](int k, int n, int c) {
-        return Jet{delta_n(S(mode, k, t), X[k * MT + e], rd(mode, c, t), n, R, mass, L),
-                   delta_n(X[k * MT + e], XT[k * MT + e], rdt(mode, c, t), n, R, mass, L)};
+        const cdouble x = X(mode, k);
+        return Jet{delta_n(S(mode, k), x, rd(mode, c), n, R, mass, L),
+                   delta_n(x, XT(mode, k), rdt(mode, c), n, R, mass, L)};
       };
-      d[0] = {delta_n(S(mode, xF, t), X[xF * MT + e], a.dpsi[sidx(I, 1, mode, 0, r, t)], 1, R,
-                      mass, L),
-              delta_n(X[xF * MT + e], XT[xF * MT + e], a.dpsi_t[sidx(I, 1, mode, 0, r, t)], 1, R,
-                      mass, L)};
+      const cdouble xf = X(mode, xF);
+      d[0] = {delta_n(S(mode, xF), xf, a.dpsi[sidx(I, 1, mode, 0, r, t)], 1, R, mass, L),
+              delta_n(xf, XT(mode, xF), a.dpsi_t[sidx(I, 1, mode, 0, r, t)], 1, R, mass, L)};
       d[1] = dl(xU, 3, 6);
-      d[2] = {rg(mode, kRangEth2C, t), ANG[aEth2C * MT + e]};
-      d[3] = {rg(mode, kRangEthP2CSharp, t), ANG[aEthP2CS * MT + e]};
-      d[4] = {EX[e], EX[2 * MT + e]};
+      d[2] = {rg(mode, kRangEth2C), rgt(mode, kRangEth2C)};
+      d[3] = {rg(mode, kRangEthP2CSharp), rgt(mode, kRangEthP2CSharp)};
+      d[4] = {ex(mode, 0), ex(mode, 2)};
       d[5] = dl(xC, 2, 5);
       d[6] = dl(xG, 2, 0);
-      d[7] = {rg(mode, kRangEth2G, t), ANG[aEth2G * MT + e]};
-      d[8] = {EX[MT + e], EX[3 * MT + e]};
-      d[9] = {delta_n(cconj(S(s, xC, t)), cconj(X[xC * MT + se]), cconj(rd(s, 5, t)), 2, R, mass,
-                      L),
-              delta_n(cconj(X[xC * MT + se]), cconj(XT[xC * MT + se]), cconj(rdt(s, 5, t)), 2, R,
-                      mass, L)};
-      d[10] = {rg(mode, kRangEthP1BSharp, t), ANG[aEthP1BS * MT + e]};
+      d[7] = {rg(mode, kRangEth2G), rgt(mode, kRangEth2G)};
+      d[8] = {ex(mode, 1), ex(mode, 3)};
+      const cdouble xcs = cconj(X(s, xC));
+      d[9] = {delta_n(cconj(S(s, xC)), xcs, cconj(rd(s, 5)), 2, R, mass, L),
+              delta_n(xcs, cconj(XT(s, xC)), cconj(rdt(s, 5)), 2, R, mass, L)};
+      d[10] = {rg(mode, kRangEthP1BSharp), rgt(mode, kRangEthP1BSharp)};
     };
-    // ordered pairs, target-major (source_tangent_spatial.hpp:137-219)
-    const int NT = I.n_targets;
-    for (int e = tid; e < NT * T; e += kSoThreads) {
-      const int ts = e / T, t = e - ts * T;
-      const Bg bg = bg_at(a, I, false, r, t);
-      Jet totD{cmake(0.0, 0.0), cmake(0.0, 0.0)}, totT = totD;
-      for (int k = I.pair_begin[ts]; k < I.pair_begin[ts + 1]; ++k) {
-        const int m1 = I.pair_m1[k], m2 = I.pair_m2[k];
-        const int s1 = I.sharp[m1], s2 = I.sharp[m2];
-        auto fj = [&](int mode, int k2) {
-          return Jet{S(mode, k2, t), X[k2 * MT + mode * T + t]};
-        };
-        const Jet f[19] = {fj(m1, xF),  fj(m1, xG),  fj(m1, xLa), fj(m1, xPi), fj(m1, xB),
-                           fj(m1, xC),  fj(m1, xU),  fj(m2, xF),  fj(m2, xG),  fj(m2, xH),
-                           fj(m2, xB),  fj(m2, xC),  fj(m2, xU),  jconj(fj(s2, xU)),
-                           jconj(fj(s2, xC)), jconj(fj(s1, xC)), jconj(fj(s1, xB)),
-                           jconj(fj(s2, xPi)), jconj(fj(s2, xB))};
-        Jet d[11];
-        derivs(m2, t, d);
-        Jet Dp, Tp;
-        pair_source(R, bg, f, d, Dp, Tp);
-        totD = jadd(totD, Dp);
-        totT = jadd(totT, Tp);
-      }
-      SUM[(ts * 4 + 0) * T + t] = totD.v;
-      SUM[(ts * 4 + 1) * T + t] = totD.d;
-      SUM[(ts * 4 + 2) * T + t] = totT.v;
-      SUM[(ts * 4 + 3) * T + t] = totT.d;
+    const Bg bg = bg_at(a, I, false, r, t);
+    Jet totD{cmake(0.0, 0.0), cmake(0.0, 0.0)}, totT = totD;
+    for (int k = I.pair_begin[ts]; k < I.pair_begin[ts + 1]; ++k) {
+      const int m1 = I.pair_m1[k], m2 = I.pair_m2[k];
+      const int s1 = I.sharp[m1], s2 = I.sharp[m2];
+      auto fj = [&](int mode, int k2) { return Jet{S(mode, k2), X(mode, k2)}; };
+      const Jet f[19] = {fj(m1, xF),  fj(m1, xG),  fj(m1, xLa), fj(m1, xPi), fj(m1, xB),
+                         fj(m1, xC),  fj(m1, xU),  fj(m2, xF),  fj(m2, xG),  fj(m2, xH),
+                         fj(m2, xB),  fj(m2, xC),  fj(m2, xU),  jconj(fj(s2, xU)),
+                         jconj(fj(s2, xC)), jconj(fj(s1, xC)), jconj(fj(s1, xB)),
+                         jconj(fj(s2, xPi)), jconj(fj(s2, xB))};
+      Jet d[11];
+      derivs(m2, d);
+      Jet Dp, Tp;
+      pair_source(R, bg, f, d, Dp, Tp);
+      totD = jadd(totD, Dp);
+      totT = jadd(totT, Tp);
     }
-    __syncthreads();
-    for (int e = tid; e < NT * 4 * T; e += kSoThreads) {
-      const int ts = e / (4 * T), rem = e - ts * 4 * T, c = rem / T, t = rem - c * T;
-      const int mode = I.target_mode[ts];
-      const int coord = c < 2 ? kCoSecond : kCoT;
-      PRJ[e] = row_op(a.ops, op(coord, mode, kOpProject), SUM + (ts * 4 + c) * T, t, T);
-    }
-    __syncthreads();
-    for (int e = tid; e < NT * T; e += kSoThreads) {
-      const int ts = e / T, t = e - ts * T;
-      const int mode = I.target_mode[ts];
-      const int off = op(kCoT, mode, kOpLower);
-      const cdouble* tv = PRJ + (ts * 4 + 2) * T;
-      const cdouble* tt = PRJ + (ts * 4 + 3) * T;
-      cdouble ethT = cmake(0.0, 0.0);
-      if (off >= 0)
-        ethT = eth_point(true, tv[t], tt[t], row_op(a.ops, off, tv, t, T), I.coord_spin[kCoT],
-                         I.coord_boost[kCoT], R, sin_x[t], cos_x[t], aspin, L);
-      a.src[sidx(I, kSlotsSrc, mode, 0, r, t)] = PRJ[(ts * 4 + 0) * T + t];
-      a.src[sidx(I, kSlotsSrc, mode, 1, r, t)] = PRJ[(ts * 4 + 1) * T + t];
-      a.src[sidx(I, kSlotsSrc, mode, 2, r, t)] = tv[t];
-      a.src[sidx(I, kSlotsSrc, mode, 3, r, t)] = ethT;
-    }
+    const int mode = I.target_mode[ts];
+    a.raw[sidx(I, kSlotsRaw, mode, 0, r, t)] = totD.v;
+    a.raw[sidx(I, kSlotsRaw, mode, 1, r, t)] = totD.d;
+    a.raw[sidx(I, kSlotsRaw, mode, 2, r, t)] = totT.v;
+    a.raw[sidx(I, kSlotsRaw, mode, 3, r, t)] = totT.d;
+  }
+}
+
+// ------------------------------------------------------------ source projection
+// Band projection of D (spin -2) and T (spin -1) with their tangents and
+// eth'_3 T (spatial_pipeline.hpp:1183-1209): one workgroup per (instance, r).
+__global__ __launch_bounds__(64) void so_source_projection_kernel(SoArgs a) {
+  extern __shared__ __align__(16) double smem[];
+  const SoInstance& I = a.inst[a.first_inst + blockIdx.y];
+  const int r = blockIdx.x;
+  if (r >= I.N) return;
+  const int T = I.T, NT = I.n_targets, tid = threadIdx.x;
+  const double R = I.h * r;
+  const double* nodes = node_ptr(a, I);
+  cdouble* RW = reinterpret_cast<cdouble*>(smem);  // [targets][4][T] raw sums
+  cdouble* PRJ = RW + 4 * NT * T;                   // [targets][4][T] projected
+  for (int e = tid; e < NT * 4 * T; e += 64) {
+    const int ts = e / (4 * T), rem = e - ts * 4 * T, c = rem / T, t = rem - c * T;
+    RW[e] = a.raw[sidx(I, kSlotsRaw, I.target_mode[ts], c, r, t)];
+  }
+  __syncthreads();
+  for (int e = tid; e < NT * 4 * T; e += 64) {
+    const int ts = e / (4 * T), rem = e - ts * 4 * T, c = rem / T, t = rem - c * T;
+    const int coord = c < 2 ? kCoSecond : kCoT;
+    PRJ[e] = row_op(a.ops, I.op[coord][I.target_mode[ts]][kOpProject], RW + (ts * 4 + c) * T, t, T);
+  }
+  __syncthreads();
+  for (int e = tid; e < NT * T; e += 64) {
+    const int ts = e / T, t = e - ts * T;
+    const int mode = I.target_mode[ts];
+    const int off = I.op[kCoT][mode][kOpLower];
+    const cdouble* tv = PRJ + (ts * 4 + 2) * T;
+    const cdouble* tt = PRJ + (ts * 4 + 3) * T;
+    cdouble ethT = cmake(0.0, 0.0);
+    if (off >= 0)
+      ethT = eth_point(true, tv[t], tt[t], row_op(a.ops, off, tv, t, T), I.coord_spin[kCoT],
+                       I.coord_boost[kCoT], R, nodes[T + t], nodes[t], I.spin, I.length);
+    a.src[sidx(I, kSlotsSrc, mode, 0, r, t)] = PRJ[(ts * 4 + 0) * T + t];
+    a.src[sidx(I, kSlotsSrc, mode, 1, r, t)] = PRJ[(ts * 4 + 1) * T + t];
+    a.src[sidx(I, kSlotsSrc, mode, 2, r, t)] = tv[t];
+    a.src[sidx(I, kSlotsSrc, mode, 3, r, t)] = ethT;
   }
 }
 
@@ -838,9 +897,23 @@ hipError_t so_launch_drad(int which, const SoArgs& a, long long points, hipStrea
 
 hipError_t so_launch_recon(int kind, const SoArgs& a, int rows, int lds, hipStream_t s) {
   if (a.n_inst <= 0) return hipSuccess;
-  const dim3 g(rows, a.n_inst), b(kSoThreads);
+  const dim3 g(rows, a.n_inst), b(kReconThreads);
   if (kind == 0) hipLaunchKernelGGL(so_recon_kernel<0>, g, b, lds, s, a);
   else hipLaunchKernelGGL(so_recon_kernel<1>, g, b, lds, s, a);
+  return hipGetLastError();
+}
+
+hipError_t so_launch_source(const SoArgs& a, long long max_points, hipStream_t s) {
+  if (a.n_inst <= 0) return hipSuccess;
+  const long long want = (max_points + kSoThreads - 1) / kSoThreads;
+  const dim3 g(static_cast<unsigned>(want < 1 ? 1 : (want > 8192 ? 8192 : want)), a.n_inst);
+  hipLaunchKernelGGL(so_source_kernel, g, dim3(kSoThreads), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t so_launch_source_projection(const SoArgs& a, int rows, int lds, hipStream_t s) {
+  if (a.n_inst <= 0) return hipSuccess;
+  hipLaunchKernelGGL(so_source_projection_kernel, dim3(rows, a.n_inst), dim3(64), lds, s, a);
   return hipGetLastError();
 }
 

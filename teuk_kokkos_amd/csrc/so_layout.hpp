@@ -17,6 +17,7 @@ inline constexpr int kSoMaxModes = 16;
 inline constexpr int kSoMaxTargets = 16;
 inline constexpr int kSoMaxPairs = 64;
 inline constexpr int kSoThreads = 256;
+inline constexpr int kReconThreads = 128;
 
 // Angular coordinators of the stage graph (spatial_pipeline.hpp:185-218):
 // (spin, boost, band) over the parent or target modes.
@@ -66,10 +67,12 @@ struct SoTile {
 
 // Scratch slot counts (per instance, per mode).
 enum {
-  kSlotsTan = 3,     // tangent RHS of the first-order triple
+  kSlotsTan = 10,    // tangent RHS of fields 0..9 (first-order triple, reconstruction)
   kSlotsRdr = 7,     // D of the seven reconstruction fields
   kSlotsRang = 4,    // eth2 C, eth'2 C#, eth2 G, eth'1 B# of the stage chain
   kSlotsSrc = 4,     // projected D, projected D_t, projected T, eth'3 T
+  kSlotsEx = 4,      // eth1 B, eth'1 F and their tangents
+  kSlotsRaw = 4,     // ordered-pair sums D, D_t, T, T_t of the target modes
   kSlotsCons = 3,    // constraint residuals
   kSlotsTerms = 6    // constraint left/right terms
 };
@@ -97,6 +100,9 @@ struct SoArgs {
   cdouble* rdr;               // D of the stage reconstruction fields
   cdouble* rdr_t;             // D of the RHS reconstruction fields
   cdouble* rang;
+  cdouble* rang_t;            // the same four inputs of the tangent chain
+  cdouble* ex;
+  cdouble* raw;
   cdouble* src;
   cdouble* cons;
   cdouble* terms;
@@ -135,6 +141,8 @@ hipError_t so_launch_linear(int scheme, int kind, const SoArgs& a, int n_tiles, 
                             hipStream_t s);
 hipError_t so_launch_drad(int which, const SoArgs& a, long long points, hipStream_t s);
 hipError_t so_launch_recon(int kind, const SoArgs& a, int rows, int lds, hipStream_t s);
+hipError_t so_launch_source(const SoArgs& a, long long max_points, hipStream_t s);
+hipError_t so_launch_source_projection(const SoArgs& a, int rows, int lds, hipStream_t s);
 hipError_t so_launch_rk(const SoRkArgs& a, hipStream_t s);
 hipError_t so_launch_reduce(const SoReduceArgs& a, hipStream_t s);
 hipError_t so_set_lds_limit(int bytes);

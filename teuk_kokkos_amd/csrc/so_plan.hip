@@ -223,7 +223,9 @@ void Plan::so_build() {
     const long long MT = static_cast<long long>(M) * T;
     H.rec0_lds = std::max(H.rec0_lds, static_cast<int>(16 * 26 * MT));
     H.rec1_lds = std::max(H.rec1_lds,
-                          static_cast<int>(16 * (30 * MT + 8LL * I.n_targets * T)));
+                          static_cast<int>(16 * 26 * MT));
+    H.proj_lds = std::max(H.proj_lds, static_cast<int>(16 * 8LL * I.n_targets * T));
+    H.max_source = std::max(H.max_source, static_cast<long long>(I.n_targets) * N * T);
     H.max_n = std::max(H.max_n, N);
     const long long block = static_cast<long long>(M) * 13 * N * T;
     H.max_block = std::max(H.max_block, block);
@@ -275,7 +277,8 @@ void Plan::so_build() {
   const std::pair<cdouble**, int> scratch[] = {
       {&H.tan, kSlotsTan}, {&H.dpsi, 1},          {&H.dpsi_t, 1},
       {&H.rdr, kSlotsRdr}, {&H.rdr_t, kSlotsRdr}, {&H.rang, kSlotsRang},
-      {&H.src, kSlotsSrc}, {&H.cons, kSlotsCons}, {&H.terms, kSlotsTerms}};
+      {&H.src, kSlotsSrc}, {&H.cons, kSlotsCons}, {&H.terms, kSlotsTerms},
+      {&H.rang_t, kSlotsRang}, {&H.ex, kSlotsEx}, {&H.raw, kSlotsRaw}};
   for (const auto& [ptr, slots] : scratch) {
     *ptr = device_alloc<cdouble>(H.planes * slots);
     check(launch_fill_zero(*ptr, H.planes * slots, stream_), "zero scratch");
@@ -301,7 +304,8 @@ void Plan::so_free() {
   SoHost& H = *so_;
   void* ptrs[] = {H.d_bg, H.d_inst, H.d_ops,  H.d_nodes, H.d_tables, H.d_tiles[0], H.d_tiles[1],
                   H.d_tiles[2], H.d_active, H.d_dt, H.d_red, H.y, H.acc, H.ya, H.yb, H.k,
-                  H.tan, H.dpsi, H.dpsi_t, H.rdr, H.rdr_t, H.rang, H.src, H.cons, H.terms};
+                  H.tan, H.dpsi, H.dpsi_t, H.rdr, H.rdr_t, H.rang, H.src, H.cons, H.terms,
+                  H.rang_t, H.ex, H.raw};
   for (void* p : ptrs)
     if (p) hipFree(p);
   so_.reset();
@@ -335,6 +339,9 @@ void Plan::so_rhs(int first, int n, const cdouble* in, cdouble* out, bool constr
   a.rdr = H.rdr;
   a.rdr_t = H.rdr_t;
   a.rang = H.rang;
+  a.rang_t = H.rang_t;
+  a.ex = H.ex;
+  a.raw = H.raw;
   a.src = H.src;
   a.cons = H.cons;
   a.terms = H.terms;
@@ -362,7 +369,9 @@ void Plan::so_rhs(int first, int n, const cdouble* in, cdouble* out, bool constr
   check(so_launch_recon(0, a, H.max_n, H.rec0_lds, stream_), "reconstruction chain");
   linear(1);
   check(so_launch_drad(1, a, H.max_plane * kSlotsRdr, stream_), "tangent reconstruction D");
-  check(so_launch_recon(1, a, H.max_n, H.rec1_lds, stream_), "quadratic source");
+  check(so_launch_recon(1, a, H.max_n, H.rec1_lds, stream_), "tangent reconstruction chain");
+  check(so_launch_source(a, H.max_source, stream_), "quadratic source");
+  check(so_launch_source_projection(a, H.max_n, H.proj_lds, stream_), "source projection");
   linear(2);
 }
 
