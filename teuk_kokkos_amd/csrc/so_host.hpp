@@ -15,7 +15,7 @@ struct Plan::SoHost {
   std::vector<double> nodes;  // per instance: x, sqrt(1-x^2), cos(acos x), sin(acos x), w
   std::vector<SoTile> tiles[3];                      // per radial scheme
   std::vector<std::pair<int, int>> inst_tiles[3];    // per instance: (first, count)
-  int lin_lds = 0, rec0_lds = 0, rec1_lds = 0, max_n = 0;
+  int lin_lds = 0, rec0_lds = 0, rec1_lds = 0, max_n = 0, max_mt = 0;
   long long values = 0, planes = 0, max_block = 0, max_plane = 0;
   std::vector<std::array<double, 14>> norms;
   std::vector<int> active_host;
@@ -36,6 +36,7 @@ struct Plan::SoHost {
   cdouble *rang_t = nullptr, *ex = nullptr, *raw = nullptr;
   int proj_lds = 0;
   long long max_source = 0;
+  unsigned schemes = 0;  // bit k: some instance uses radial scheme k
 };
 
 }  // namespace teuk

@@ -195,19 +195,20 @@ __device__ __forceinline__ long long sidx(const SoInstance& I, int S, int mode, 
 }
 
 // out[t] = sum_t' Op[t][t'] in[t'] (in: one theta row), zero where inactive.
-// The operator row is read in chunks of 16 independent loads, so a chunk
-// costs one memory latency rather than one per coefficient.
+// The operator column entries are read in chunks of CH independent loads, so
+// a chunk costs one memory latency rather than one per coefficient.
+template <int CH = 16>
 __device__ __forceinline__ cdouble row_op(const double* ops, int off, const cdouble* in, int t,
                                           int T) {
   if (off < 0) return cmake(0.0, 0.0);
-  const double* o = ops + off + static_cast<long long>(t) * T;
+  const double* o = ops + off + t;  // column-major: entry (t, u) at u * T + t
   double re = 0.0, im = 0.0;
-  for (int k0 = 0; k0 < T; k0 += 16) {
-    double c[16];
+  for (int k0 = 0; k0 < T; k0 += CH) {
+    double c[CH];
 #pragma unroll
-    for (int j = 0; j < 16; ++j) c[j] = k0 + j < T ? o[k0 + j] : 0.0;
+    for (int j = 0; j < CH; ++j) c[j] = k0 + j < T ? o[(k0 + j) * T] : 0.0;
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
+    for (int j = 0; j < CH; ++j) {
       if (k0 + j < T) {
         re += c[j] * in[k0 + j].re;
         im += c[j] * in[k0 + j].im;
@@ -369,10 +370,6 @@ __global__ __launch_bounds__(kSoThreads) void so_drad_kernel(SoArgs a, int which
 }
 
 // ------------------------------------------------------------ reconstruction
-// Two waves per row: a row's M * N_theta points (75 for the standard
-// second-order workload) fill one pass of 128 lanes, and the LDS of a row
-// (about 31 KiB) then admits five rows per CU instead of three.
-// One workgroup per (instance, radial row), all modes and nodes in LDS.
 // KIND 0: chain of (stage, k) -> k[3..9] and the stage angular inputs, plus
 //         the independent constraints when asked.
 // KIND 1: chain of (k, tangent) -> tangent[3..9], and the source's angular
@@ -381,230 +378,229 @@ __global__ __launch_bounds__(kSoThreads) void so_drad_kernel(SoArgs a, int which
 // chain order: spatial_pipeline.hpp:664-754; constraints: 768-893,
 // reconstruction_constraints.hpp:24-56; source inputs: 934-1181,
 // source_tangent_spatial.hpp:65-224; projection: 1183-1209.
-template <int KIND>
-__global__ __launch_bounds__(kReconThreads) void so_recon_kernel(SoArgs a) {
+//
+// One workgroup per (instance, r) and one thread per (mode, theta) element,
+// so every value a thread only needs at its own element stays in registers
+// across the phases.  LDS holds just the rows other threads read: the theta
+// rows the angular operators contract over and the sharp partners
+// conj(X[-m]).  Rows: F, G, B, Pi, C of the input, the time derivatives of
+// B and C, and four rows of reconstructed time derivatives awaiting their
+// band projection: 11 * M * N_theta * 16 bytes (13.2 KiB for 5 modes x 15
+// nodes, where register use rather than LDS then bounds the rows per CU).
+template <int KIND, int NT>
+__global__ __launch_bounds__(NT) void so_recon_kernel(SoArgs a) {
+  constexpr int CH = 8;  // operator loads in flight per row_op chunk
   extern __shared__ __align__(16) double smem[];
   const int inst = a.first_inst + blockIdx.y;
   const SoInstance& I = a.inst[inst];
   const int r = blockIdx.x;
   if (r >= I.N) return;
   const int M = I.M, T = I.T, MT = M * T, tid = threadIdx.x;
+  const bool live = tid < MT;
+  const int e = live ? tid : 0;
+  const int mode = e / T, t = e - mode * T;
+  const int se = I.sharp[mode] * T + t;
   const double R = I.h * r, R2 = R * R, R3 = R2 * R;
   const double mass = I.mass, aspin = I.spin, L = I.length;
   const double* nodes = node_ptr(a, I);
-  const double* cos_x = nodes;
-  const double* sin_x = nodes + T;
-  const double* cos_th = nodes + 2 * T;
-  const double* sin_th = nodes + 3 * T;
-  cdouble* X = reinterpret_cast<cdouble*>(smem);  // [8][MT] F G La H B Pi C U of `in`
-  cdouble* XT = X + 8 * MT;                       // [8][MT] their time derivatives
-  cdouble* ANG = XT + 8 * MT;                     // [6][MT]
-  cdouble* RAW = ANG + 6 * MT;                    // [4][MT]
-  enum { xF = 0, xG, xLa, xH, xB, xPi, xC, xU };
-  enum { aEth1F = 0, aEth2G, aEth2C, aEth2Pi, aEthP1BS, aEthP2CS };
-  // state field of LDS slot k: F is field 2, G..U are fields 3..9
-  auto slot_of = [](int k) { return 2 + k; };
-  auto at = [&](cdouble* base, int k, int mode) { return base + k * MT + mode * T; };
-  auto op = [&](int coord, int mode, int kind) { return I.op[coord][mode][kind]; };
-  // eth / eth' of rows (x, dt) of coordinator `coord` for every (mode, t)
-  auto ghp = [&](int coord, bool prime, const cdouble* xb, const cdouble* db, cdouble* ob) {
-    for (int e = tid; e < MT; e += kReconThreads) {
-      const int mode = e / T, t = e - mode * T;
-      const int off = op(coord, mode, prime ? kOpLower : kOpRaise);
-      if (off < 0) {
-        ob[e] = cmake(0.0, 0.0);
-        continue;
-      }
-      const cdouble ang = row_op(a.ops, off, xb + mode * T, t, T);
-      ob[e] = eth_point(prime, xb[e], db[e], ang, I.coord_spin[coord], I.coord_boost[coord], R,
-                        sin_x[t], cos_x[t], aspin, L);
-    }
+  const double sx = nodes[T + t], cx = nodes[t];
+  cdouble* LX = reinterpret_cast<cdouble*>(smem);
+  enum { lF = 0, lG, lB, lPi, lC, ltB, ltC, lRaw };
+  auto row = [&](int k) { return LX + k * MT; };
+  auto op = [&](int coord, int kind) { return I.op[coord][mode][kind]; };
+  // eth / eth' of coordinator `coord` at this element: theta row `xr` of the
+  // field (every mode), its own value `xo` and time derivative `dto`
+  auto ghp = [&](int coord, bool prime, const cdouble* xr, cdouble xo, cdouble dto) {
+    const int off = op(coord, prime ? kOpLower : kOpRaise);
+    if (off < 0) return cmake(0.0, 0.0);
+    const cdouble ang = row_op<CH>(a.ops, off, xr + mode * T, t, T);
+    return eth_point(prime, xo, dto, ang, I.coord_spin[coord], I.coord_boost[coord], R, sx, cx,
+                     aspin, L);
   };
-  auto project = [&](int coord, const cdouble* in, cdouble* out) {
-    for (int e = tid; e < MT; e += kReconThreads) {
-      const int mode = e / T, t = e - mode * T;
-      out[e] = row_op(a.ops, op(coord, mode, kOpProject), in + mode * T, t, T);
-    }
+  auto project = [&](int coord, const cdouble* in) {
+    return row_op<CH>(a.ops, op(coord, kOpProject), in + mode * T, t, T);
   };
-
-  // stage the row: `in` fields and the first-order time derivative
-  for (int e = tid; e < 8 * MT; e += kReconThreads) {
-    const int k = e / MT, rem = e - k * MT, mode = rem / T, t = rem - mode * T;
-    X[e] = KIND == 0 ? a.in[fidx(I, mode, slot_of(k), r, t)] : a.out[fidx(I, mode, slot_of(k), r, t)];
-  }
-  for (int e = tid; e < MT; e += kReconThreads) {
-    const int mode = e / T, t = e - mode * T;
-    XT[e] = KIND == 0 ? a.out[fidx(I, mode, 2, r, t)] : a.tan[sidx(I, kSlotsTan, mode, 2, r, t)];
-  }
-  __syncthreads();
+  const cdouble* src = KIND == 0 ? a.in : a.out;
+  cdouble* dst = KIND == 0 ? a.out : a.tan;
   const cdouble* rdr = KIND == 0 ? a.rdr : a.rdr_t;
-  auto dr = [&](int mode, int c, int t) { return rdr[sidx(I, kSlotsRdr, mode, c, r, t)]; };
-  ghp(kCoFirst, false, at(X, xF, 0), at(XT, xF, 0), at(ANG, aEth1F, 0));
+  // state field of chain slot k: F is field 2, G..U are fields 3..9
+  auto load = [&](int k) { return src[fidx(I, mode, 2 + k, r, t)]; };
+  auto store = [&](int k, cdouble v) {
+    if (KIND == 0) dst[fidx(I, mode, 2 + k, r, t)] = v;
+    else dst[sidx(I, kSlotsTan, mode, 2 + k, r, t)] = v;
+  };
+  auto dr = [&](int c) { return rdr[sidx(I, kSlotsRdr, mode, c, r, t)]; };
+
+  cdouble xF{}, xG{}, xLa{}, xH{}, xB{}, xPi{}, xC{}, xU{}, tF{}, tG{}, tH{}, tPi{}, tB{}, tC{};
+  cdouble e1F{}, e2G{}, e2C{}, e2Pi{};
+  Bg bg{};
+  // stage the row
+  if (live) {
+    xF = load(0);
+    xG = load(1);
+    xLa = load(2);
+    xH = load(3);
+    xB = load(4);
+    xPi = load(5);
+    xC = load(6);
+    xU = load(7);
+    tF = KIND == 0 ? a.out[fidx(I, mode, 2, r, t)] : a.tan[sidx(I, kSlotsTan, mode, 2, r, t)];
+    bg = bg_at(a, I, true, r, t);
+    row(lF)[e] = xF;
+    row(lG)[e] = xG;
+    row(lB)[e] = xB;
+    row(lPi)[e] = xPi;
+    row(lC)[e] = xC;
+  }
   __syncthreads();
   // pass 1: G, Lambda
-  for (int e = tid; e < MT; e += kReconThreads) {
-    const int mode = e / T, t = e - mode * T;
-    const Bg bg = bg_at(a, I, true, r, t);
-    const cdouble g = X[xG * MT + e], lam = X[xLa * MT + e], f = X[xF * MT + e];
-    const cdouble dg = csub(cadd(cmul(cscale(-4.0 * R, bg.mu0), g), ANG[aEth1F * MT + e]),
-                            cmul(cscale(R, bg.tau0), f));
-    const cdouble dl = csub(cmul(cscale(-R, cadd(bg.mu0, cconj(bg.mu0))), lam), f);
-    RAW[0 * MT + e] = recon_dt(g, dr(mode, 0, t), dg, 2, R, mass, L);
-    RAW[1 * MT + e] = recon_dt(lam, dr(mode, 1, t), dl, 1, R, mass, L);
+  if (live) {
+    e1F = ghp(kCoFirst, false, row(lF), xF, tF);
+    const cdouble dg = csub(cadd(cmul(cscale(-4.0 * R, bg.mu0), xG), e1F),
+                            cmul(cscale(R, bg.tau0), xF));
+    const cdouble dl = csub(cmul(cscale(-R, cadd(bg.mu0, cconj(bg.mu0))), xLa), xF);
+    row(lRaw)[e] = recon_dt(xG, dr(0), dg, 2, R, mass, L);
+    row(lRaw + 1)[e] = recon_dt(xLa, dr(1), dl, 1, R, mass, L);
   }
   __syncthreads();
-  project(kCoG, RAW, at(XT, xG, 0));
-  project(kCoFirst, RAW + MT, at(XT, xLa, 0));
-  __syncthreads();
-  ghp(kCoG, false, at(X, xG, 0), at(XT, xG, 0), at(ANG, aEth2G, 0));
+  if (live) {
+    tG = project(kCoG, row(lRaw));
+    store(1, tG);
+    store(2, project(kCoFirst, row(lRaw + 1)));
+    e2G = ghp(kCoG, false, row(lG), xG, tG);
+  }
   __syncthreads();
   // pass 2: H, B, Pi, C
-  for (int e = tid; e < MT; e += kReconThreads) {
-    const int mode = e / T, t = e - mode * T;
-    const Bg bg = bg_at(a, I, true, r, t);
+  if (live) {
     const cdouble mub = cconj(bg.mu0), pib = cconj(bg.pi0);
-    const cdouble g = X[xG * MT + e], lam = X[xLa * MT + e], hh = X[xH * MT + e];
-    const cdouble b = X[xB * MT + e], pi = X[xPi * MT + e], c = X[xC * MT + e];
-    const cdouble dh = csub(cadd(cmul(cscale(-3.0 * R, bg.mu0), hh), ANG[aEth2G * MT + e]),
-                            cmul(cscale(2.0 * R, bg.tau0), g));
-    const cdouble db = csub(cmul(cscale(R, csub(bg.mu0, mub)), b), cscale(2.0, lam));
+    const cdouble dh = csub(cadd(cmul(cscale(-3.0 * R, bg.mu0), xH), e2G),
+                            cmul(cscale(2.0 * R, bg.tau0), xG));
+    const cdouble db = csub(cmul(cscale(R, csub(bg.mu0, mub)), xB), cscale(2.0, xLa));
     const cdouble pt = cadd(pib, bg.tau0);
-    const cdouble dpi = cadd(csub(cneg(g), cmul(cscale(R, pt), lam)),
-                             cmul(cmul(cscale(0.5 * R2, bg.mu0), pt), b));
-    const cdouble dc = csub(csub(cmul(cscale(-R, mub), c), cscale(2.0, pi)),
-                            cmul(cscale(R, bg.tau0), b));
-    RAW[0 * MT + e] = recon_dt(hh, dr(mode, 2, t), dh, 3, R, mass, L);
-    RAW[1 * MT + e] = recon_dt(b, dr(mode, 3, t), db, 1, R, mass, L);
-    RAW[2 * MT + e] = recon_dt(pi, dr(mode, 4, t), dpi, 2, R, mass, L);
-    RAW[3 * MT + e] = recon_dt(c, dr(mode, 5, t), dc, 2, R, mass, L);
+    const cdouble dpi = cadd(csub(cneg(xG), cmul(cscale(R, pt), xLa)),
+                             cmul(cmul(cscale(0.5 * R2, bg.mu0), pt), xB));
+    const cdouble dc = csub(csub(cmul(cscale(-R, mub), xC), cscale(2.0, xPi)),
+                            cmul(cscale(R, bg.tau0), xB));
+    row(lRaw)[e] = recon_dt(xH, dr(2), dh, 3, R, mass, L);
+    row(lRaw + 1)[e] = recon_dt(xB, dr(3), db, 1, R, mass, L);
+    row(lRaw + 2)[e] = recon_dt(xPi, dr(4), dpi, 2, R, mass, L);
+    row(lRaw + 3)[e] = recon_dt(xC, dr(5), dc, 2, R, mass, L);
   }
   __syncthreads();
-  project(kCoScalar, RAW, at(XT, xH, 0));
-  project(kCoB, RAW + MT, at(XT, xB, 0));
-  project(kCoPi, RAW + 2 * MT, at(XT, xPi, 0));
-  project(kCoC, RAW + 3 * MT, at(XT, xC, 0));
+  if (live) {
+    tH = project(kCoScalar, row(lRaw));
+    tB = project(kCoB, row(lRaw + 1));
+    tPi = project(kCoPi, row(lRaw + 2));
+    tC = project(kCoC, row(lRaw + 3));
+    store(3, tH);
+    store(4, tB);
+    store(5, tPi);
+    store(6, tC);
+    row(ltB)[e] = tB;
+    row(ltC)[e] = tC;
+    e2C = ghp(kCoC, false, row(lC), xC, tC);
+    e2Pi = ghp(kCoPi, false, row(lPi), xPi, tPi);
+  }
   __syncthreads();
-  ghp(kCoC, false, at(X, xC, 0), at(XT, xC, 0), at(ANG, aEth2C, 0));
-  ghp(kCoPi, false, at(X, xPi, 0), at(XT, xPi, 0), at(ANG, aEth2Pi, 0));
   // sharp fields conj(X[-m]) (spatial_pipeline.hpp:895-932)
-  for (int e = tid; e < MT; e += kReconThreads) {
-    const int mode = e / T, t = e - mode * T, se = I.sharp[mode] * T + t;
-    RAW[0 * MT + e] = cconj(X[xB * MT + se]);
-    RAW[1 * MT + e] = cconj(X[xC * MT + se]);
-    RAW[2 * MT + e] = cconj(XT[xB * MT + se]);
-    RAW[3 * MT + e] = cconj(XT[xC * MT + se]);
+  if (live) {
+    row(lRaw)[e] = cconj(row(lB)[se]);
+    row(lRaw + 1)[e] = cconj(row(lC)[se]);
+    row(lRaw + 2)[e] = cconj(row(ltB)[se]);
+    row(lRaw + 3)[e] = cconj(row(ltC)[se]);
   }
   __syncthreads();
-  ghp(kCoBSharp, true, RAW, RAW + 2 * MT, at(ANG, aEthP1BS, 0));
-  ghp(kCoCSharp, true, RAW + MT, RAW + 3 * MT, at(ANG, aEthP2CS, 0));
-  __syncthreads();
-  // pass 3: U
-  for (int e = tid; e < MT; e += kReconThreads) {
-    const int mode = e / T, t = e - mode * T, se = I.sharp[mode] * T + t;
-    const Bg bg = bg_at(a, I, true, r, t);
+  // pass 3: U.  The time-derivative rows of B and C are free from here on:
+  // the raw U derivative goes to the first, the H row (for the constraint
+  // eth') to the second.
+  if (live) {
+    const cdouble ep1BS = ghp(kCoBSharp, true, row(lRaw), row(lRaw)[e], row(lRaw + 2)[e]);
+    const cdouble ep2CS = ghp(kCoCSharp, true, row(lRaw + 1), row(lRaw + 1)[e], row(lRaw + 3)[e]);
     const cdouble mu0b = cconj(bg.mu0), pi0b = cconj(bg.pi0), tau0b = cconj(bg.tau0);
-    const cdouble hh = X[xH * MT + e], pi = X[xPi * MT + e], c = X[xC * MT + e];
-    const cdouble u = X[xU * MT + e];
-    const cdouble pi_s = cconj(X[xPi * MT + se]), b_s = cconj(X[xB * MT + se]),
-                  c_s = cconj(X[xC * MT + se]);
-    cdouble du = cmul(cscale(-R, mu0b), u);
-    du = cadd(du, cmul(cscale(-R, bg.mu0), ANG[aEth2C * MT + e]));
-    du = cadd(du, cmul(cmul(cscale(-R2, bg.mu0), cadd(pi0b, cscale(2.0, bg.tau0))), c));
-    du = cadd(du, cscale(-2.0, ANG[aEth2Pi * MT + e]));
-    du = cadd(du, cmul(cscale(-2.0 * R, pi0b), pi));
-    du = cadd(du, cscale(-2.0, hh));
+    const cdouble pi_s = cconj(row(lPi)[se]), b_s = cconj(row(lB)[se]), c_s = cconj(row(lC)[se]);
+    cdouble du = cmul(cscale(-R, mu0b), xU);
+    du = cadd(du, cmul(cscale(-R, bg.mu0), e2C));
+    du = cadd(du, cmul(cmul(cscale(-R2, bg.mu0), cadd(pi0b, cscale(2.0, bg.tau0))), xC));
+    du = cadd(du, cscale(-2.0, e2Pi));
+    du = cadd(du, cmul(cscale(-2.0 * R, pi0b), xPi));
+    du = cadd(du, cscale(-2.0, xH));
     du = cadd(du, cmul(cscale(-2.0 * R, bg.pi0), pi_s));
-    du = cadd(du, cmul(cscale(-R, bg.pi0), ANG[aEthP1BS * MT + e]));
+    du = cadd(du, cmul(cscale(-R, bg.pi0), ep1BS));
     du = cadd(du, cmul(cmul(cscale(R2, bg.pi0), bg.pi0), b_s));
-    du = cadd(du, cmul(cscale(R, bg.mu0), ANG[aEthP2CS * MT + e]));
+    du = cadd(du, cmul(cscale(R, bg.mu0), ep2CS));
     const cdouble sc = cadd(csub(cscale(-3.0, cmul(bg.mu0, bg.pi0)), cscale(-2.0, cmul(mu0b, bg.pi0))),
                             cscale(-2.0, cmul(bg.mu0, tau0b)));
     du = cadd(du, cmul(cscale(R2, sc), c_s));
-    RAW[e] = recon_dt(u, dr(mode, 6, t), du, 3, R, mass, L);
+    row(ltB)[e] = recon_dt(xU, dr(6), du, 3, R, mass, L);
+    row(ltC)[e] = xH;
+    cdouble* ro = KIND == 0 ? a.rang : a.rang_t;
+    ro[sidx(I, kSlotsRang, mode, kRangEth2C, r, t)] = e2C;
+    ro[sidx(I, kSlotsRang, mode, kRangEthP2CSharp, r, t)] = ep2CS;
+    ro[sidx(I, kSlotsRang, mode, kRangEth2G, r, t)] = e2G;
+    ro[sidx(I, kSlotsRang, mode, kRangEthP1BSharp, r, t)] = ep1BS;
   }
   __syncthreads();
-  project(kCoScalar, RAW, at(XT, xU, 0));
-  __syncthreads();
-
   if constexpr (KIND == 0) {
-    for (int e = tid; e < 7 * MT; e += kReconThreads) {
-      const int k = 1 + e / MT, rem = e % MT, mode = rem / T, t = rem - mode * T;
-      a.out[fidx(I, mode, slot_of(k), r, t)] = XT[k * MT + rem];
-    }
-    for (int e = tid; e < 4 * MT; e += kReconThreads) {
-      const int k = e / MT, rem = e - k * MT, mode = rem / T, t = rem - mode * T;
-      const int src = k == kRangEth2C ? aEth2C
-                      : k == kRangEthP2CSharp ? aEthP2CS
-                      : k == kRangEth2G ? aEth2G
-                                        : aEthP1BS;
-      a.rang[sidx(I, kSlotsRang, mode, k, r, t)] = ANG[src * MT + rem];
-    }
-    if (!a.want_constraints) return;
-    ghp(kCoG, true, at(X, xG, 0), at(XT, xG, 0), RAW);
-    ghp(kCoScalar, true, at(X, xH, 0), at(XT, xH, 0), RAW + MT);
-    __syncthreads();
-    for (int e = tid; e < MT; e += kReconThreads) {
-      const int mode = e / T, t = e - mode * T, se = I.sharp[mode] * T + t;
-      const Bg bg = bg_at(a, I, false, r, t);
-      const cdouble F = X[xF * MT + e], G = X[xG * MT + e], H = X[xH * MT + e];
-      const cdouble La = X[xLa * MT + e], Pi = X[xPi * MT + e], B = X[xB * MT + e];
-      const cdouble C = X[xC * MT + e], Uv = X[xU * MT + e];
-      const cdouble Us = cconj(X[xU * MT + se]);
-      const int m = I.modes[mode];
-      const cdouble th1 = thorn_n(F, XT[xF * MT + e], a.dpsi[sidx(I, 1, mode, 0, r, t)], 1, -2, -2, m,
-                                  R, cos_x[t], mass, aspin, L, bg.eps0);
-      const cdouble th2 = thorn_n(G, XT[xG * MT + e], a.rdr[sidx(I, kSlotsRdr, mode, 0, r, t)], 2, -1,
-                                  -1, m, R, cos_x[t], mass, aspin, L, bg.eps0);
-      const cdouble l3 = cadd(cadd(cscale(R, RAW[e]), cmul(cscale(4.0 * R2, bg.pi0), G)),
-                              cmul(bg.rho0, F));
-      const cdouble r3 = cadd(th1, cmul(cscale(3.0 * R2, bg.psi20), La));
-      const cdouble bgt = cmul(bg.psi20, csub(csub(cmul(cscale(-3.0 * R3, bg.mu0), C),
-                                                   cmul(cscale(1.5 * R3, bg.tau0), B)),
-                                              cscale(3.0 * R2, Pi)));
-      const cdouble l2 = csub(th2, cmul(cscale(2.0, bg.rho0), G));
-      const cdouble r2 = csub(cadd(cscale(R, RAW[MT + e]), cmul(cscale(3.0 * R2, bg.pi0), H)), bgt);
-      const cdouble lh = cdiv(Uv, bg.mu0);
-      const cdouble rh = cdiv(Us, cconj(bg.mu0));
-      const cdouble tv[6] = {l3, r3, l2, r2, lh, rh};
-      for (int k = 0; k < 3; ++k)
-        a.cons[sidx(I, kSlotsCons, mode, k, r, t)] = csub(tv[2 * k], tv[2 * k + 1]);
-      for (int k = 0; k < 6; ++k) a.terms[sidx(I, kSlotsTerms, mode, k, r, t)] = tv[k];
-    }
-    return;
+    if (live) store(7, project(kCoScalar, row(ltB)));
+    if (!a.want_constraints || !live) return;
+    // reloaded rather than held in registers through the chain
+    xF = load(0);
+    xG = load(1);
+    xLa = load(2);
+    xH = load(3);
+    xB = load(4);
+    xPi = load(5);
+    xC = load(6);
+    xU = load(7);
+    tF = a.out[fidx(I, mode, 2, r, t)];
+    tG = a.out[fidx(I, mode, 3, r, t)];
+    tH = a.out[fidx(I, mode, 5, r, t)];
+    const cdouble ethpG = ghp(kCoG, true, row(lG), xG, tG);
+    const cdouble ethpH = ghp(kCoScalar, true, row(ltC), xH, tH);
+    const Bg bs = bg_at(a, I, false, r, t);
+    const cdouble Us = cconj(a.in[fidx(I, I.sharp[mode], 9, r, t)]);
+    const int m = I.modes[mode];
+    const cdouble th1 = thorn_n(xF, tF, a.dpsi[sidx(I, 1, mode, 0, r, t)], 1, -2, -2, m, R, cx,
+                                mass, aspin, L, bs.eps0);
+    const cdouble th2 = thorn_n(xG, tG, a.rdr[sidx(I, kSlotsRdr, mode, 0, r, t)], 2, -1, -1, m, R,
+                                cx, mass, aspin, L, bs.eps0);
+    const cdouble l3 = cadd(cadd(cscale(R, ethpG), cmul(cscale(4.0 * R2, bs.pi0), xG)),
+                            cmul(bs.rho0, xF));
+    const cdouble r3 = cadd(th1, cmul(cscale(3.0 * R2, bs.psi20), xLa));
+    const cdouble bgt = cmul(bs.psi20, csub(csub(cmul(cscale(-3.0 * R3, bs.mu0), xC),
+                                                 cmul(cscale(1.5 * R3, bs.tau0), xB)),
+                                            cscale(3.0 * R2, xPi)));
+    const cdouble l2 = csub(th2, cmul(cscale(2.0, bs.rho0), xG));
+    const cdouble r2 = csub(cadd(cscale(R, ethpH), cmul(cscale(3.0 * R2, bs.pi0), xH)), bgt);
+    const cdouble lh = cdiv(xU, bs.mu0);
+    const cdouble rh = cdiv(Us, cconj(bs.mu0));
+    const cdouble tv[6] = {l3, r3, l2, r2, lh, rh};
+    for (int k = 0; k < 3; ++k)
+      a.cons[sidx(I, kSlotsCons, mode, k, r, t)] = csub(tv[2 * k], tv[2 * k + 1]);
+    for (int k = 0; k < 6; ++k) a.terms[sidx(I, kSlotsTerms, mode, k, r, t)] = tv[k];
   } else {
-    // tangent chain results and the source's angular inputs to HBM: the
-    // reconstruction tangents, the four angular inputs, eth1 B and eth'1 F
-    // with their tangents
-    // eth1 B, eth'1 F and their tangents reuse LDS rows the chain is done
-    // with: the eth1 F and eth2 Pi slots of ANG and the last two RAW rows
-    cdouble* EXr[4] = {at(ANG, aEth1F, 0), at(ANG, aEth2Pi, 0), RAW + 2 * MT, RAW + 3 * MT};
-    for (int e = tid; e < 7 * MT; e += kReconThreads) {
-      const int k = 1 + e / MT, rem = e % MT, mode = rem / T, t = rem - mode * T;
-      a.tan[sidx(I, kSlotsTan, mode, slot_of(k), r, t)] = XT[k * MT + rem];
-    }
-    for (int e = tid; e < 4 * MT; e += kReconThreads) {
-      const int k = e / MT, rem = e - k * MT, mode = rem / T, t = rem - mode * T;
-      const int src = k == kRangEth2C ? aEth2C
-                      : k == kRangEthP2CSharp ? aEthP2CS
-                      : k == kRangEth2G ? aEth2G
-                                        : aEthP1BS;
-      a.rang_t[sidx(I, kSlotsRang, mode, k, r, t)] = ANG[src * MT + rem];
-    }
-    for (int e = tid; e < MT; e += kReconThreads) {
-      const int mode = e / T, t = e - mode * T;
-      RAW[e] = a.in[fidx(I, mode, 6, r, t)];
-      RAW[MT + e] = a.in[fidx(I, mode, 2, r, t)];
+    // the tangent of U, then the source's extra angular inputs: eth1 B and
+    // eth'1 F of the stage with their tangents (the stage's B and F rows
+    // go to the two first raw rows, free since pass 3)
+    cdouble sB{}, sF{};
+    if (live) {
+      store(7, project(kCoScalar, row(ltB)));
+      sB = a.in[fidx(I, mode, 6, r, t)];
+      sF = a.in[fidx(I, mode, 2, r, t)];
+      row(lRaw)[e] = sB;
+      row(lRaw + 1)[e] = sF;
     }
     __syncthreads();
-    ghp(kCoB, false, RAW, at(X, xB, 0), EXr[0]);
-    ghp(kCoFirst, true, RAW + MT, at(X, xF, 0), EXr[1]);
-    ghp(kCoB, false, at(X, xB, 0), at(XT, xB, 0), EXr[2]);
-    ghp(kCoFirst, true, at(X, xF, 0), at(XT, xF, 0), EXr[3]);
-    __syncthreads();  // the rows were written by other threads
-    for (int e = tid; e < 4 * MT; e += kReconThreads) {
-      const int k = e / MT, rem = e - k * MT, mode = rem / T, t = rem - mode * T;
-      a.ex[sidx(I, kSlotsEx, mode, k, r, t)] = EXr[k][rem];
-    }
+    if (!live) return;
+    // reloaded rather than held in registers through the chain
+    xB = load(4);
+    xF = load(0);
+    tB = a.tan[sidx(I, kSlotsTan, mode, 6, r, t)];
+    tF = a.tan[sidx(I, kSlotsTan, mode, 2, r, t)];
+    a.ex[sidx(I, kSlotsEx, mode, 0, r, t)] = ghp(kCoB, false, row(lRaw), sB, xB);
+    a.ex[sidx(I, kSlotsEx, mode, 1, r, t)] = ghp(kCoFirst, true, row(lRaw + 1), sF, xF);
+    a.ex[sidx(I, kSlotsEx, mode, 2, r, t)] = ghp(kCoB, false, row(lB), xB, tB);
+    a.ex[sidx(I, kSlotsEx, mode, 3, r, t)] = ghp(kCoFirst, true, row(lF), xF, tF);
   }
 }
 
@@ -885,21 +881,31 @@ hipError_t so_launch_linear(int scheme, int kind, const SoArgs& a, int n_tiles, 
   return launch_linear_k<2>(scheme, a, n_tiles, lds, s);
 }
 
-hipError_t so_launch_drad(int which, const SoArgs& a, long long points, hipStream_t s) {
+hipError_t so_launch_drad(int which, const SoArgs& a, long long points, unsigned schemes,
+                          hipStream_t s) {
   if (a.n_inst <= 0) return hipSuccess;
   const long long want = (points + kSoThreads - 1) / kSoThreads;
   const dim3 g(static_cast<unsigned>(want < 1 ? 1 : (want > 4096 ? 4096 : want)), a.n_inst);
-  hipLaunchKernelGGL(so_drad_kernel<0>, g, dim3(kSoThreads), 0, s, a, which);
-  hipLaunchKernelGGL(so_drad_kernel<1>, g, dim3(kSoThreads), 0, s, a, which);
-  hipLaunchKernelGGL(so_drad_kernel<2>, g, dim3(kSoThreads), 0, s, a, which);
+  if (schemes & 1u) hipLaunchKernelGGL(so_drad_kernel<0>, g, dim3(kSoThreads), 0, s, a, which);
+  if (schemes & 2u) hipLaunchKernelGGL(so_drad_kernel<1>, g, dim3(kSoThreads), 0, s, a, which);
+  if (schemes & 4u) hipLaunchKernelGGL(so_drad_kernel<2>, g, dim3(kSoThreads), 0, s, a, which);
   return hipGetLastError();
 }
 
-hipError_t so_launch_recon(int kind, const SoArgs& a, int rows, int lds, hipStream_t s) {
+hipError_t so_launch_recon(int kind, const SoArgs& a, int rows, int max_mt, int lds,
+                           hipStream_t s) {
   if (a.n_inst <= 0) return hipSuccess;
-  const dim3 g(rows, a.n_inst), b(kReconThreads);
-  if (kind == 0) hipLaunchKernelGGL(so_recon_kernel<0>, g, b, lds, s, a);
-  else hipLaunchKernelGGL(so_recon_kernel<1>, g, b, lds, s, a);
+  if (max_mt > kReconMaxThreads) return hipErrorInvalidValue;
+  const dim3 g(rows, a.n_inst);
+  if (max_mt <= kReconThreads) {
+    const dim3 b(kReconThreads);
+    if (kind == 0) hipLaunchKernelGGL((so_recon_kernel<0, kReconThreads>), g, b, lds, s, a);
+    else hipLaunchKernelGGL((so_recon_kernel<1, kReconThreads>), g, b, lds, s, a);
+  } else {
+    const dim3 b((max_mt + 63) / 64 * 64);
+    if (kind == 0) hipLaunchKernelGGL((so_recon_kernel<0, kReconMaxThreads>), g, b, lds, s, a);
+    else hipLaunchKernelGGL((so_recon_kernel<1, kReconMaxThreads>), g, b, lds, s, a);
+  }
   return hipGetLastError();
 }
 
@@ -942,8 +948,10 @@ hipError_t so_set_lds_limit(int bytes) {
       reinterpret_cast<const void*>(&so_linear_kernel<2, 0>),
       reinterpret_cast<const void*>(&so_linear_kernel<2, 1>),
       reinterpret_cast<const void*>(&so_linear_kernel<2, 2>),
-      reinterpret_cast<const void*>(&so_recon_kernel<0>),
-      reinterpret_cast<const void*>(&so_recon_kernel<1>)};
+      reinterpret_cast<const void*>(&so_recon_kernel<0, kReconThreads>),
+      reinterpret_cast<const void*>(&so_recon_kernel<1, kReconThreads>),
+      reinterpret_cast<const void*>(&so_recon_kernel<0, kReconMaxThreads>),
+      reinterpret_cast<const void*>(&so_recon_kernel<1, kReconMaxThreads>)};
   for (const void* f : fns) {
     const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     if (e != hipSuccess) return e;

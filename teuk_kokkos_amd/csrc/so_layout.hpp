@@ -17,7 +17,9 @@ inline constexpr int kSoMaxModes = 16;
 inline constexpr int kSoMaxTargets = 16;
 inline constexpr int kSoMaxPairs = 64;
 inline constexpr int kSoThreads = 256;
-inline constexpr int kReconThreads = 128;
+inline constexpr int kReconThreads = 128;      // reconstruction workgroup for M * N_theta <= 128
+inline constexpr int kReconMaxThreads = 1024;  // largest M * N_theta a plan accepts
+inline constexpr int kReconLdsRows = 11;       // theta rows of M * N_theta values in its LDS
 
 // Angular coordinators of the stage graph (spatial_pipeline.hpp:185-218):
 // (spin, boost, band) over the parent or target modes.
@@ -139,8 +141,13 @@ inline constexpr int kSoReduceDoubles = 19;
 
 hipError_t so_launch_linear(int scheme, int kind, const SoArgs& a, int n_tiles, int lds,
                             hipStream_t s);
-hipError_t so_launch_drad(int which, const SoArgs& a, long long points, hipStream_t s);
-hipError_t so_launch_recon(int kind, const SoArgs& a, int rows, int lds, hipStream_t s);
+// `schemes`: bit k set when some instance uses radial scheme k
+hipError_t so_launch_drad(int which, const SoArgs& a, long long points, unsigned schemes,
+                          hipStream_t s);
+// one workgroup per (instance, r) of roundup(max_mt, 64) threads, or 128 for
+// max_mt <= 128; max_mt = largest M * N_theta of the plan
+hipError_t so_launch_recon(int kind, const SoArgs& a, int rows, int max_mt, int lds,
+                           hipStream_t s);
 hipError_t so_launch_source(const SoArgs& a, long long max_points, hipStream_t s);
 hipError_t so_launch_source_projection(const SoArgs& a, int rows, int lds, hipStream_t s);
 hipError_t so_launch_rk(const SoRkArgs& a, hipStream_t s);

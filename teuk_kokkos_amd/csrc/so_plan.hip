@@ -7,6 +7,7 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <string>
 #include <tuple>
 
 #include "plan_detail.hpp"
@@ -150,7 +151,11 @@ void Plan::so_build() {
         if (it == op_index.end()) {
           const auto ops = nodal_operators(kCoordSpin[c], m, E, T);
           off = static_cast<int>(H.ops.size());
-          for (const auto& o : ops) H.ops.insert(H.ops.end(), o.begin(), o.end());
+          // stored column-major (entry (t, u) at u * T + t): the lanes of a
+          // row operator read consecutive t of one column together
+          for (const auto& o : ops)
+            for (int u = 0; u < T; ++u)
+              for (int t = 0; t < T; ++t) H.ops.push_back(o[static_cast<size_t>(t) * T + u]);
           op_index[key] = off;
         } else {
           off = it->second;
@@ -220,10 +225,14 @@ void Plan::so_build() {
       }
     H.inst_tiles[sc].resize(instance_count(), {0, 0});
     H.inst_tiles[sc][i] = {first, static_cast<int>(H.tiles[sc].size()) - first};
+    H.schemes |= 1u << sc;
     const long long MT = static_cast<long long>(M) * T;
-    H.rec0_lds = std::max(H.rec0_lds, static_cast<int>(16 * 26 * MT));
-    H.rec1_lds = std::max(H.rec1_lds,
-                          static_cast<int>(16 * 26 * MT));
+    if (MT > kReconMaxThreads)
+      throw std::invalid_argument("second-order plans need stored modes x ntheta <= " +
+                                  std::to_string(kReconMaxThreads));
+    H.max_mt = std::max(H.max_mt, static_cast<int>(MT));
+    H.rec0_lds = std::max(H.rec0_lds, static_cast<int>(16 * kReconLdsRows * MT));
+    H.rec1_lds = H.rec0_lds;
     H.proj_lds = std::max(H.proj_lds, static_cast<int>(16 * 8LL * I.n_targets * T));
     H.max_source = std::max(H.max_source, static_cast<long long>(I.n_targets) * N * T);
     H.max_n = std::max(H.max_n, N);
@@ -365,11 +374,12 @@ void Plan::so_rhs(int first, int n, const cdouble* in, cdouble* out, bool constr
     }
   };
   linear(0);
-  check(so_launch_drad(0, a, H.max_plane * kSlotsRdr, stream_), "reconstruction D");
-  check(so_launch_recon(0, a, H.max_n, H.rec0_lds, stream_), "reconstruction chain");
+  check(so_launch_drad(0, a, H.max_plane * kSlotsRdr, H.schemes, stream_), "reconstruction D");
+  check(so_launch_recon(0, a, H.max_n, H.max_mt, H.rec0_lds, stream_), "reconstruction chain");
   linear(1);
-  check(so_launch_drad(1, a, H.max_plane * kSlotsRdr, stream_), "tangent reconstruction D");
-  check(so_launch_recon(1, a, H.max_n, H.rec1_lds, stream_), "tangent reconstruction chain");
+  check(so_launch_drad(1, a, H.max_plane * kSlotsRdr, H.schemes, stream_), "tangent reconstruction D");
+  check(so_launch_recon(1, a, H.max_n, H.max_mt, H.rec1_lds, stream_),
+        "tangent reconstruction chain");
   check(so_launch_source(a, H.max_source, stream_), "quadratic source");
   check(so_launch_source_projection(a, H.max_n, H.proj_lds, stream_), "source projection");
   linear(2);
