@@ -348,24 +348,49 @@ __global__ __launch_bounds__(kSoThreads) void so_linear_kernel(SoArgs a) {
 
 // ------------------------------------------------------------ radial D
 // D of the seven reconstruction fields of the stage (which 0) or of the RHS
-// (which 1) (full_spatial.hpp:400-455).  grid.y = instance.
+// (which 1) (full_spatial.hpp:400-455).  One workgroup per (mode, field,
+// kDradRows radial rows, kDradCols theta columns) of an instance: the rows
+// the stencils reach are staged in LDS once, so HBM sees each value about
+// once instead of once per stencil tap.  grid.y = instance.
 template <int SCHEME>
 __global__ __launch_bounds__(kSoThreads) void so_drad_kernel(SoArgs a, int which) {
+  __shared__ __align__(16) cdouble tile[kDradSpan * kDradCols];
   const SoInstance& I = a.inst[a.first_inst + blockIdx.y];
   if (I.scheme != SCHEME) return;
-  const cdouble* src = which == 0 ? a.in : a.out;
-  cdouble* dst = which == 0 ? a.rdr : a.rdr_t;
+  constexpr int W = SchemeShape<SCHEME>::W, C = SchemeShape<SCHEME>::C;
+  constexpr int B = SchemeShape<SCHEME>::B;
+  const int N = I.N, T = I.T;
+  const int n_rt = (N + kDradRows - 1) / kDradRows, n_tc = (T + kDradCols - 1) / kDradCols;
+  int q = blockIdx.x;
+  if (q >= I.M * 7 * n_rt * n_tc) return;
+  const int tc = q % n_tc;
+  q /= n_tc;
+  const int rt = q % n_rt;
+  q /= n_rt;
+  const int c = q % 7, mode = q / 7;
+  const int r0 = rt * kDradRows, r1 = min(N, r0 + kDradRows);
+  const int t0 = tc * kDradCols, nt = min(kDradCols, T - t0);
+  // rows [lo, hi) cover every stencil of rows [r0, r1): interior taps reach
+  // B rows, the closures the first or last C rows
+  const bool left = r0 < W, right = r1 - 1 + W >= N;
+  int lo = left ? 0 : r0 - B;
+  int hi = right ? N : r1 + B;
+  if (right) lo = min(lo, N - C);
+  if (left) hi = max(hi, C);
+  lo = max(lo, 0);
+  hi = min(hi, N);
+  const cdouble* src = (which == 0 ? a.in : a.out) + fidx(I, mode, 3 + c, 0, 0);
+  for (int e = threadIdx.x; e < (hi - lo) * nt; e += kSoThreads) {
+    const int rr = e / nt, tt = e - rr * nt;
+    tile[rr * kDradCols + tt] = src[static_cast<long long>(lo + rr) * T + t0 + tt];
+  }
+  __syncthreads();
+  cdouble* dst = (which == 0 ? a.rdr : a.rdr_t) + sidx(I, kSlotsRdr, mode, c, 0, 0);
   const SchemeTable& TB = a.tables[SCHEME];
-  const long long total = static_cast<long long>(I.M) * 7 * I.N * I.T;
-  for (long long e = static_cast<long long>(blockIdx.x) * kSoThreads + threadIdx.x; e < total;
-       e += static_cast<long long>(gridDim.x) * kSoThreads) {
-    const int t = static_cast<int>(e % I.T);
-    const long long q = e / I.T;
-    const int r = static_cast<int>(q % I.N);
-    const int mc = static_cast<int>(q / I.N);
-    const int c = mc % 7, mode = mc / 7;
-    const cdouble* col = src + fidx(I, mode, 3 + c, 0, t);
-    dst[sidx(I, kSlotsRdr, mode, c, r, t)] = apply_d<SCHEME>(TB, col, I.T, 0, r, I.N, I.inv_h);
+  for (int e = threadIdx.x; e < (r1 - r0) * nt; e += kSoThreads) {
+    const int rr = e / nt, tt = e - rr * nt, r = r0 + rr;
+    dst[static_cast<long long>(r) * T + t0 + tt] =
+        apply_d<SCHEME>(TB, tile + tt, kDradCols, lo, r, N, I.inv_h);
   }
 }
 
@@ -881,11 +906,10 @@ hipError_t so_launch_linear(int scheme, int kind, const SoArgs& a, int n_tiles, 
   return launch_linear_k<2>(scheme, a, n_tiles, lds, s);
 }
 
-hipError_t so_launch_drad(int which, const SoArgs& a, long long points, unsigned schemes,
+hipError_t so_launch_drad(int which, const SoArgs& a, int blocks, unsigned schemes,
                           hipStream_t s) {
-  if (a.n_inst <= 0) return hipSuccess;
-  const long long want = (points + kSoThreads - 1) / kSoThreads;
-  const dim3 g(static_cast<unsigned>(want < 1 ? 1 : (want > 4096 ? 4096 : want)), a.n_inst);
+  if (a.n_inst <= 0 || blocks <= 0) return hipSuccess;
+  const dim3 g(blocks, a.n_inst);
   if (schemes & 1u) hipLaunchKernelGGL(so_drad_kernel<0>, g, dim3(kSoThreads), 0, s, a, which);
   if (schemes & 2u) hipLaunchKernelGGL(so_drad_kernel<1>, g, dim3(kSoThreads), 0, s, a, which);
   if (schemes & 4u) hipLaunchKernelGGL(so_drad_kernel<2>, g, dim3(kSoThreads), 0, s, a, which);

@@ -17,6 +17,14 @@ inline constexpr int kSoMaxModes = 16;
 inline constexpr int kSoMaxTargets = 16;
 inline constexpr int kSoMaxPairs = 64;
 inline constexpr int kSoThreads = 256;
+// radial D workgroups: kDradRows output rows x kDradCols theta columns, with
+// the stencil reach (at most one closure width each side) staged in LDS
+inline constexpr int kDradRows = 64;
+inline constexpr int kDradCols = 16;
+inline constexpr int kDradSpan = kDradRows + 2 * kMaxClosureWidth;
+inline int so_drad_blocks(int M, int N, int T) {
+  return M * 7 * ((N + kDradRows - 1) / kDradRows) * ((T + kDradCols - 1) / kDradCols);
+}
 inline constexpr int kReconThreads = 128;      // reconstruction workgroup for M * N_theta <= 128
 inline constexpr int kReconMaxThreads = 1024;  // largest M * N_theta a plan accepts
 inline constexpr int kReconLdsRows = 11;       // theta rows of M * N_theta values in its LDS
@@ -141,8 +149,9 @@ inline constexpr int kSoReduceDoubles = 19;
 
 hipError_t so_launch_linear(int scheme, int kind, const SoArgs& a, int n_tiles, int lds,
                             hipStream_t s);
-// `schemes`: bit k set when some instance uses radial scheme k
-hipError_t so_launch_drad(int which, const SoArgs& a, long long points, unsigned schemes,
+// `blocks`: largest so_drad_blocks() of the plan's instances; `schemes`: bit
+// k set when some instance uses radial scheme k
+hipError_t so_launch_drad(int which, const SoArgs& a, int blocks, unsigned schemes,
                           hipStream_t s);
 // one workgroup per (instance, r) of roundup(max_mt, 64) threads, or 128 for
 // max_mt <= 128; max_mt = largest M * N_theta of the plan

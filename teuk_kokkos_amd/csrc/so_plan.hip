@@ -231,6 +231,7 @@ void Plan::so_build() {
       throw std::invalid_argument("second-order plans need stored modes x ntheta <= " +
                                   std::to_string(kReconMaxThreads));
     H.max_mt = std::max(H.max_mt, static_cast<int>(MT));
+    H.max_drad = std::max(H.max_drad, so_drad_blocks(M, N, T));
     H.rec0_lds = std::max(H.rec0_lds, static_cast<int>(16 * kReconLdsRows * MT));
     H.rec1_lds = H.rec0_lds;
     H.proj_lds = std::max(H.proj_lds, static_cast<int>(16 * 8LL * I.n_targets * T));
@@ -374,10 +375,10 @@ void Plan::so_rhs(int first, int n, const cdouble* in, cdouble* out, bool constr
     }
   };
   linear(0);
-  check(so_launch_drad(0, a, H.max_plane * kSlotsRdr, H.schemes, stream_), "reconstruction D");
+  check(so_launch_drad(0, a, H.max_drad, H.schemes, stream_), "reconstruction D");
   check(so_launch_recon(0, a, H.max_n, H.max_mt, H.rec0_lds, stream_), "reconstruction chain");
   linear(1);
-  check(so_launch_drad(1, a, H.max_plane * kSlotsRdr, H.schemes, stream_), "tangent reconstruction D");
+  check(so_launch_drad(1, a, H.max_drad, H.schemes, stream_), "tangent reconstruction D");
   check(so_launch_recon(1, a, H.max_n, H.max_mt, H.rec1_lds, stream_),
         "tangent reconstruction chain");
   check(so_launch_source(a, H.max_source, stream_), "quadratic source");
