@@ -138,45 +138,48 @@ __device__ __forceinline__ Jet jconj(Jet a) { return {cconj(a.v), cconj(a.d)}; }
 // corrected_ordered_pair_source (second_order.hpp:74-142) on jets.
 // f: F1 G1 Lambda1 Pi1 B1 C1 U1 F2 G2 H2 B2 C2 U2 U2# C2# C1# B1# Pi2# B2#
 // d: d1F2 d3U2 eth2C2 eth'2C2# eth1B2 d2C2 d2G2 eth2G2 eth'1F2 d2C2# eth'1B2#
-__device__ void pair_source(double R, const Bg& bg, const Jet* f, const Jet* d, Jet& D, Jet& T) {
+// f(i) and d(i) return the field and derivative jets on demand, so each is
+// loaded next to the terms that use it rather than all at once.
+template <class FJ, class DJ>
+__device__ __forceinline__ void pair_source(double R, const Bg& bg, FJ f, DJ d, Jet& D, Jet& T) {
   enum { F1, G1, La1, Pi1, B1, C1, U1, F2, G2, H2, B2, C2, U2, U2s, C2s, C1s, B1s, Pi2s, B2s };
   enum { d1F2, d3U2, e2C2, ep2C2s, e1B2, d2C2, d2G2, e2G2, ep1F2, d2C2s, ep1B2s };
   const cdouble mu0b = cconj(bg.mu0), pi0b = cconj(bg.pi0), tau0b = cconj(bg.tau0);
   const double R2 = R * R;
   Jet acc;
   // sd01-sd02
-  acc = jmul(jscaler(f[U1], 0.5), jadd(jdiv(d[d1F2], bg.mu0), jscaler(f[F2], R)));
+  acc = jmul(jscaler(f(U1), 0.5), jadd(jdiv(d(d1F2), bg.mu0), jscaler(f(F2), R)));
   // sd03
-  acc = jadd(acc, jmul(f[F1], jscaler(d[e2C2], 0.5 * R)));
+  acc = jadd(acc, jmul(f(F1), jscaler(d(e2C2), 0.5 * R)));
   // sd04
-  acc = jadd(acc, jmul(f[F1], jscale(f[C2], cscale(0.5 * R2, cadd(pi0b, cscale(2.0, bg.tau0))))));
+  acc = jadd(acc, jmul(f(F1), jscale(f(C2), cscale(0.5 * R2, cadd(pi0b, cscale(2.0, bg.tau0))))));
   // sd05
-  acc = jadd(acc, jmul(f[F1], jadd(jdiv(d[d3U2], bg.mu0), jscaler(f[U2s], R))));
+  acc = jadd(acc, jmul(f(F1), jadd(jdiv(d(d3U2), bg.mu0), jscaler(f(U2s), R))));
   // sd06
-  acc = jadd(acc, jmul(f[F1], jscaler(d[ep2C2s], -0.5 * R)));
+  acc = jadd(acc, jmul(f(F1), jscaler(d(ep2C2s), -0.5 * R)));
   // sd07
-  acc = jadd(acc, jmul(f[F1], jscale(f[C2s], cscale(0.5 * R2, cadd(cscale(5.0, bg.pi0),
+  acc = jadd(acc, jmul(f(F1), jscale(f(C2s), cscale(0.5 * R2, cadd(cscale(5.0, bg.pi0),
                                                                   cscale(4.0, tau0b))))));
   // sd08 - sd11: -0.5 G1 * (...)
-  const Jet g1 = jscaler(f[G1], -0.5);
-  acc = jadd(acc, jmul(g1, jscaler(d[e1B2], R)));
-  acc = jadd(acc, jmul(g1, jscale(f[B2], cscale(R2, cadd(pi0b, bg.tau0)))));
-  acc = jadd(acc, jmul(g1, jscaler(d[d2C2], R)));
-  acc = jadd(acc, jmul(g1, jscale(f[C2], cscale(R2, cadd(cscale(-2.0, bg.mu0), mu0b)))));
+  const Jet g1 = jscaler(f(G1), -0.5);
+  acc = jadd(acc, jmul(g1, jscaler(d(e1B2), R)));
+  acc = jadd(acc, jmul(g1, jscale(f(B2), cscale(R2, cadd(pi0b, bg.tau0)))));
+  acc = jadd(acc, jmul(g1, jscaler(d(d2C2), R)));
+  acc = jadd(acc, jmul(g1, jscale(f(C2), cscale(R2, cadd(cscale(-2.0, bg.mu0), mu0b)))));
   // sd12 - sd15
-  acc = jadd(acc, jmul(jscaler(f[C1], -R), d[d2G2]));
-  acc = jadd(acc, jmul(jscaler(f[B1], 0.5 * R), d[e2G2]));
-  acc = jadd(acc, jmul(jscaler(f[Pi1], 4.0 * R), f[G2]));
-  acc = jadd(acc, jmul(jscaler(f[La1], -3.0 * R), f[H2]));
+  acc = jadd(acc, jmul(jscaler(f(C1), -R), d(d2G2)));
+  acc = jadd(acc, jmul(jscaler(f(B1), 0.5 * R), d(e2G2)));
+  acc = jadd(acc, jmul(jscaler(f(Pi1), 4.0 * R), f(G2)));
+  acc = jadd(acc, jmul(jscaler(f(La1), -3.0 * R), f(H2)));
   D = acc;
   // st01-st02
-  Jet t = jmul(jneg(f[C1s]),
-               jadd(d[d1F2], jscale(f[F2], cscale(R, cadd(bg.mu0, cscale(2.0, mu0b))))));
-  t = jadd(t, jmul(jscaler(f[B1s], 0.5), d[ep1F2]));
-  t = jadd(t, jmul(f[F1], f[Pi2s]));
-  t = jadd(t, jmul(jneg(f[F1]), d[d2C2s]));
-  t = jadd(t, jmul(f[F1], d[ep1B2s]));
-  t = jadd(t, jmul(f[F1], jscale(f[B2s], cscale(-0.5 * R, cadd(bg.pi0, tau0b)))));
+  Jet t = jmul(jneg(f(C1s)),
+               jadd(d(d1F2), jscale(f(F2), cscale(R, cadd(bg.mu0, cscale(2.0, mu0b))))));
+  t = jadd(t, jmul(jscaler(f(B1s), 0.5), d(ep1F2)));
+  t = jadd(t, jmul(f(F1), f(Pi2s)));
+  t = jadd(t, jmul(jneg(f(F1)), d(d2C2s)));
+  t = jadd(t, jmul(f(F1), d(ep1B2s)));
+  t = jadd(t, jmul(f(F1), jscale(f(B2s), cscale(-0.5 * R, cadd(bg.pi0, tau0b)))));
   T = t;
 }
 
@@ -395,10 +398,11 @@ __global__ __launch_bounds__(kSoThreads) void so_drad_kernel(SoArgs a, int which
 }
 
 // ------------------------------------------------------------ reconstruction
-// KIND 0: chain of (stage, k) -> k[3..9] and the stage angular inputs, plus
-//         the independent constraints when asked.
-// KIND 1: chain of (k, tangent) -> tangent[3..9], and the source's angular
-//         inputs of the tangent chain and of eth1 B, eth'1 F.
+// KIND 0: chain of (stage, k) -> k[3..9], the stage angular inputs of the
+//         source and its eth1 B, eth'1 F (with k as their time derivatives),
+//         plus the independent constraints when asked.
+// KIND 1: chain of (k, tangent) -> tangent[3..9], the tangent angular inputs
+//         and eth1 B, eth'1 F of k (with the tangent as time derivatives).
 // Reconstruction passes: full_spatial.hpp:486-693, reconstruction.hpp:48-105;
 // chain order: spatial_pipeline.hpp:664-754; constraints: 768-893,
 // reconstruction_constraints.hpp:24-56; source inputs: 934-1181,
@@ -424,7 +428,7 @@ __global__ __launch_bounds__(NT) void so_recon_kernel(SoArgs a) {
   const bool live = tid < MT;
   const int e = live ? tid : 0;
   const int mode = e / T, t = e - mode * T;
-  const int se = I.sharp[mode] * T + t;
+  const int sm = I.sharp[mode], se = sm * T + t;
   const double R = I.h * r, R2 = R * R, R3 = R2 * R;
   const double mass = I.mass, aspin = I.spin, L = I.length;
   const double* nodes = node_ptr(a, I);
@@ -433,14 +437,25 @@ __global__ __launch_bounds__(NT) void so_recon_kernel(SoArgs a) {
   enum { lF = 0, lG, lB, lPi, lC, ltB, ltC, lRaw };
   auto row = [&](int k) { return LX + k * MT; };
   auto op = [&](int coord, int kind) { return I.op[coord][mode][kind]; };
+  auto eth = [&](int coord, bool prime, cdouble xo, cdouble dto, cdouble ang) {
+    return eth_point(prime, xo, dto, ang, I.coord_spin[coord], I.coord_boost[coord], R, sx, cx,
+                     aspin, L);
+  };
   // eth / eth' of coordinator `coord` at this element: theta row `xr` of the
   // field (every mode), its own value `xo` and time derivative `dto`
   auto ghp = [&](int coord, bool prime, const cdouble* xr, cdouble xo, cdouble dto) {
     const int off = op(coord, prime ? kOpLower : kOpRaise);
     if (off < 0) return cmake(0.0, 0.0);
-    const cdouble ang = row_op<CH>(a.ops, off, xr + mode * T, t, T);
-    return eth_point(prime, xo, dto, ang, I.coord_spin[coord], I.coord_boost[coord], R, sx, cx,
-                     aspin, L);
+    return eth(coord, prime, xo, dto, row_op<CH>(a.ops, off, xr + mode * T, t, T));
+  };
+  // eth' of the sharp field conj(X[-m]) from the rows of X and its time
+  // derivative: the operators are real, so the angular part is the
+  // conjugate of the operator applied to the partner's row (bit for bit)
+  auto ghp_sharp = [&](int coord, const cdouble* xr, const cdouble* dr_row) {
+    const int off = op(coord, kOpLower);
+    if (off < 0) return cmake(0.0, 0.0);
+    const cdouble ang = cconj(row_op<CH>(a.ops, off, xr + sm * T, t, T));
+    return eth(coord, true, cconj(xr[se]), cconj(dr_row[se]), ang);
   };
   auto project = [&](int coord, const cdouble* in) {
     return row_op<CH>(a.ops, op(coord, kOpProject), in + mode * T, t, T);
@@ -455,10 +470,15 @@ __global__ __launch_bounds__(NT) void so_recon_kernel(SoArgs a) {
     else dst[sidx(I, kSlotsTan, mode, 2 + k, r, t)] = v;
   };
   auto dr = [&](int c) { return rdr[sidx(I, kSlotsRdr, mode, c, r, t)]; };
+  // eth1 B and eth'1 F of `src` with the chain's time derivatives: slots
+  // 0, 1 from the stage chain, 2, 3 from the tangent chain
+  auto ex = [&](int k, cdouble v) { a.ex[sidx(I, kSlotsEx, mode, 2 * KIND + k, r, t)] = v; };
 
   cdouble xF{}, xG{}, xLa{}, xH{}, xB{}, xPi{}, xC{}, xU{}, tF{}, tG{}, tH{}, tPi{}, tB{}, tC{};
   cdouble e1F{}, e2G{}, e2C{}, e2Pi{};
   Bg bg{};
+  // D of the seven fields, each requested one phase before it is needed
+  cdouble d0{}, d1{}, d2{}, d3{}, d4{}, d5{}, d6{};
   // stage the row
   if (live) {
     xF = load(0);
@@ -471,6 +491,8 @@ __global__ __launch_bounds__(NT) void so_recon_kernel(SoArgs a) {
     xU = load(7);
     tF = KIND == 0 ? a.out[fidx(I, mode, 2, r, t)] : a.tan[sidx(I, kSlotsTan, mode, 2, r, t)];
     bg = bg_at(a, I, true, r, t);
+    d0 = dr(0);
+    d1 = dr(1);
     row(lF)[e] = xF;
     row(lG)[e] = xG;
     row(lB)[e] = xB;
@@ -481,14 +503,19 @@ __global__ __launch_bounds__(NT) void so_recon_kernel(SoArgs a) {
   // pass 1: G, Lambda
   if (live) {
     e1F = ghp(kCoFirst, false, row(lF), xF, tF);
+    ex(1, ghp(kCoFirst, true, row(lF), xF, tF));
     const cdouble dg = csub(cadd(cmul(cscale(-4.0 * R, bg.mu0), xG), e1F),
                             cmul(cscale(R, bg.tau0), xF));
     const cdouble dl = csub(cmul(cscale(-R, cadd(bg.mu0, cconj(bg.mu0))), xLa), xF);
-    row(lRaw)[e] = recon_dt(xG, dr(0), dg, 2, R, mass, L);
-    row(lRaw + 1)[e] = recon_dt(xLa, dr(1), dl, 1, R, mass, L);
+    row(lRaw)[e] = recon_dt(xG, d0, dg, 2, R, mass, L);
+    row(lRaw + 1)[e] = recon_dt(xLa, d1, dl, 1, R, mass, L);
   }
   __syncthreads();
   if (live) {
+    d2 = dr(2);
+    d3 = dr(3);
+    d4 = dr(4);
+    d5 = dr(5);
     tG = project(kCoG, row(lRaw));
     store(1, tG);
     store(2, project(kCoFirst, row(lRaw + 1)));
@@ -506,13 +533,14 @@ __global__ __launch_bounds__(NT) void so_recon_kernel(SoArgs a) {
                              cmul(cmul(cscale(0.5 * R2, bg.mu0), pt), xB));
     const cdouble dc = csub(csub(cmul(cscale(-R, mub), xC), cscale(2.0, xPi)),
                             cmul(cscale(R, bg.tau0), xB));
-    row(lRaw)[e] = recon_dt(xH, dr(2), dh, 3, R, mass, L);
-    row(lRaw + 1)[e] = recon_dt(xB, dr(3), db, 1, R, mass, L);
-    row(lRaw + 2)[e] = recon_dt(xPi, dr(4), dpi, 2, R, mass, L);
-    row(lRaw + 3)[e] = recon_dt(xC, dr(5), dc, 2, R, mass, L);
+    row(lRaw)[e] = recon_dt(xH, d2, dh, 3, R, mass, L);
+    row(lRaw + 1)[e] = recon_dt(xB, d3, db, 1, R, mass, L);
+    row(lRaw + 2)[e] = recon_dt(xPi, d4, dpi, 2, R, mass, L);
+    row(lRaw + 3)[e] = recon_dt(xC, d5, dc, 2, R, mass, L);
   }
   __syncthreads();
   if (live) {
+    d6 = dr(6);
     tH = project(kCoScalar, row(lRaw));
     tB = project(kCoB, row(lRaw + 1));
     tPi = project(kCoPi, row(lRaw + 2));
@@ -525,22 +553,16 @@ __global__ __launch_bounds__(NT) void so_recon_kernel(SoArgs a) {
     row(ltC)[e] = tC;
     e2C = ghp(kCoC, false, row(lC), xC, tC);
     e2Pi = ghp(kCoPi, false, row(lPi), xPi, tPi);
+    ex(0, ghp(kCoB, false, row(lB), xB, tB));
   }
   __syncthreads();
-  // sharp fields conj(X[-m]) (spatial_pipeline.hpp:895-932)
+  // pass 3: U, with the sharp fields conj(X[-m]) (spatial_pipeline.hpp:
+  // 895-932) read from their partners' rows.  The raw rows are free from
+  // here on: the first takes the raw U derivative, the second the H row
+  // (for the constraint eth').
   if (live) {
-    row(lRaw)[e] = cconj(row(lB)[se]);
-    row(lRaw + 1)[e] = cconj(row(lC)[se]);
-    row(lRaw + 2)[e] = cconj(row(ltB)[se]);
-    row(lRaw + 3)[e] = cconj(row(ltC)[se]);
-  }
-  __syncthreads();
-  // pass 3: U.  The time-derivative rows of B and C are free from here on:
-  // the raw U derivative goes to the first, the H row (for the constraint
-  // eth') to the second.
-  if (live) {
-    const cdouble ep1BS = ghp(kCoBSharp, true, row(lRaw), row(lRaw)[e], row(lRaw + 2)[e]);
-    const cdouble ep2CS = ghp(kCoCSharp, true, row(lRaw + 1), row(lRaw + 1)[e], row(lRaw + 3)[e]);
+    const cdouble ep1BS = ghp_sharp(kCoBSharp, row(lB), row(ltB));
+    const cdouble ep2CS = ghp_sharp(kCoCSharp, row(lC), row(ltC));
     const cdouble mu0b = cconj(bg.mu0), pi0b = cconj(bg.pi0), tau0b = cconj(bg.tau0);
     const cdouble pi_s = cconj(row(lPi)[se]), b_s = cconj(row(lB)[se]), c_s = cconj(row(lC)[se]);
     cdouble du = cmul(cscale(-R, mu0b), xU);
@@ -556,8 +578,8 @@ __global__ __launch_bounds__(NT) void so_recon_kernel(SoArgs a) {
     const cdouble sc = cadd(csub(cscale(-3.0, cmul(bg.mu0, bg.pi0)), cscale(-2.0, cmul(mu0b, bg.pi0))),
                             cscale(-2.0, cmul(bg.mu0, tau0b)));
     du = cadd(du, cmul(cscale(R2, sc), c_s));
-    row(ltB)[e] = recon_dt(xU, dr(6), du, 3, R, mass, L);
-    row(ltC)[e] = xH;
+    row(lRaw)[e] = recon_dt(xU, d6, du, 3, R, mass, L);
+    row(lRaw + 1)[e] = xH;
     cdouble* ro = KIND == 0 ? a.rang : a.rang_t;
     ro[sidx(I, kSlotsRang, mode, kRangEth2C, r, t)] = e2C;
     ro[sidx(I, kSlotsRang, mode, kRangEthP2CSharp, r, t)] = ep2CS;
@@ -565,9 +587,10 @@ __global__ __launch_bounds__(NT) void so_recon_kernel(SoArgs a) {
     ro[sidx(I, kSlotsRang, mode, kRangEthP1BSharp, r, t)] = ep1BS;
   }
   __syncthreads();
+  if (!live) return;
+  store(7, project(kCoScalar, row(lRaw)));
   if constexpr (KIND == 0) {
-    if (live) store(7, project(kCoScalar, row(ltB)));
-    if (!a.want_constraints || !live) return;
+    if (!a.want_constraints) return;
     // reloaded rather than held in registers through the chain
     xF = load(0);
     xG = load(1);
@@ -581,9 +604,9 @@ __global__ __launch_bounds__(NT) void so_recon_kernel(SoArgs a) {
     tG = a.out[fidx(I, mode, 3, r, t)];
     tH = a.out[fidx(I, mode, 5, r, t)];
     const cdouble ethpG = ghp(kCoG, true, row(lG), xG, tG);
-    const cdouble ethpH = ghp(kCoScalar, true, row(ltC), xH, tH);
+    const cdouble ethpH = ghp(kCoScalar, true, row(lRaw + 1), xH, tH);
     const Bg bs = bg_at(a, I, false, r, t);
-    const cdouble Us = cconj(a.in[fidx(I, I.sharp[mode], 9, r, t)]);
+    const cdouble Us = cconj(a.in[fidx(I, sm, 9, r, t)]);
     const int m = I.modes[mode];
     const cdouble th1 = thorn_n(xF, tF, a.dpsi[sidx(I, 1, mode, 0, r, t)], 1, -2, -2, m, R, cx,
                                 mass, aspin, L, bs.eps0);
@@ -603,29 +626,6 @@ __global__ __launch_bounds__(NT) void so_recon_kernel(SoArgs a) {
     for (int k = 0; k < 3; ++k)
       a.cons[sidx(I, kSlotsCons, mode, k, r, t)] = csub(tv[2 * k], tv[2 * k + 1]);
     for (int k = 0; k < 6; ++k) a.terms[sidx(I, kSlotsTerms, mode, k, r, t)] = tv[k];
-  } else {
-    // the tangent of U, then the source's extra angular inputs: eth1 B and
-    // eth'1 F of the stage with their tangents (the stage's B and F rows
-    // go to the two first raw rows, free since pass 3)
-    cdouble sB{}, sF{};
-    if (live) {
-      store(7, project(kCoScalar, row(ltB)));
-      sB = a.in[fidx(I, mode, 6, r, t)];
-      sF = a.in[fidx(I, mode, 2, r, t)];
-      row(lRaw)[e] = sB;
-      row(lRaw + 1)[e] = sF;
-    }
-    __syncthreads();
-    if (!live) return;
-    // reloaded rather than held in registers through the chain
-    xB = load(4);
-    xF = load(0);
-    tB = a.tan[sidx(I, kSlotsTan, mode, 6, r, t)];
-    tF = a.tan[sidx(I, kSlotsTan, mode, 2, r, t)];
-    a.ex[sidx(I, kSlotsEx, mode, 0, r, t)] = ghp(kCoB, false, row(lRaw), sB, xB);
-    a.ex[sidx(I, kSlotsEx, mode, 1, r, t)] = ghp(kCoFirst, true, row(lRaw + 1), sF, xF);
-    a.ex[sidx(I, kSlotsEx, mode, 2, r, t)] = ghp(kCoB, false, row(lB), xB, tB);
-    a.ex[sidx(I, kSlotsEx, mode, 3, r, t)] = ghp(kCoFirst, true, row(lF), xF, tF);
   }
 }
 
@@ -655,43 +655,65 @@ __global__ __launch_bounds__(kSoThreads) void so_source_kernel(SoArgs a) {
     auto rg = [&](int mode, int k) { return a.rang[sidx(I, kSlotsRang, mode, k, r, t)]; };
     auto rgt = [&](int mode, int k) { return a.rang_t[sidx(I, kSlotsRang, mode, k, r, t)]; };
     auto ex = [&](int mode, int k) { return a.ex[sidx(I, kSlotsEx, mode, k, r, t)]; };
-    // the eleven derivative jets of a mode (spatial_pipeline.hpp:1019-1180)
-    auto derivs = [&](int mode, Jet* d) {
-      const int s = I.sharp[mode];
-      auto dl = [&](int k, int n, int c) {
-        const cdouble x = X(mode, k);
-        return Jet{delta_n(S(mode, k), x, rd(mode, c), n, R, mass, L),
-                   delta_n(x, XT(mode, k), rdt(mode, c), n, R, mass, L)};
-      };
-      const cdouble xf = X(mode, xF);
-      d[0] = {delta_n(S(mode, xF), xf, a.dpsi[sidx(I, 1, mode, 0, r, t)], 1, R, mass, L),
-              delta_n(xf, XT(mode, xF), a.dpsi_t[sidx(I, 1, mode, 0, r, t)], 1, R, mass, L)};
-      d[1] = dl(xU, 3, 6);
-      d[2] = {rg(mode, kRangEth2C), rgt(mode, kRangEth2C)};
-      d[3] = {rg(mode, kRangEthP2CSharp), rgt(mode, kRangEthP2CSharp)};
-      d[4] = {ex(mode, 0), ex(mode, 2)};
-      d[5] = dl(xC, 2, 5);
-      d[6] = dl(xG, 2, 0);
-      d[7] = {rg(mode, kRangEth2G), rgt(mode, kRangEth2G)};
-      d[8] = {ex(mode, 1), ex(mode, 3)};
-      const cdouble xcs = cconj(X(s, xC));
-      d[9] = {delta_n(cconj(S(s, xC)), xcs, cconj(rd(s, 5)), 2, R, mass, L),
-              delta_n(xcs, cconj(XT(s, xC)), cconj(rdt(s, 5)), 2, R, mass, L)};
-      d[10] = {rg(mode, kRangEthP1BSharp), rgt(mode, kRangEthP1BSharp)};
-    };
     const Bg bg = bg_at(a, I, false, r, t);
     Jet totD{cmake(0.0, 0.0), cmake(0.0, 0.0)}, totT = totD;
     for (int k = I.pair_begin[ts]; k < I.pair_begin[ts + 1]; ++k) {
       const int m1 = I.pair_m1[k], m2 = I.pair_m2[k];
       const int s1 = I.sharp[m1], s2 = I.sharp[m2];
       auto fj = [&](int mode, int k2) { return Jet{S(mode, k2), X(mode, k2)}; };
-      const Jet f[19] = {fj(m1, xF),  fj(m1, xG),  fj(m1, xLa), fj(m1, xPi), fj(m1, xB),
-                         fj(m1, xC),  fj(m1, xU),  fj(m2, xF),  fj(m2, xG),  fj(m2, xH),
-                         fj(m2, xB),  fj(m2, xC),  fj(m2, xU),  jconj(fj(s2, xU)),
-                         jconj(fj(s2, xC)), jconj(fj(s1, xC)), jconj(fj(s1, xB)),
-                         jconj(fj(s2, xPi)), jconj(fj(s2, xB))};
-      Jet d[11];
-      derivs(m2, d);
+      // field jets: value from the stage, tangent from its RHS
+      auto f = [&](int i) -> Jet {
+        switch (i) {
+          case 0: return fj(m1, xF);
+          case 1: return fj(m1, xG);
+          case 2: return fj(m1, xLa);
+          case 3: return fj(m1, xPi);
+          case 4: return fj(m1, xB);
+          case 5: return fj(m1, xC);
+          case 6: return fj(m1, xU);
+          case 7: return fj(m2, xF);
+          case 8: return fj(m2, xG);
+          case 9: return fj(m2, xH);
+          case 10: return fj(m2, xB);
+          case 11: return fj(m2, xC);
+          case 12: return fj(m2, xU);
+          case 13: return jconj(fj(s2, xU));
+          case 14: return jconj(fj(s2, xC));
+          case 15: return jconj(fj(s1, xC));
+          case 16: return jconj(fj(s1, xB));
+          case 17: return jconj(fj(s2, xPi));
+          default: return jconj(fj(s2, xB));
+        }
+      };
+      // the eleven derivative jets of m2 (spatial_pipeline.hpp:1019-1180)
+      auto dl = [&](int mode, int k2, int n, int c) {
+        const cdouble x = X(mode, k2);
+        return Jet{delta_n(S(mode, k2), x, rd(mode, c), n, R, mass, L),
+                   delta_n(x, XT(mode, k2), rdt(mode, c), n, R, mass, L)};
+      };
+      auto d = [&](int i) -> Jet {
+        switch (i) {
+          case 0: {
+            const cdouble xf = X(m2, xF);
+            return {delta_n(S(m2, xF), xf, a.dpsi[sidx(I, 1, m2, 0, r, t)], 1, R, mass, L),
+                    delta_n(xf, XT(m2, xF), a.dpsi_t[sidx(I, 1, m2, 0, r, t)], 1, R, mass, L)};
+          }
+          case 1: return dl(m2, xU, 3, 6);
+          case 2: return {rg(m2, kRangEth2C), rgt(m2, kRangEth2C)};
+          case 3: return {rg(m2, kRangEthP2CSharp), rgt(m2, kRangEthP2CSharp)};
+          case 4: return {ex(m2, 0), ex(m2, 2)};
+          case 5: return dl(m2, xC, 2, 5);
+          case 6: return dl(m2, xG, 2, 0);
+          case 7: return {rg(m2, kRangEth2G), rgt(m2, kRangEth2G)};
+          case 8: return {ex(m2, 1), ex(m2, 3)};
+          case 9: {
+            const cdouble xcs = cconj(X(s2, xC));
+            return {delta_n(cconj(S(s2, xC)), xcs, cconj(rd(s2, 5)), 2, R, mass, L),
+                    delta_n(xcs, cconj(XT(s2, xC)), cconj(rdt(s2, 5)), 2, R, mass, L)};
+          }
+          default: return {rg(m2, kRangEthP1BSharp), rgt(m2, kRangEthP1BSharp)};
+        }
+      };
       Jet Dp, Tp;
       pair_source(R, bg, f, d, Dp, Tp);
       totD = jadd(totD, Dp);
