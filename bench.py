@@ -158,7 +158,7 @@ def main() -> int:
         achieved = BYTES_PER_GRID_STEP_13 / 4.0 * grid / stage_s / 1e9
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                    "kernel": "second-order stage graph (8 launches per RK stage)",
+                    "kernel": "second-order stage graph (10 launches per RK stage)",
                     "avg_stage_ms": stage_s * 1e3}
     elif timed_steps:
         # One RK stage = one launch per execution group, the groups running
