@@ -4,6 +4,7 @@
 #include "plan_detail.hpp"
 #include "so_host.hpp"
 #include "init_layout.hpp"
+#include "import_layout.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -128,7 +129,10 @@ Plan::~Plan() {
                   static_cast<void*>(d_mats_), static_cast<void*>(d_nodes_),
                   static_cast<void*>(d_tables_), static_cast<void*>(d_line_mode_),
                   static_cast<void*>(d_y_), static_cast<void*>(d_acc_), static_cast<void*>(d_ya_),
-                  static_cast<void*>(d_yb_), static_cast<void*>(d_k_)})
+                  static_cast<void*>(d_yb_), static_cast<void*>(d_k_),
+                  static_cast<void*>(d_imp_in_), static_cast<void*>(d_imp_res_),
+                  static_cast<void*>(d_imp_flags_), d_imp_rows_,
+                  static_cast<void*>(d_imp_mats_)})
     if (p) hipFree(p);
   so_free();
   if (stream_) hipStreamDestroy(stream_);
@@ -618,72 +622,151 @@ void Plan::initialize_gaussian() {
   }
 }
 
-// Strict band-validated import (pipeline_bands.hpp:108-232).
-double Plan::import_state(int i, const cplx* host, double tol) {
-  if (so_mode_) return so_import(i, host, tol);
+// Strict band-validated import (pipeline_bands.hpp:108-232) on the GPU.
+// Every (mode, field, r) theta row x is projected onto the field's declared
+// band (spin of the field, m, ell_max_first or ell_max_second; nothing for a
+// mode the field is inactive on) and |x - S A x| / |x| is reduced to its
+// maximum over the instance.
+double Plan::import_check(int i, const cplx* host, double tol) {
+  if (!std::isfinite(tol) || tol < 0.0)
+    throw std::invalid_argument("pipeline band tolerance must be nonnegative");
+  import_release();
   Instance& in = inst_.at(i);
   const int M = in.M, N = in.N, T = in.T;
   static const int spins[13] = {-2, -2, -2, -1, -2, 0, -2, -1, -1, 0, -2, -2, -2};
-  auto at = [&](int mode, int f, int r) { return host + ((static_cast<size_t>(mode) * 13 + f) * N + r) * T; };
-  double combined = 0.0;
+  std::vector<ImportRow> rows(static_cast<size_t>(M) * 13);
+  std::vector<double> mats;
+  std::map<std::tuple<int, int, int>, ImportRow> pool;
   for (int mi = 0; mi < M; ++mi) {
     const int m = in.registry.modes()[mi];
     for (int f = 0; f < 13; ++f) {
+      ImportRow R{-1, 0, 0, 0, 0};
       const bool active = f >= 10 ? in.registry.is_target(m) : in.registry.is_parent(m);
       const int emax = f >= 10 ? in.params.ell_max_second : in.params.ell_max_first;
-      std::unique_ptr<BandTransform> bt;
-      if (active) bt = std::make_unique<BandTransform>(make_band_transform(spins[f], m, emax, T));
-      for (int r = 0; r < N; ++r) {
-        const cplx* v = at(mi, f, r);
-        std::vector<cplx> proj(T, cplx(0.0, 0.0));
-        if (active) proj = bt->synthesize(bt->analyze(v));
-        double line2 = 0.0, res2 = 0.0;
-        for (int j = 0; j < T; ++j) {
-          line2 += std::norm(v[j]);
-          res2 += std::norm(v[j] - proj[j]);
+      if (active) {
+        const auto key = std::make_tuple(spins[f], m, emax);
+        auto it = pool.find(key);
+        if (it == pool.end()) {
+          const BandTransform bt = make_band_transform(spins[f], m, emax, T);
+          ImportRow P{-1, 0, 0, 0, bt.modes() > 0 ? bt.modes() : 0};
+          if (P.n > kMaxImportBand)
+            throw std::invalid_argument("imported band is wider than the import supports");
+          P.s_off = static_cast<int>(mats.size());
+          mats.insert(mats.end(), bt.S.begin(), bt.S.end());
+          P.a_off = static_cast<int>(mats.size());
+          mats.insert(mats.end(), bt.A.begin(), bt.A.end());
+          P.c_off = P.a_off;
+          if (bt.gram) {
+            P.c_off = static_cast<int>(mats.size());
+            mats.insert(mats.end(), bt.B.begin(), bt.B.end());
+          }
+          it = pool.emplace(key, P).first;
         }
-        const double rel =
-            std::sqrt(res2) / std::max(std::sqrt(line2), std::numeric_limits<double>::min());
-        combined = std::max(combined, rel);
+        R = it->second;
       }
+      if (!so_mode_ && f < 3 && in.line_of_mode[mi] >= 0)
+        R.modal_base = lines_[in.line_of_mode[mi]].base;
+      rows[static_cast<size_t>(mi) * 13 + f] = R;
     }
   }
-  if (!std::isfinite(tol) || tol < 0.0)
-    throw std::invalid_argument("pipeline band tolerance must be nonnegative");
-  if (!std::isfinite(combined) || combined > tol)
+  const long long count = nodal_value_count(i);
+  if (count > imp_cap_) {
+    for (void* p : {static_cast<void*>(d_imp_in_), static_cast<void*>(d_imp_res_)})
+      if (p) hipFree(p);
+    d_imp_in_ = d_imp_res_ = nullptr;
+    imp_cap_ = 0;
+    d_imp_in_ = device_alloc<cdouble>(count);
+    d_imp_res_ = device_alloc<cdouble>(count);
+    imp_cap_ = count;
+  }
+  if (!d_imp_flags_) d_imp_flags_ = device_alloc<unsigned long long>(2);
+  d_imp_rows_ = device_alloc<ImportRow>(rows.size());
+  d_imp_mats_ = device_alloc<double>(std::max<size_t>(mats.size(), 1));
+  check(hipMemcpy(d_imp_rows_, rows.data(), rows.size() * sizeof(ImportRow),
+                  hipMemcpyHostToDevice),
+        "upload import bands");
+  if (!mats.empty())
+    check(hipMemcpy(d_imp_mats_, mats.data(), mats.size() * sizeof(double),
+                    hipMemcpyHostToDevice),
+          "upload import operators");
+  static_assert(sizeof(cplx) == sizeof(cdouble), "complex layouts agree");
+  check(hipMemcpy(d_imp_in_, host, count * sizeof(cdouble), hipMemcpyHostToDevice),
+        "stage imported state");
+  check(hipMemsetAsync(d_imp_flags_, 0, 2 * sizeof(unsigned long long), stream_),
+        "clear import flags");
+  ImportArgs a{};
+  a.rows = static_cast<const ImportRow*>(d_imp_rows_);
+  a.mats = d_imp_mats_;
+  a.in = d_imp_in_;
+  a.worst = d_imp_flags_;
+  a.M = M;
+  a.N = N;
+  a.T = T;
+  check(launch_band_check(a, stream_), "band check");
+  unsigned long long bits = 0;
+  check(hipMemcpyAsync(&bits, d_imp_flags_, sizeof(bits), hipMemcpyDeviceToHost, stream_),
+        "read band check");
+  check(hipStreamSynchronize(stream_), "band check");
+  double combined;
+  std::memcpy(&combined, &bits, sizeof(combined));
+  if (!std::isfinite(combined) || combined > tol) {
+    import_release();
     throw std::runtime_error("imported pipeline state contains meaningful off-band angular content");
+  }
+  return combined;
+}
+
+void Plan::import_release() {
+  for (void* p : {d_imp_rows_, static_cast<void*>(d_imp_mats_)})
+    if (p) hipFree(p);
+  d_imp_rows_ = nullptr;
+  d_imp_mats_ = nullptr;
+}
+
+double Plan::import_state(int i, const cplx* host, double tol) {
+  if (so_mode_) return so_import(i, host, tol);
+  const double combined = import_check(i, host, tol);
   // Split: evolved fields (0..2 of first-order modes) go to the modal state
   // plus their off-band residual; everything else is kept as passive data.
-  std::vector<cdouble> modal(static_cast<size_t>(modal_count_));
-  check(hipMemcpy(modal.data(), d_y_, modal.size() * sizeof(cdouble), hipMemcpyDeviceToHost),
-        "read state");
-  in.passive.assign(host, host + nodal_value_count(i));
-  in.residual.assign(static_cast<size_t>(nodal_value_count(i)), cplx(0.0, 0.0));
-  bool any_residual = false;
-  for (int mi = 0; mi < M; ++mi) {
-    const int li = in.line_of_mode[mi];
-    if (li < 0) continue;
-    const LineDesc& L = lines_[li];
-    const BandTransform bt = make_band_transform(-2, L.m, in.params.ell_max_first, T);
-    for (int f = 0; f < 3; ++f)
-      for (int r = 0; r < N; ++r) {
-        const cplx* v = at(mi, f, r);
-        const std::vector<cplx> c = bt.coefficients_of(v);
-        const std::vector<cplx> back = bt.synthesize(c);
-        for (int l = 0; l < L.n_ell; ++l)
-          modal[L.base + static_cast<size_t>(r) * 3 * L.n_ell + f * L.n_ell + l] = {c[l].real(),
-                                                                                     c[l].imag()};
-        const size_t o = ((static_cast<size_t>(mi) * 13 + f) * N + r) * T;
-        for (int j = 0; j < T; ++j) {
-          in.residual[o + j] = v[j] - back[j];
-          if (in.residual[o + j] != cplx(0.0, 0.0)) any_residual = true;
-          in.passive[o + j] = 0.0;
-        }
-      }
+  Instance& in = inst_.at(i);
+  const int M = in.M, N = in.N, T = in.T;
+  const long long count = nodal_value_count(i);
+  ImportArgs a{};
+  a.rows = static_cast<const ImportRow*>(d_imp_rows_);
+  a.mats = d_imp_mats_;
+  a.in = d_imp_in_;
+  a.residual = d_imp_res_;
+  a.modal = d_y_;
+  a.any_residual = reinterpret_cast<int*>(d_imp_flags_ + 1);
+  a.M = M;
+  a.N = N;
+  a.T = T;
+  try {
+    check(hipMemsetAsync(d_imp_res_, 0, count * sizeof(cdouble), stream_), "clear residuals");
+    check(launch_import_modal(a, stream_), "import split");
+    int any = 0;
+    check(hipMemcpyAsync(&any, d_imp_flags_ + 1, sizeof(int), hipMemcpyDeviceToHost, stream_),
+          "read residual flag");
+    check(hipStreamSynchronize(stream_), "import split");
+    in.passive.assign(host, host + count);
+    for (int mi = 0; mi < M; ++mi) {
+      if (in.line_of_mode[mi] < 0) continue;
+      for (int f = 0; f < 3; ++f)
+        std::fill_n(in.passive.begin() + ((static_cast<size_t>(mi) * 13 + f) * N) * T,
+                    static_cast<size_t>(N) * T, cplx(0.0, 0.0));
+    }
+    in.residual.clear();
+    if (any) {
+      in.residual.resize(static_cast<size_t>(count));
+      check(hipMemcpy(in.residual.data(), d_imp_res_, count * sizeof(cdouble),
+                      hipMemcpyDeviceToHost),
+            "read residuals");
+    }
+  } catch (...) {
+    import_release();
+    throw;
   }
-  if (!any_residual) in.residual.clear();
-  check(hipMemcpy(d_y_, modal.data(), modal.size() * sizeof(cdouble), hipMemcpyHostToDevice),
-        "write state");
+  import_release();
   return combined;
 }
 

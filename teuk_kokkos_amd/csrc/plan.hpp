@@ -116,6 +116,13 @@ class Plan {
   void so_step(const std::vector<double>& t, const std::vector<double>& dt);
   std::vector<WaveformRecordC> so_sample_waveform(int i);
   void so_evaluate_rhs(int i, const cplx* d_stage, cplx* d_out);
+  // Band-validated import on the GPU: stages instance i's host state, checks
+  // every theta row against its field's band and returns the largest
+  // relative off-band residual (throws above `tol`, the state untouched).
+  // On return the staged rows and the band operators of the split stay on
+  // the device until import_release().
+  double import_check(int i, const cplx* host, double tol);
+  void import_release();
   void build_lines();
   void build_tiles();
   void upload_tables();
@@ -147,6 +154,14 @@ class Plan {
   int* d_line_mode_ = nullptr;
   cdouble *d_y_ = nullptr, *d_acc_ = nullptr, *d_ya_ = nullptr, *d_yb_ = nullptr,
           *d_k_ = nullptr;
+  // import staging, grown on demand: the instance's rows, their residuals,
+  // the worst-residual bits and the any-residual flag; the band operators of
+  // the import in progress
+  cdouble *d_imp_in_ = nullptr, *d_imp_res_ = nullptr;
+  unsigned long long* d_imp_flags_ = nullptr;
+  long long imp_cap_ = 0;
+  void* d_imp_rows_ = nullptr;
+  double* d_imp_mats_ = nullptr;
 
   bool use_graph_ = false;
   hipGraphExec_t graph_exec_ = nullptr;

@@ -545,40 +545,13 @@ void Plan::so_initialize() {
 }
 
 double Plan::so_import(int i, const cplx* host, double tol) {
-  Instance& in = inst_.at(i);
-  const int M = in.M, N = in.N, T = in.T;
-  static const int spins[13] = {-2, -2, -2, -1, -2, 0, -2, -1, -1, 0, -2, -2, -2};
-  double combined = 0.0;
-  for (int mi = 0; mi < M; ++mi) {
-    const int m = in.registry.modes()[mi];
-    for (int f = 0; f < 13; ++f) {
-      const bool active = f >= 10 ? in.registry.is_target(m) : in.registry.is_parent(m);
-      const int emax = f >= 10 ? in.params.ell_max_second : in.params.ell_max_first;
-      std::unique_ptr<BandTransform> bt;
-      if (active) bt = std::make_unique<BandTransform>(make_band_transform(spins[f], m, emax, T));
-      for (int r = 0; r < N; ++r) {
-        const cplx* v = host + ((static_cast<size_t>(mi) * 13 + f) * N + r) * T;
-        std::vector<cplx> proj(T, cplx(0.0, 0.0));
-        if (active) proj = bt->synthesize(bt->analyze(v));
-        double line2 = 0.0, res2 = 0.0;
-        for (int j = 0; j < T; ++j) {
-          line2 += std::norm(v[j]);
-          res2 += std::norm(v[j] - proj[j]);
-        }
-        combined = std::max(
-            combined, std::sqrt(res2) / std::max(std::sqrt(line2), std::numeric_limits<double>::min()));
-      }
-    }
-  }
-  if (!std::isfinite(tol) || tol < 0.0)
-    throw std::invalid_argument("pipeline band tolerance must be nonnegative");
-  if (!std::isfinite(combined) || combined > tol)
-    throw std::runtime_error("imported pipeline state contains meaningful off-band angular content");
+  const double combined = import_check(i, host, tol);
   const SoInstance& I = so_->inst[i];
-  static_assert(sizeof(cplx) == sizeof(cdouble), "complex layouts agree");
-  check(hipMemcpy(so_->y + I.base, host, nodal_value_count(i) * sizeof(cdouble),
-                  hipMemcpyHostToDevice),
+  check(hipMemcpyAsync(so_->y + I.base, d_imp_in_, nodal_value_count(i) * sizeof(cdouble),
+                       hipMemcpyDeviceToDevice, stream_),
         "write state");
+  check(hipStreamSynchronize(stream_), "import");
+  import_release();
   return combined;
 }
 
