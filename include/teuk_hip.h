@@ -156,6 +156,40 @@ int teuk_set_source_activation(teuk_plan* plan, int instance,
  * controlling normalized value and 1.0 / 0.0 for all_finite. */
 int teuk_get_source_constraint_norms(const teuk_plan* plan, int instance, double* norms14);
 
+/* The sampled diagnostics of the reference driver (solver_driver.hpp:205-242)
+ * for one instance: evaluate_rhs_at_time of the current state at `time`,
+ * then PipelineDiagnostics::sample_pipeline (pipeline_diagnostics.hpp:
+ * 145-305), PipelineTransportEquationDiagnostics::sample
+ * (pipeline_reconstruction_diagnostics.hpp:43-190),
+ * PipelineIndependentReconstructionDiagnostics::sample
+ * (pipeline_independent_reconstruction_diagnostics.hpp:33-98) and the horizon
+ * maxima of HorizonTransverseDiagnostics (horizon_diagnostics.hpp:113-160).
+ * Writes TEUK_DIAG_COUNT doubles in the layout below.  Fails with
+ * TEUK_ERUNTIME and the reference's message when a sampled value is not
+ * finite.  Call between steps: it overwrites the plan's RHS scratch. */
+enum {
+  TEUK_DIAG_STATE_RMS = 0,            /* 13 fields */
+  TEUK_DIAG_STATE_MAX = 13,
+  TEUK_DIAG_RHS_RMS = 26,
+  TEUK_DIAG_RHS_MAX = 39,
+  TEUK_DIAG_FIRST_CONSTRAINT = 52,    /* rms, max of Q - D Psi */
+  TEUK_DIAG_SECOND_CONSTRAINT = 54,
+  TEUK_DIAG_SOURCE = 56,              /* rms, max of source_over_r3 */
+  TEUK_DIAG_FORCING = 58,             /* rms, max of the latched forcing */
+  TEUK_DIAG_INDEPENDENT = 60,         /* rms, max: psi3, psi2, h_ll, combined */
+  TEUK_DIAG_TRANSPORT = 68,           /* rms, max: G, Lambda, H, B, Pi, C, U, combined */
+  TEUK_DIAG_HORIZON_FIRST = 84,       /* max |d^k FirstPsi/dR^k| at R_H, k = 0..4 */
+  TEUK_DIAG_HORIZON_SECOND = 89,      /* the same for SecondPsi */
+  TEUK_DIAG_SOURCE_CONSTRAINT_MAX = 94,
+  TEUK_DIAG_SOURCE_ACTIVE = 95,
+  TEUK_DIAG_ACTIVATION_TIME = 96,
+  TEUK_DIAG_SCRI_FINITE = 97,
+  TEUK_DIAG_HORIZON_FINITE = 98,
+  TEUK_DIAG_ALL_FINITE = 99,
+  TEUK_DIAG_COUNT = 100
+};
+int teuk_sample_diagnostics(teuk_plan* plan, int instance, double time, double* out);
+
 /* ---------------------------------------------------------------------
  * Execution tuning and measurement (no reference counterpart: they replace
  * the reference's SpatialPipelineTiming, spatial_pipeline.hpp:27-38)

@@ -411,7 +411,8 @@ class Pipeline:
         return [int(flags[k]) for k in range(n)]
 
     INTERMEDIATES = {"forcing": (0, 1), "source_over_r3": (1, 1), "inner": (2, 2),
-                     "inner_tangent": (3, 2), "constraints": (4, 3), "constraint_terms": (5, 6)}
+                     "inner_tangent": (3, 2), "constraints": (4, 3), "constraint_terms": (5, 6),
+                     "radial_derivatives": (6, 7), "angular_inputs": (7, 6)}
 
     def intermediate(self, name: str) -> np.ndarray:
         which, slots = self.INTERMEDIATES[name]

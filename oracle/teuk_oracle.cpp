@@ -1129,6 +1129,8 @@ long long oracle_intermediate(const OraclePipeline* p, int which, double* out) {
     case 3: v = &p->proj_t; break;
     case 4: v = &p->constraints; break;
     case 5: v = &p->terms; break;
+    case 6: v = &p->rdr; break;
+    case 7: v = &p->rang; break;
     default: return -1;
   }
   if (out) std::memcpy(static_cast<void*>(out), static_cast<const void*>(v->data()), v->size() * sizeof(cplx));

@@ -64,3 +64,14 @@ def rel_err(a: np.ndarray, b: np.ndarray) -> float:
     """max |a - b| / max |b| (state-wide relative error)."""
     scale = np.abs(b).max()
     return float(np.abs(a - b).max() / (scale if scale > 0 else 1.0))
+
+
+def horizon_weights(d: int, h: float) -> np.ndarray:
+    """Backward 9-point stencil for the d-th radial derivative at the last
+    grid point: the moment equations sum_j w_j (-j)^k = k! delta_kd,
+    k = 0..8 (horizon_diagnostics.hpp:18-79), solved in float64."""
+    import math
+    A = np.array([[float(-j) ** k for j in range(9)] for k in range(9)])
+    b = np.zeros(9)
+    b[d] = math.factorial(d)
+    return np.linalg.solve(A, b) * h ** (-d)

@@ -14,6 +14,7 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libteuk_hip.so")
+DIAG_COUNT = 100  # TEUK_DIAG_COUNT (include/teuk_hip.h)
 
 TEUK_OK, TEUK_EINVAL, TEUK_ERUNTIME = 0, 1, 2
 SCHEMES = {"d4-2": 0, "d8-4": 1, "d10-5": 2}
@@ -73,6 +74,7 @@ SIGNATURES = {
     "teuk_get_source_activation": (_I, [_P, _I, ctypes.POINTER(SourceActivation)]),
     "teuk_set_source_activation": (_I, [_P, _I, ctypes.POINTER(SourceActivation)]),
     "teuk_get_source_constraint_norms": (_I, [_P, _I, ctypes.POINTER(ctypes.c_double)]),
+    "teuk_sample_diagnostics": (_I, [_P, _I, _D, ctypes.POINTER(ctypes.c_double)]),
     "teuk_plan_set_graph": (_I, [_P, _I]),
     "teuk_plan_set_lds_budget": (_I, [_P, _I]),
     "teuk_plan_set_tile_rows": (_I, [_P, _I]),
@@ -332,6 +334,30 @@ class Plan:
         return {"absolute_maximum": fam(0), "normalized_maximum": fam(1),
                 "weighted_rms": fam(2), "normalized_weighted": fam(3),
                 "controlling_normalized": v[12], "all_finite": v[13] > 0.5}
+
+    def sample_diagnostics(self, time: float, instance: int = 0) -> dict:
+        """The reference driver's sampled diagnostics (solver_driver.hpp:
+        205-242) of one instance at `time`: RMS and maximum of every state
+        and RHS field, of the two reduction constraints, source_over_r3 and
+        the forcing, the independent reconstruction constraints and the
+        transport residuals (each a (rms, max) pair), the horizon derivative
+        maxima of FirstPsi and SecondPsi, and the latch."""
+        v = (ctypes.c_double * DIAG_COUNT)()
+        self._check(lib().teuk_sample_diagnostics(self._h, instance, time, v))
+        a = np.array(v[:])
+        pair = lambda k: (a[k], a[k + 1])
+        return {
+            "state_rms": a[0:13], "state_max": a[13:26], "rhs_rms": a[26:39],
+            "rhs_max": a[39:52], "first_constraint": pair(52), "second_constraint": pair(54),
+            "source_over_r3": pair(56), "forcing": pair(58),
+            "psi3_bianchi": pair(60), "psi2_bianchi": pair(62), "hll_reality": pair(64),
+            "independent_combined": pair(66),
+            "transport": {n: pair(68 + 2 * k)
+                          for k, n in enumerate(("G", "Lambda", "H", "B", "Pi", "C", "U"))},
+            "transport_combined": pair(82),
+            "first_horizon": a[84:89], "second_horizon": a[89:94],
+            "source_constraint_max": a[94], "source_active": a[95] > 0.5,
+            "activation_time": a[96], "raw": a}
 
     # ---------------------------------------------------------------- tuning
     def set_graph(self, enable: bool):

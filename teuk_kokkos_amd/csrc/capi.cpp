@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "config.hpp"
+#include "diag_layout.hpp"
 #include "host_math.hpp"
 #include "plan.hpp"
 
@@ -284,6 +285,21 @@ int teuk_get_source_constraint_norms(const teuk_plan* plan, int instance, double
     check_instance(pl, instance);
     if (!norms14) throw std::invalid_argument("null norms output");
     pl.constraint_norms(instance, norms14);
+  });
+}
+
+static_assert(TEUK_DIAG_COUNT == teuk::kDiagCount && TEUK_DIAG_TRANSPORT == teuk::kDiagTransport &&
+                  TEUK_DIAG_HORIZON_SECOND == teuk::kDiagHorizonSecond &&
+                  TEUK_DIAG_ALL_FINITE == teuk::kDiagAllFinite,
+              "diagnostics layout of the C ABI and the plan agree");
+
+int teuk_sample_diagnostics(teuk_plan* plan, int instance, double time, double* out) {
+  std::string dummy;
+  return guarded(plan ? plan->error : dummy, [&] {
+    teuk::Plan& pl = plan_of(plan);
+    check_instance(pl, instance);
+    if (!out) throw std::invalid_argument("null diagnostics output");
+    pl.sample_diagnostics(instance, time, out);
   });
 }
 

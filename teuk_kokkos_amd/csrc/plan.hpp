@@ -98,6 +98,11 @@ class Plan {
   // maximum, normalized maximum, weighted RMS and normalized weighted norm,
   // then the controlling value and the finiteness flag.
   void constraint_norms(int i, double* out14) const;
+  // The reference's sampled diagnostics of instance i (solver_driver.hpp:
+  // 205-242): one RHS evaluation of the current state at `time`, then the
+  // pipeline, independent-constraint, transport-residual and horizon
+  // reports, kDiagCount doubles in the DiagOutput layout (diag_plan.hip).
+  void sample_diagnostics(int i, double time, double* out);
   static SourceActivation initial_activation(const RunParameters& p);
 
  private:

@@ -33,7 +33,7 @@ struct Plan::SoHost {
   cdouble *y = nullptr, *acc = nullptr, *ya = nullptr, *yb = nullptr, *k = nullptr;
   cdouble *tan = nullptr, *dpsi = nullptr, *dpsi_t = nullptr, *rdr = nullptr, *rdr_t = nullptr;
   cdouble *rang = nullptr, *src = nullptr, *cons = nullptr, *terms = nullptr;
-  cdouble *rang_t = nullptr, *ex = nullptr, *raw = nullptr;
+  cdouble *rang_t = nullptr, *ex = nullptr, *raw = nullptr, *rangx = nullptr;
   int proj_lds = 0;
   long long max_source = 0;
   unsigned schemes = 0;  // bit k: some instance uses radial scheme k

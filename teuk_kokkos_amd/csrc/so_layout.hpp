@@ -84,7 +84,8 @@ enum {
   kSlotsEx = 4,      // eth1 B, eth'1 F and their tangents
   kSlotsRaw = 4,     // ordered-pair sums D, D_t, T, T_t of the target modes
   kSlotsCons = 3,    // constraint residuals
-  kSlotsTerms = 6    // constraint left/right terms
+  kSlotsTerms = 6,   // constraint left/right terms
+  kSlotsRangX = 2    // eth1 F, eth2 Pi of the stage chain (constraint passes only)
 };
 enum { kRangEth2C = 0, kRangEthP2CSharp = 1, kRangEth2G = 2, kRangEthP1BSharp = 3 };
 
@@ -116,6 +117,7 @@ struct SoArgs {
   cdouble* src;
   cdouble* cons;
   cdouble* terms;
+  cdouble* rangx;             // written with the constraints
   int want_constraints;
   int first_inst;             // instances [first_inst, first_inst + n_inst)
   int n_inst;

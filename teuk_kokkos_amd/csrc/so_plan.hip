@@ -288,7 +288,8 @@ void Plan::so_build() {
       {&H.tan, kSlotsTan}, {&H.dpsi, 1},          {&H.dpsi_t, 1},
       {&H.rdr, kSlotsRdr}, {&H.rdr_t, kSlotsRdr}, {&H.rang, kSlotsRang},
       {&H.src, kSlotsSrc}, {&H.cons, kSlotsCons}, {&H.terms, kSlotsTerms},
-      {&H.rang_t, kSlotsRang}, {&H.ex, kSlotsEx}, {&H.raw, kSlotsRaw}};
+      {&H.rang_t, kSlotsRang}, {&H.ex, kSlotsEx}, {&H.raw, kSlotsRaw},
+      {&H.rangx, kSlotsRangX}};
   for (const auto& [ptr, slots] : scratch) {
     *ptr = device_alloc<cdouble>(H.planes * slots);
     check(launch_fill_zero(*ptr, H.planes * slots, stream_), "zero scratch");
@@ -315,7 +316,7 @@ void Plan::so_free() {
   void* ptrs[] = {H.d_bg, H.d_inst, H.d_ops,  H.d_nodes, H.d_tables, H.d_tiles[0], H.d_tiles[1],
                   H.d_tiles[2], H.d_active, H.d_dt, H.d_red, H.y, H.acc, H.ya, H.yb, H.k,
                   H.tan, H.dpsi, H.dpsi_t, H.rdr, H.rdr_t, H.rang, H.src, H.cons, H.terms,
-                  H.rang_t, H.ex, H.raw};
+                  H.rang_t, H.ex, H.raw, H.rangx};
   for (void* p : ptrs)
     if (p) hipFree(p);
   so_.reset();
@@ -355,6 +356,7 @@ void Plan::so_rhs(int first, int n, const cdouble* in, cdouble* out, bool constr
   a.src = H.src;
   a.cons = H.cons;
   a.terms = H.terms;
+  a.rangx = H.rangx;
   a.want_constraints = constraints ? 1 : 0;
   a.first_inst = first;
   a.n_inst = n;
