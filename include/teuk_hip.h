@@ -189,6 +189,13 @@ enum {
   TEUK_DIAG_COUNT = 100
 };
 int teuk_sample_diagnostics(teuk_plan* plan, int instance, double time, double* out);
+/* The per-pair source report of the last teuk_sample_diagnostics call on a
+ * second-order instance (solver_driver.hpp:251-280): for every ordered pair
+ * in registry order, 7 doubles: m1, m2, target, RMS and maximum of |D|, RMS
+ * and maximum of |T| over (r, theta).  *count receives the pair count;
+ * out may be null to query it. */
+int teuk_source_pairs(const teuk_plan* plan, int instance, double* out, int capacity,
+                      int* count);
 
 /* ---------------------------------------------------------------------
  * Execution tuning and measurement (no reference counterpart: they replace

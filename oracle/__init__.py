@@ -412,7 +412,8 @@ class Pipeline:
 
     INTERMEDIATES = {"forcing": (0, 1), "source_over_r3": (1, 1), "inner": (2, 2),
                      "inner_tangent": (3, 2), "constraints": (4, 3), "constraint_terms": (5, 6),
-                     "radial_derivatives": (6, 7), "angular_inputs": (7, 6)}
+                     "radial_derivatives": (6, 7), "angular_inputs": (7, 6),
+                     "pair_sums": (8, 2)}
 
     def intermediate(self, name: str) -> np.ndarray:
         which, slots = self.INTERMEDIATES[name]

@@ -1131,6 +1131,7 @@ long long oracle_intermediate(const OraclePipeline* p, int which, double* out) {
     case 5: v = &p->terms; break;
     case 6: v = &p->rdr; break;
     case 7: v = &p->rang; break;
+    case 8: v = &p->inner; break;
     default: return -1;
   }
   if (out) std::memcpy(static_cast<void*>(out), static_cast<const void*>(v->data()), v->size() * sizeof(cplx));

@@ -115,7 +115,8 @@ int oracle_last_stage_sources(const OraclePipeline* p, int* flags);
  * 2 projected inner (D, T), 3 projected inner tangent, 4 constraint
  * residuals (M x 3 x ...), 5 constraint terms (M x 6 x ...), 6 radial D of
  * the stage reconstruction fields G..U (M x 7 x ...), 7 the stage chain's
- * angular inputs Eth1F, Eth2G, Eth2C, Eth2Pi, Eth'1B#, Eth'2C# (M x 6 x ...).
+ * angular inputs Eth1F, Eth2G, Eth2C, Eth2Pi, Eth'1B#, Eth'2C# (M x 6 x ...),
+ * 8 the ordered-pair sums D, T before projection (M x 2 x ...).
  * Returns the
  * complex value count, or -1. */
 long long oracle_intermediate(const OraclePipeline* p, int which, double* out);

@@ -176,3 +176,39 @@ def test_diagnostics_match_oracle(name):
         assert got["transport"]["H"] == (0.0, 0.0) and got["psi3_bianchi"] == (0.0, 0.0)
         Fs = np.abs(state[:, 2]).max()
         assert close(got["transport"]["Lambda"][1], Fs, 1e-12)
+
+
+def test_source_pair_report_matches_oracle_single_pair_targets():
+    """Targets reached by one ordered pair (m = +-4 from (+-2, +-2)) carry
+    that pair's D and T: its report equals the RMS and maximum of the
+    oracle's ordered-pair sums at the target mode (solver_driver.hpp:251-280).
+    Target 0 sums two pairs, whose reports add up to at least its sum."""
+    params = CASES["second_kerr"]
+    dt = params["final_time"] / params["steps"]
+    ref = H.oracle_initialized(params)
+    ref.advance(3, dt)
+    state = ref.get_state()
+    ref.evaluate_rhs(state)
+    sums = ref.intermediate("pair_sums")  # (M, 2, N, T)
+    modes = ref.setup.modes
+    with T.Plan(W.config_text(**dict(params))) as plan:
+        plan.initialize()
+        plan.import_state(state)
+        plan.sample_diagnostics(3 * dt)
+        pairs = plan.source_pairs()
+    assert [(p["m1"], p["m2"], p["target"]) for p in pairs] == \
+        [(-2, -2, -4), (-2, 2, 0), (2, -2, 0), (2, 2, 4)]
+    for p in pairs:
+        v = sums[modes.index(p["target"])]
+        for c, key in enumerate(("D", "T")):
+            a = np.abs(v[c])
+            rms, mx = math.sqrt((a ** 2).mean()), a.max()
+            if p["target"] != 0:
+                assert close(p[key + "_rms"], rms, 1e-11), (p, key)
+                assert close(p[key + "_max"], mx, 1e-11), (p, key)
+            else:
+                assert p[key + "_max"] > 0.0
+    zero = [p for p in pairs if p["target"] == 0]
+    for c, key in enumerate(("D", "T")):
+        a = np.abs(sums[modes.index(0), c]).max()
+        assert zero[0][key + "_max"] + zero[1][key + "_max"] >= a * (1 - 1e-12)

@@ -75,6 +75,7 @@ SIGNATURES = {
     "teuk_set_source_activation": (_I, [_P, _I, ctypes.POINTER(SourceActivation)]),
     "teuk_get_source_constraint_norms": (_I, [_P, _I, ctypes.POINTER(ctypes.c_double)]),
     "teuk_sample_diagnostics": (_I, [_P, _I, _D, ctypes.POINTER(ctypes.c_double)]),
+    "teuk_source_pairs": (_I, [_P, _I, ctypes.POINTER(ctypes.c_double), _I, _IP]),
     "teuk_plan_set_graph": (_I, [_P, _I]),
     "teuk_plan_set_lds_budget": (_I, [_P, _I]),
     "teuk_plan_set_tile_rows": (_I, [_P, _I]),
@@ -358,6 +359,23 @@ class Plan:
             "first_horizon": a[84:89], "second_horizon": a[89:94],
             "source_constraint_max": a[94], "source_active": a[95] > 0.5,
             "activation_time": a[96], "raw": a}
+
+    def source_pairs(self, instance: int = 0) -> list:
+        """Per-pair source report of the last sample_diagnostics call
+        (second-order plans): dicts with m1, m2, target and the RMS and
+        maximum of |D| and |T|."""
+        n = ctypes.c_int()
+        self._check(lib().teuk_source_pairs(self._h, instance, None, 0, ctypes.byref(n)))
+        v = (ctypes.c_double * max(7 * n.value, 1))()
+        self._check(lib().teuk_source_pairs(self._h, instance, v, n.value, ctypes.byref(n)))
+        keys = ("m1", "m2", "target", "D_rms", "D_max", "T_rms", "T_max")
+        out = []
+        for k in range(n.value):
+            rec = dict(zip(keys, v[7 * k:7 * k + 7]))
+            for key in ("m1", "m2", "target"):
+                rec[key] = int(rec[key])
+            out.append(rec)
+        return out
 
     # ---------------------------------------------------------------- tuning
     def set_graph(self, enable: bool):

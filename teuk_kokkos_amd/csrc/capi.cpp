@@ -303,6 +303,23 @@ int teuk_sample_diagnostics(teuk_plan* plan, int instance, double time, double* 
   });
 }
 
+int teuk_source_pairs(const teuk_plan* plan, int instance, double* out, int capacity,
+                      int* count) {
+  auto* p = const_cast<teuk_plan*>(plan);
+  std::string dummy;
+  return guarded(p ? p->error : dummy, [&] {
+    const teuk::Plan& pl = plan_of(p);
+    check_instance(pl, instance);
+    if (!count) throw std::invalid_argument("null pair count");
+    const auto& rep = pl.instance(instance).pair_report;
+    const int n = static_cast<int>(rep.size() / 7);
+    *count = n;
+    if (!out) return;
+    if (capacity < n) throw std::invalid_argument("pair report capacity too small");
+    std::copy(rep.begin(), rep.end(), out);
+  });
+}
+
 int teuk_plan_set_graph(teuk_plan* plan, int enable) {
   std::string dummy;
   return guarded(plan ? plan->error : dummy, [&] { plan_of(plan).set_graph(enable != 0); });

@@ -93,6 +93,7 @@ void Plan::sample_diagnostics(int i, double time, double* out) {
   double* d_out = device_alloc<double>(2 * kDqCount + 1);
   std::vector<double> red(2 * kDqCount + 1);
   std::vector<cplx> tail(static_cast<size_t>(M) * 2 * kHorizonStencil * T);
+  std::vector<cplx> pairs_host;
   auto release = [&] {
     for (void* p : {static_cast<void*>(d_sharp), static_cast<void*>(d_nodes),
                     static_cast<void*>(d_partial), static_cast<void*>(d_pflags),
@@ -108,7 +109,27 @@ void Plan::sample_diagnostics(int i, double time, double* out) {
       SoHost& H = *so_;
       const SoInstance& I = H.inst.at(i);
       so_upload_active();
-      so_rhs(i, 1, H.y, H.k, true);
+      const long long pair_values =
+          static_cast<long long>(in.registry.ordered_pairs().size()) * 2 * N * T;
+      if (pair_values > H.pairs_cap) {
+        if (H.pairs) (void)hipFree(H.pairs);
+        H.pairs = nullptr;
+        H.pairs_cap = 0;
+        H.pairs = device_alloc<cdouble>(pair_values);
+        H.pairs_cap = pair_values;
+      }
+      H.capture_pairs = true;
+      try {
+        so_rhs(i, 1, H.y, H.k, true);
+      } catch (...) {
+        H.capture_pairs = false;
+        throw;
+      }
+      H.capture_pairs = false;
+      pairs_host.resize(static_cast<size_t>(pair_values));
+      check(hipMemcpyAsync(pairs_host.data(), H.pairs, pair_values * sizeof(cdouble),
+                           hipMemcpyDeviceToHost, stream_),
+            "read pair sources");
       state = H.y + I.base;
       rhs = H.k + I.base;
       a.src = H.src + I.pbase * kSlotsSrc;
@@ -261,6 +282,27 @@ void Plan::sample_diagnostics(int i, double time, double* out) {
   out[kDiagSourceConstraintMax] = std::max({norms[0], norms[4], norms[8]});
   out[kDiagSourceActive] = in.activation.active ? 1.0 : 0.0;
   out[kDiagActivationTime] = in.activation.activation_time;
+  // per ordered pair: RMS and maximum of |D| and |T| over (r, theta)
+  // (solver_driver.hpp:251-280)
+  in.pair_report.clear();
+  if (so_mode_) {
+    const auto& pairs = in.registry.ordered_pairs();
+    const size_t plane = static_cast<size_t>(N) * T;
+    const double inverse_points = 1.0 / static_cast<double>(plane);
+    for (size_t k = 0; k < pairs.size(); ++k) {
+      double sq[2] = {0.0, 0.0}, mx2[2] = {0.0, 0.0};
+      for (int c = 0; c < 2; ++c)
+        for (size_t p = 0; p < plane; ++p) {
+          const double v = std::abs(pairs_host[(2 * k + c) * plane + p]);
+          sq[c] += v * v;
+          mx2[c] = std::max(mx2[c], v);
+        }
+      in.pair_report.insert(in.pair_report.end(),
+                            {static_cast<double>(pairs[k].m1), static_cast<double>(pairs[k].m2),
+                             static_cast<double>(pairs[k].target), std::sqrt(sq[0] * inverse_points),
+                             mx2[0], std::sqrt(sq[1] * inverse_points), mx2[1]});
+    }
+  }
   out[kDiagScriFinite] = 1.0;
   out[kDiagHorizonFinite] = 1.0;
   out[kDiagAllFinite] = 1.0;

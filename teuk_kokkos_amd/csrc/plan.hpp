@@ -50,6 +50,9 @@ class Plan {
     // evolved fields; empty when the state was generated in band.
     std::vector<cplx> passive;
     std::vector<cplx> residual;
+    // per ordered pair of the last diagnostics sample: m1, m2, target, then
+    // RMS and maximum of |D| and of |T| over (r, theta)
+    std::vector<double> pair_report;
     SourceActivation activation;
     int scheme_group = 0, tile_first = 0, tile_count = 0;
     explicit Instance(RunParameters p);

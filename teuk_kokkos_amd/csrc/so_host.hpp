@@ -37,6 +37,10 @@ struct Plan::SoHost {
   int proj_lds = 0;
   long long max_source = 0;
   unsigned schemes = 0;  // bit k: some instance uses radial scheme k
+  // per-pair source values of a diagnostics evaluation (one instance)
+  cdouble* pairs = nullptr;
+  long long pairs_cap = 0;
+  bool capture_pairs = false;
 };
 
 }  // namespace teuk

@@ -627,6 +627,11 @@ __global__ __launch_bounds__(kSoThreads) void so_source_kernel(SoArgs a) {
       };
       Jet Dp, Tp;
       pair_source(R, bg, f, d, Dp, Tp);
+      if (a.pairs) {
+        const long long plane = static_cast<long long>(N) * T, p = r * T + t;
+        a.pairs[2 * k * plane + p] = Dp.v;
+        a.pairs[(2 * k + 1) * plane + p] = Tp.v;
+      }
       totD = jadd(totD, Dp);
       totT = jadd(totT, Tp);
     }

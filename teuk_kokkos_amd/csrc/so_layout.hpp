@@ -118,6 +118,8 @@ struct SoArgs {
   cdouble* cons;
   cdouble* terms;
   cdouble* rangx;             // written with the constraints
+  cdouble* pairs;             // single-instance evaluations only, else null: the
+                              // value of D and T of every ordered pair, (pair, 2, N, T)
   int want_constraints;
   int first_inst;             // instances [first_inst, first_inst + n_inst)
   int n_inst;

@@ -316,7 +316,7 @@ void Plan::so_free() {
   void* ptrs[] = {H.d_bg, H.d_inst, H.d_ops,  H.d_nodes, H.d_tables, H.d_tiles[0], H.d_tiles[1],
                   H.d_tiles[2], H.d_active, H.d_dt, H.d_red, H.y, H.acc, H.ya, H.yb, H.k,
                   H.tan, H.dpsi, H.dpsi_t, H.rdr, H.rdr_t, H.rang, H.src, H.cons, H.terms,
-                  H.rang_t, H.ex, H.raw, H.rangx};
+                  H.rang_t, H.ex, H.raw, H.rangx, H.pairs};
   for (void* p : ptrs)
     if (p) hipFree(p);
   so_.reset();
@@ -357,6 +357,7 @@ void Plan::so_rhs(int first, int n, const cdouble* in, cdouble* out, bool constr
   a.cons = H.cons;
   a.terms = H.terms;
   a.rangx = H.rangx;
+  a.pairs = H.capture_pairs && n == 1 ? H.pairs : nullptr;
   a.want_constraints = constraints ? 1 : 0;
   a.first_inst = first;
   a.n_inst = n;

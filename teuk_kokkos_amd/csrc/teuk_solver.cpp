@@ -6,13 +6,18 @@
 //   teuk_solver backend
 //   teuk_solver --help
 //
-// Writes resolved_config.cfg and waveforms.csv into output.directory, and the
+// Writes resolved_config.cfg, diagnostics.csv (also echoed to stdout),
+// source_pairs.csv and waveforms.csv into output.directory at step 0 and every
+// output.diagnostic_every steps (solver_driver.hpp:154-317), then the
 // evolution timing (step loop only, fenced at sampling steps) to stdout.
+#include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <filesystem>
 #include <fstream>
+#include <iomanip>
 #include <iostream>
 #include <sstream>
 #include <string>
@@ -47,6 +52,67 @@ void write_waveforms(std::ofstream& out, teuk_plan* plan) {
     out << buf;
   }
   out.flush();
+}
+
+// surface_gravity (diagnostics.hpp:18-26)
+double surface_gravity(double mass, double spin) {
+  if (mass <= 0.0 || std::abs(spin) > mass) return std::nan("");
+  const double root = std::sqrt(mass * mass - spin * spin);
+  const double rp = mass + root, rm = mass - root;
+  return (rp - rm) / (2.0 * (rp * rp + spin * spin));
+}
+
+const char* kDiagnosticsHeader =
+    "step,time,kappa_time,first_psi_rms,second_psi_rms,"
+    "first_constraint_rms,second_constraint_rms,source_rms,forcing_rms,"
+    "source_active,source_activation_time,source_constraint_abs_max,"
+    "source_constraint_normalized_max,source_constraint_normalized_l2,"
+    "transport_residual_rms,psi3_bianchi_rms,psi2_bianchi_rms,"
+    "hll_reality_rms,first_horizon_d0,first_horizon_d1,first_horizon_d2,"
+    "first_horizon_d3,first_horizon_d4,second_horizon_d0,"
+    "second_horizon_d1,second_horizon_d2,second_horizon_d3,"
+    "second_horizon_d4";
+
+// One diagnostics.csv line and the source_pairs.csv rows of a sample
+// (solver_driver.hpp:205-280).
+void write_diagnostics(std::ofstream& diag, std::ofstream& pairs, teuk_plan* plan,
+                       unsigned long long step, double time, double kappa, bool second) {
+  double d[TEUK_DIAG_COUNT];
+  if (teuk_sample_diagnostics(plan, 0, time, d) != TEUK_OK)
+    throw std::runtime_error(teuk_last_error(plan));
+  double norms[14] = {};
+  if (second && teuk_get_source_constraint_norms(plan, 0, norms) != TEUK_OK)
+    throw std::runtime_error(teuk_last_error(plan));
+  const double normalized_maximum = std::max({norms[1], norms[5], norms[9]});
+  const double normalized_l2 = std::max({norms[3], norms[7], norms[11]});
+  std::ostringstream line;
+  line << std::setprecision(17) << step << ',' << time << ',' << kappa * time << ','
+       << d[TEUK_DIAG_STATE_RMS + 2] << ',' << d[TEUK_DIAG_STATE_RMS + 12] << ','
+       << d[TEUK_DIAG_FIRST_CONSTRAINT] << ',' << d[TEUK_DIAG_SECOND_CONSTRAINT] << ','
+       << d[TEUK_DIAG_SOURCE] << ',' << d[TEUK_DIAG_FORCING] << ','
+       << (d[TEUK_DIAG_SOURCE_ACTIVE] > 0.5 ? 1 : 0) << ',' << d[TEUK_DIAG_ACTIVATION_TIME] << ','
+       << d[TEUK_DIAG_SOURCE_CONSTRAINT_MAX] << ',' << normalized_maximum << ',' << normalized_l2
+       << ',' << d[TEUK_DIAG_TRANSPORT + 14] << ',' << d[TEUK_DIAG_INDEPENDENT] << ','
+       << d[TEUK_DIAG_INDEPENDENT + 2] << ',' << d[TEUK_DIAG_INDEPENDENT + 4];
+  for (int k = 0; k < 5; ++k) line << ',' << d[TEUK_DIAG_HORIZON_FIRST + k];
+  for (int k = 0; k < 5; ++k) line << ',' << d[TEUK_DIAG_HORIZON_SECOND + k];
+  std::cout << line.str() << '\n';
+  diag << line.str() << '\n';
+  diag.flush();
+  int n = 0;
+  if (teuk_source_pairs(plan, 0, nullptr, 0, &n) != TEUK_OK)
+    throw std::runtime_error(teuk_last_error(plan));
+  std::vector<double> rec(static_cast<size_t>(7) * (n > 0 ? n : 1));
+  if (teuk_source_pairs(plan, 0, rec.data(), n, &n) != TEUK_OK)
+    throw std::runtime_error(teuk_last_error(plan));
+  for (int k = 0; k < n; ++k) {
+    const double* r = rec.data() + 7 * k;
+    pairs << std::setprecision(17) << step << ',' << time << ',' << k << ','
+          << static_cast<int>(r[0]) << ',' << static_cast<int>(r[1]) << ','
+          << static_cast<int>(r[2]) << ',' << r[3] << ',' << r[4] << ',' << r[5] << ',' << r[6]
+          << '\n';
+  }
+  pairs.flush();
 }
 
 int run(const std::string& path, const std::vector<std::string>& overrides) {
@@ -92,8 +158,19 @@ int run(const std::string& path, const std::vector<std::string>& overrides) {
   ok(teuk_initialize(plan));
   const int steps = std::stoi(value_of(resolved, "steps"));
   const int every = std::stoi(value_of(resolved, "output.diagnostic_every"));
+  const bool second = value_of(resolved, "second_order.enabled") == "true";
+  const double kappa =
+      surface_gravity(std::stod(value_of(resolved, "mass")), std::stod(value_of(resolved, "spin")));
+  std::ofstream diag(outdir / "diagnostics.csv");
+  std::ofstream pairs(outdir / "source_pairs.csv");
   std::ofstream wf(outdir / "waveforms.csv");
+  if (!diag || !pairs || !wf) throw std::runtime_error("cannot open spatial diagnostic output");
+  pairs << "step,time,pair,m1,m2,target,D_rms,D_max,T_rms,T_max\n";
   wf << "step,time,boundary,order,m,ell,rescaled_re,rescaled_im,endpoint_re,endpoint_im\n";
+  std::cout << kDiagnosticsHeader << '\n';
+  diag << kDiagnosticsHeader << '\n';
+  double time = 0.0;
+  write_diagnostics(diag, pairs, plan, 0, time, kappa, second);
   write_waveforms(wf, plan);
   double evolution = 0.0;
   const auto total_start = std::chrono::steady_clock::now();
@@ -103,6 +180,10 @@ int run(const std::string& path, const std::vector<std::string>& overrides) {
     if (s % every == 0 || s == steps) {
       ok(teuk_sync(plan));
       evolution += std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+      uint64_t done = 0;
+      ok(teuk_plan_progress(plan, 0, &time, &done));
+      write_diagnostics(diag, pairs, plan, static_cast<unsigned long long>(done), time, kappa,
+                        second);
       write_waveforms(wf, plan);
       start = std::chrono::steady_clock::now();
     }
@@ -113,7 +194,8 @@ int run(const std::string& path, const std::vector<std::string>& overrides) {
   const double points = teuk_plan_grid_points(plan) * steps;
   std::cout << "backend=hip gfx950\n"
             << "boundary_policy=zero-SAT; no incoming propagating modes\n"
-            << "source_policy=disabled\n"
+            << "source_policy=" << (second ? value_of(resolved, "second_order.source_mode")
+                                           : std::string("disabled")) << '\n'
             << "evolution_wall_seconds=" << evolution << '\n'
             << "total_wall_seconds=" << total << '\n'
             << "grid_point_steps_per_second=" << points / evolution << '\n';
