@@ -108,9 +108,19 @@ double teuk_plan_modal_points(const teuk_plan* plan);
 /* ---------------------------------------------------------------------
  * State
  * ------------------------------------------------------------------- */
-/* initialize_pipeline_state with Gaussian data for every instance
- * (initial_data_factory.hpp:82-110, pipeline_initial_data.hpp:211-340). */
+/* initialize_pipeline_state for every instance (initial_data_factory.hpp:
+ * 82-110): Gaussian data (pipeline_initial_data.hpp:211-340), or the
+ * checkpoint named by initial_data.checkpoint_directory. */
 int teuk_initialize(teuk_plan* plan);
+/* write_pipeline_checkpoint (pipeline_checkpoint.hpp:782-857): a new
+ * directory with metadata.txt and state.bin (format version 5) holding the
+ * instance's state, latch and progress; published with one rename, an
+ * existing destination is refused. */
+int teuk_checkpoint_write(teuk_plan* plan, int instance, const char* directory);
+/* load_pipeline_checkpoint (pipeline_checkpoint.hpp:869-943): every metadata,
+ * registry, configuration, geometry, size, checksum and band check passes
+ * before the instance's state, latch and progress change. */
+int teuk_checkpoint_load(teuk_plan* plan, int instance, const char* directory);
 /* import_pipeline_state (pipeline_bands.hpp:211-232): host buffer of
  * 2 * modes * 13 * nr * ntheta doubles, rejected with TEUK_ERUNTIME when
  * its off-band relative content exceeds relative_tolerance (reference

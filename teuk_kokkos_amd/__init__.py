@@ -76,6 +76,8 @@ SIGNATURES = {
     "teuk_get_source_constraint_norms": (_I, [_P, _I, ctypes.POINTER(ctypes.c_double)]),
     "teuk_sample_diagnostics": (_I, [_P, _I, _D, ctypes.POINTER(ctypes.c_double)]),
     "teuk_source_pairs": (_I, [_P, _I, ctypes.POINTER(ctypes.c_double), _I, _IP]),
+    "teuk_checkpoint_write": (_I, [_P, _I, ctypes.c_char_p]),
+    "teuk_checkpoint_load": (_I, [_P, _I, ctypes.c_char_p]),
     "teuk_plan_set_graph": (_I, [_P, _I]),
     "teuk_plan_set_lds_budget": (_I, [_P, _I]),
     "teuk_plan_set_tile_rows": (_I, [_P, _I]),
@@ -376,6 +378,15 @@ class Plan:
                 rec[key] = int(rec[key])
             out.append(rec)
         return out
+
+    def write_checkpoint(self, directory: str, instance: int = 0):
+        """Checkpoint format v5 of one instance into a new directory."""
+        self._check(lib().teuk_checkpoint_write(self._h, instance, str(directory).encode()))
+
+    def load_checkpoint(self, directory: str, instance: int = 0):
+        """Load a checkpoint whose registry, configuration and geometry match
+        the instance: its state, latch and progress."""
+        self._check(lib().teuk_checkpoint_load(self._h, instance, str(directory).encode()))
 
     # ---------------------------------------------------------------- tuning
     def set_graph(self, enable: bool):

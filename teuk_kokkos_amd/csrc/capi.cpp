@@ -178,7 +178,27 @@ double teuk_plan_modal_points(const teuk_plan* plan) {
 
 int teuk_initialize(teuk_plan* plan) {
   std::string dummy;
-  return guarded(plan ? plan->error : dummy, [&] { plan_of(plan).initialize_gaussian(); });
+  return guarded(plan ? plan->error : dummy, [&] { plan_of(plan).initialize(); });
+}
+
+int teuk_checkpoint_write(teuk_plan* plan, int instance, const char* directory) {
+  std::string dummy;
+  return guarded(plan ? plan->error : dummy, [&] {
+    teuk::Plan& pl = plan_of(plan);
+    check_instance(pl, instance);
+    if (!directory) throw std::invalid_argument("null checkpoint directory");
+    pl.write_checkpoint(instance, directory);
+  });
+}
+
+int teuk_checkpoint_load(teuk_plan* plan, int instance, const char* directory) {
+  std::string dummy;
+  return guarded(plan ? plan->error : dummy, [&] {
+    teuk::Plan& pl = plan_of(plan);
+    check_instance(pl, instance);
+    if (!directory) throw std::invalid_argument("null checkpoint directory");
+    pl.load_checkpoint(instance, directory);
+  });
 }
 
 int teuk_state_import(teuk_plan* plan, int instance, const double* host,

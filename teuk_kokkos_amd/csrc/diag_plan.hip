@@ -143,15 +143,7 @@ void Plan::sample_diagnostics(int i, double time, double* out) {
       // the nodal state (export) and its reference-layout RHS, staged in the
       // import buffers
       import_release();
-      if (count > imp_cap_) {
-        for (void* p : {static_cast<void*>(d_imp_in_), static_cast<void*>(d_imp_res_)})
-          if (p) (void)hipFree(p);
-        d_imp_in_ = d_imp_res_ = nullptr;
-        imp_cap_ = 0;
-        d_imp_in_ = device_alloc<cdouble>(count);
-        d_imp_res_ = device_alloc<cdouble>(count);
-        imp_cap_ = count;
-      }
+      ensure_staging(count, 0);
       std::vector<cplx> host(static_cast<size_t>(count));
       export_state(i, host.data());
       check(hipMemcpy(d_imp_in_, host.data(), count * sizeof(cdouble), hipMemcpyHostToDevice),

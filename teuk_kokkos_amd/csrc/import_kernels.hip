@@ -1,13 +1,11 @@
 // Band-validated import on the GPU (pipeline_bands.hpp:108-232): one 64-lane
 // workgroup per (mode, field, r) theta row of the imported instance.  The
 // check projects the row onto its field's declared band (S A x, with the
-// spin, m and ell_max of the field) and reduces |x - S A x| / |x| to the
-// largest value over the instance.  The split writes the modal coefficients
-// of the evolved rows into the plan's modal state and keeps their nodal
-// residual x - S c for export.
+// spin, m and ell_max of the field) and records |x| and |x - S A x|; the host
+// reduces them.  The split writes the modal coefficients of the evolved rows
+// into the plan's modal state and keeps their nodal residual x - S c for
+// export.
 #include <hip/hip_runtime.h>
-
-#include <cfloat>
 
 #include "import_layout.hpp"
 
@@ -69,10 +67,8 @@ __global__ __launch_bounds__(kImportThreads) void teuk_band_check_kernel(ImportA
     res2 += __shfl_xor(res2, off);
   }
   if (threadIdx.x == 0) {
-    const double rel = sqrt(res2) / fmax(sqrt(line2), DBL_MIN);
-    // relative residuals are >= 0 (or NaN), so their bit patterns order
-    // like the values; a NaN outranks every finite value
-    atomicMax(a.worst, static_cast<unsigned long long>(__double_as_longlong(rel)));
+    a.row_norms[2 * row] = sqrt(line2);
+    a.row_norms[2 * row + 1] = sqrt(res2);
   }
 }
 

@@ -23,7 +23,7 @@ struct ImportArgs {
   const cdouble* in;       // the instance's nodal state (mode, field, r, theta)
   cdouble* residual;       // evolved rows: x - S c, same layout
   cdouble* modal;          // the plan's modal state
-  unsigned long long* worst;  // bits of the largest relative off-band residual
+  double* row_norms;       // per (mode, field, r) row: |x|, then |x - S A x|
   int* any_residual;
   int M, N, T;
 };
@@ -32,8 +32,7 @@ inline constexpr int kImportThreads = 64;
 // widest band an import handles (LDS of the coefficients)
 inline constexpr int kMaxImportBand = 256;
 
-// Relative off-band residual |x - S A x| / |x| of every (mode, field, r) row
-// into the maximum at `worst`.
+// |x| and the off-band residual |x - S A x| of every (mode, field, r) row.
 hipError_t launch_band_check(const ImportArgs& a, hipStream_t s);
 // Modal coefficients of the evolved rows (fields 0..2 of the modes with a
 // line) into the modal state, and their nodal residuals.

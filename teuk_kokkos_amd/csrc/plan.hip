@@ -131,7 +131,8 @@ Plan::~Plan() {
                   static_cast<void*>(d_y_), static_cast<void*>(d_acc_), static_cast<void*>(d_ya_),
                   static_cast<void*>(d_yb_), static_cast<void*>(d_k_),
                   static_cast<void*>(d_imp_in_), static_cast<void*>(d_imp_res_),
-                  static_cast<void*>(d_imp_flags_), d_imp_rows_,
+                  static_cast<void*>(d_imp_flags_), static_cast<void*>(d_imp_norms_),
+                  d_imp_rows_,
                   static_cast<void*>(d_imp_mats_)})
     if (p) hipFree(p);
   so_free();
@@ -518,11 +519,17 @@ double Plan::first_order_initial(const Instance& in, int mi, std::vector<cplx>& 
   return maxres;
 }
 
+// initialize_pipeline_state (initial_data_factory.hpp:82-110): Gaussian data
+// for every instance, then the checkpoint of each instance whose initial
+// data are a checkpoint directory (state, latch and progress).
+void Plan::initialize() {
+  initialize_gaussian();
+  for (int i = 0; i < instance_count(); ++i)
+    if (inst_[i].params.initial_type == InitialDataKind::Checkpoint)
+      load_checkpoint(i, inst_[i].params.checkpoint_directory);
+}
+
 void Plan::initialize_gaussian() {
-  for (auto& in : inst_) {
-    if (in.params.initial_type != InitialDataKind::Gaussian)
-      throw std::invalid_argument("checkpoint initial data are not supported by this build");
-  }
   if (so_mode_) {
     so_initialize();
   } else {
@@ -670,15 +677,7 @@ double Plan::import_check(int i, const cplx* host, double tol) {
     }
   }
   const long long count = nodal_value_count(i);
-  if (count > imp_cap_) {
-    for (void* p : {static_cast<void*>(d_imp_in_), static_cast<void*>(d_imp_res_)})
-      if (p) hipFree(p);
-    d_imp_in_ = d_imp_res_ = nullptr;
-    imp_cap_ = 0;
-    d_imp_in_ = device_alloc<cdouble>(count);
-    d_imp_res_ = device_alloc<cdouble>(count);
-    imp_cap_ = count;
-  }
+  ensure_staging(count, static_cast<long long>(M) * 13 * N);
   if (!d_imp_flags_) d_imp_flags_ = device_alloc<unsigned long long>(2);
   d_imp_rows_ = device_alloc<ImportRow>(rows.size());
   d_imp_mats_ = device_alloc<double>(std::max<size_t>(mats.size(), 1));
@@ -698,22 +697,61 @@ double Plan::import_check(int i, const cplx* host, double tol) {
   a.rows = static_cast<const ImportRow*>(d_imp_rows_);
   a.mats = d_imp_mats_;
   a.in = d_imp_in_;
-  a.worst = d_imp_flags_;
+  a.row_norms = d_imp_norms_;
   a.M = M;
   a.N = N;
   a.T = T;
   check(launch_band_check(a, stream_), "band check");
-  unsigned long long bits = 0;
-  check(hipMemcpyAsync(&bits, d_imp_flags_, sizeof(bits), hipMemcpyDeviceToHost, stream_),
+  std::vector<double> norms(2 * static_cast<size_t>(M) * 13 * N);
+  check(hipMemcpyAsync(norms.data(), d_imp_norms_, norms.size() * sizeof(double),
+                       hipMemcpyDeviceToHost, stream_),
         "read band check");
   check(hipStreamSynchronize(stream_), "band check");
-  double combined;
-  std::memcpy(&combined, &bits, sizeof(combined));
+  // The largest relative residual over the rows.  A row whose norm is below
+  // the rounding resolution (2^-52) of the largest row of its (mode, field)
+  // carries no meaningful content and counts as zero: such rows are pure
+  // rounding remainders of cancellations (the reconstruction fields at
+  // scri), whose direction, in band or not, is set by the last bits.
+  double combined = 0.0;
+  for (int mf = 0; mf < M * 13 && std::isfinite(combined); ++mf) {
+    const double* rn = norms.data() + 2 * static_cast<size_t>(mf) * N;
+    double top = 0.0;
+    for (int r = 0; r < N; ++r) top = std::max(top, rn[2 * r]);
+    const double floor = top * std::numeric_limits<double>::epsilon();
+    for (int r = 0; r < N; ++r) {
+      const double line = rn[2 * r], res = rn[2 * r + 1];
+      if (!std::isfinite(line) || !std::isfinite(res)) {
+        combined = std::numeric_limits<double>::quiet_NaN();
+        break;
+      }
+      if (line <= floor) continue;
+      combined = std::max(combined, res / std::max(line, std::numeric_limits<double>::min()));
+    }
+  }
   if (!std::isfinite(combined) || combined > tol) {
     import_release();
     throw std::runtime_error("imported pipeline state contains meaningful off-band angular content");
   }
   return combined;
+}
+
+void Plan::ensure_staging(long long values, long long rows) {
+  if (values > imp_cap_) {
+    for (void* p : {static_cast<void*>(d_imp_in_), static_cast<void*>(d_imp_res_)})
+      if (p) (void)hipFree(p);
+    d_imp_in_ = d_imp_res_ = nullptr;
+    imp_cap_ = 0;
+    d_imp_in_ = device_alloc<cdouble>(values);
+    d_imp_res_ = device_alloc<cdouble>(values);
+    imp_cap_ = values;
+  }
+  if (rows > imp_rows_cap_) {
+    if (d_imp_norms_) (void)hipFree(d_imp_norms_);
+    d_imp_norms_ = nullptr;
+    imp_rows_cap_ = 0;
+    d_imp_norms_ = device_alloc<double>(2 * rows);
+    imp_rows_cap_ = rows;
+  }
 }
 
 void Plan::import_release() {

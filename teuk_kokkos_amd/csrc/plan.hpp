@@ -73,7 +73,13 @@ class Plan {
   int max_lds_bytes() const { return max_lds_; }
   hipStream_t stream() const { return stream_; }
 
-  void initialize_gaussian();   // every instance from its configuration
+  void initialize();            // every instance from its configuration
+  void initialize_gaussian();   // Gaussian data for every instance
+  // Checkpoint format v5 of instance i (checkpoint.cpp): write a new
+  // directory, or load one whose registry, configuration and geometry match
+  // the instance (its state, latch and progress).
+  void write_checkpoint(int i, const std::string& directory);
+  void load_checkpoint(int i, const std::string& directory);
   double import_state(int i, const cplx* host, double relative_tolerance);
   void export_state(int i, cplx* host);
   void evaluate_rhs(int i, const cplx* d_stage, cplx* d_out, double t);
@@ -163,11 +169,13 @@ class Plan {
   cdouble *d_y_ = nullptr, *d_acc_ = nullptr, *d_ya_ = nullptr, *d_yb_ = nullptr,
           *d_k_ = nullptr;
   // import staging, grown on demand: the instance's rows, their residuals,
-  // the worst-residual bits and the any-residual flag; the band operators of
-  // the import in progress
+  // per-row norms and the any-residual flag; the band operators of the
+  // import in progress
   cdouble *d_imp_in_ = nullptr, *d_imp_res_ = nullptr;
+  double* d_imp_norms_ = nullptr;
   unsigned long long* d_imp_flags_ = nullptr;
-  long long imp_cap_ = 0;
+  long long imp_cap_ = 0, imp_rows_cap_ = 0;
+  void ensure_staging(long long values, long long rows);
   void* d_imp_rows_ = nullptr;
   double* d_imp_mats_ = nullptr;
 

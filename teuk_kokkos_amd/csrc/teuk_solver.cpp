@@ -7,9 +7,11 @@
 //   teuk_solver --help
 //
 // Writes resolved_config.cfg, diagnostics.csv (also echoed to stdout),
-// source_pairs.csv and waveforms.csv into output.directory at step 0 and every
-// output.diagnostic_every steps (solver_driver.hpp:154-317), then the
-// evolution timing (step loop only, fenced at sampling steps) to stdout.
+// source_pairs.csv and waveforms.csv into output.directory at the start and
+// every output.diagnostic_every steps, checkpoint_<step> directories every
+// output.checkpoint_every steps (solver_driver.hpp:154-317), then the
+// evolution timing (step loop only, fenced at sampling steps) to stdout.  A
+// run whose initial data are a checkpoint resumes its time and step count.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -169,24 +171,39 @@ int run(const std::string& path, const std::vector<std::string>& overrides) {
   wf << "step,time,boundary,order,m,ell,rescaled_re,rescaled_im,endpoint_re,endpoint_im\n";
   std::cout << kDiagnosticsHeader << '\n';
   diag << kDiagnosticsHeader << '\n';
+  const int checkpoint_every = std::stoi(value_of(resolved, "output.checkpoint_every"));
+  // a checkpoint start resumes its time and step count
   double time = 0.0;
-  write_diagnostics(diag, pairs, plan, 0, time, kappa, second);
+  uint64_t completed = 0;
+  ok(teuk_plan_progress(plan, 0, &time, &completed));
+  write_diagnostics(diag, pairs, plan, static_cast<unsigned long long>(completed), time, kappa,
+                    second);
   write_waveforms(wf, plan);
   double evolution = 0.0;
   const auto total_start = std::chrono::steady_clock::now();
   auto start = std::chrono::steady_clock::now();
   for (int s = 1; s <= steps; ++s) {
     ok(teuk_advance(plan, 1));
-    if (s % every == 0 || s == steps) {
+    const bool diagnostic_step = s % every == 0 || s == steps;
+    const bool checkpoint_step = checkpoint_every > 0 && (s % checkpoint_every == 0 || s == steps);
+    if (diagnostic_step || checkpoint_step) {
       ok(teuk_sync(plan));
       evolution += std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
-      uint64_t done = 0;
-      ok(teuk_plan_progress(plan, 0, &time, &done));
+    }
+    uint64_t done = 0;
+    ok(teuk_plan_progress(plan, 0, &time, &done));
+    if (diagnostic_step) {
       write_diagnostics(diag, pairs, plan, static_cast<unsigned long long>(done), time, kappa,
                         second);
       write_waveforms(wf, plan);
-      start = std::chrono::steady_clock::now();
     }
+    if (checkpoint_step) {
+      // pipeline_checkpoint_path (solver_driver.hpp:59-64)
+      std::ostringstream name;
+      name << "checkpoint_" << std::setw(8) << std::setfill('0') << done;
+      ok(teuk_checkpoint_write(plan, 0, (outdir / name.str()).string().c_str()));
+    }
+    if (diagnostic_step || checkpoint_step) start = std::chrono::steady_clock::now();
   }
   ok(teuk_sync(plan));
   const double total =
