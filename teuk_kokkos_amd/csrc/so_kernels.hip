@@ -329,18 +329,24 @@ __global__ __launch_bounds__(NT) void so_recon_kernel(SoArgs a) {
   extern __shared__ __align__(16) double smem[];
   const int inst = a.first_inst + blockIdx.y;
   const SoInstance& I = a.inst[inst];
-  const int r = blockIdx.x;
-  if (r >= I.N) return;
   const int M = I.M, T = I.T, MT = M * T, tid = threadIdx.x;
-  const bool live = tid < MT;
-  const int e = live ? tid : 0;
+  // radial rows of this workgroup: thread tid works on element e of row
+  // r0 + sub; rows past the grid and threads past the rows only take part
+  // in the barriers
+  const int rpw = NT == kReconMaxThreads ? 1 : NT / MT;
+  const int r0 = blockIdx.x * rpw;
+  if (r0 >= I.N) return;
+  const int sub = tid / MT;
+  const int r = r0 + (sub < rpw ? sub : 0);
+  const bool live = sub < rpw && r < I.N;
+  const int e = live ? tid - sub * MT : 0;
   const int mode = e / T, t = e - mode * T;
   const int sm = I.sharp[mode], se = sm * T + t;
   const double R = I.h * r, R2 = R * R, R3 = R2 * R;
   const double mass = I.mass, aspin = I.spin, L = I.length;
   const double* nodes = node_ptr(a, I);
   const double sx = nodes[T + t], cx = nodes[t];
-  cdouble* LX = reinterpret_cast<cdouble*>(smem);
+  cdouble* LX = reinterpret_cast<cdouble*>(smem) + (live ? sub : 0) * kReconLdsRows * MT;
   enum { lF = 0, lG, lB, lPi, lC, ltB, ltC, lRaw };
   auto row = [&](int k) { return LX + k * MT; };
   auto op = [&](int coord, int kind) { return I.op[coord][mode][kind]; };
@@ -854,11 +860,11 @@ hipError_t so_launch_drad(int which, const SoArgs& a, int blocks, unsigned schem
   return hipGetLastError();
 }
 
-hipError_t so_launch_recon(int kind, const SoArgs& a, int rows, int max_mt, int lds,
+hipError_t so_launch_recon(int kind, const SoArgs& a, int groups, int max_mt, int lds,
                            hipStream_t s) {
   if (a.n_inst <= 0) return hipSuccess;
   if (max_mt > kReconMaxThreads) return hipErrorInvalidValue;
-  const dim3 g(rows, a.n_inst);
+  const dim3 g(groups, a.n_inst);
   if (max_mt <= kReconThreads) {
     const dim3 b(kReconThreads);
     if (kind == 0) hipLaunchKernelGGL((so_recon_kernel<0, kReconThreads>), g, b, lds, s, a);

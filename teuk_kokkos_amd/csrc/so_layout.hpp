@@ -25,9 +25,13 @@ inline constexpr int kDradSpan = kDradRows + 2 * kMaxClosureWidth;
 inline int so_drad_blocks(int M, int N, int T) {
   return M * 7 * ((N + kDradRows - 1) / kDradRows) * ((T + kDradCols - 1) / kDradCols);
 }
-inline constexpr int kReconThreads = 128;      // reconstruction workgroup for M * N_theta <= 128
+// Reconstruction workgroups: kReconThreads threads holding
+// kReconThreads / (M * N_theta) radial rows when M * N_theta <= kReconThreads,
+// else one row of up to kReconMaxThreads threads.
+inline constexpr int kReconThreads = 256;
 inline constexpr int kReconMaxThreads = 1024;  // largest M * N_theta a plan accepts
-inline constexpr int kReconLdsRows = 11;       // theta rows of M * N_theta values in its LDS
+inline constexpr int kReconLdsRows = 11;       // theta rows of M * N_theta values per radial row
+inline int recon_rows_per_group(int mt) { return mt <= kReconThreads ? kReconThreads / mt : 1; }
 
 // Angular coordinators of the stage graph (spatial_pipeline.hpp:185-218):
 // (spin, boost, band) over the parent or target modes.
@@ -157,9 +161,10 @@ hipError_t so_launch_linear(int scheme, int kind, const SoArgs& a, int n_tiles, 
 // k set when some instance uses radial scheme k
 hipError_t so_launch_drad(int which, const SoArgs& a, int blocks, unsigned schemes,
                           hipStream_t s);
-// one workgroup per (instance, r) of roundup(max_mt, 64) threads, or 128 for
-// max_mt <= 128; max_mt = largest M * N_theta of the plan
-hipError_t so_launch_recon(int kind, const SoArgs& a, int rows, int max_mt, int lds,
+// workgroups of recon_rows_per_group(M * N_theta) radial rows of one
+// instance: `groups` = the largest count over the plan's instances,
+// max_mt = the largest M * N_theta
+hipError_t so_launch_recon(int kind, const SoArgs& a, int groups, int max_mt, int lds,
                            hipStream_t s);
 hipError_t so_launch_source(const SoArgs& a, long long max_points, hipStream_t s);
 hipError_t so_launch_source_projection(const SoArgs& a, int rows, int lds, hipStream_t s);

@@ -232,8 +232,6 @@ void Plan::so_build() {
                                   std::to_string(kReconMaxThreads));
     H.max_mt = std::max(H.max_mt, static_cast<int>(MT));
     H.max_drad = std::max(H.max_drad, so_drad_blocks(M, N, T));
-    H.rec0_lds = std::max(H.rec0_lds, static_cast<int>(16 * kReconLdsRows * MT));
-    H.rec1_lds = H.rec0_lds;
     H.proj_lds = std::max(H.proj_lds, static_cast<int>(16 * 8LL * I.n_targets * T));
     H.max_source = std::max(H.max_source, static_cast<long long>(I.n_targets) * N * T);
     H.max_n = std::max(H.max_n, N);
@@ -246,6 +244,17 @@ void Plan::so_build() {
     H.max_nt = std::max(H.max_nt, static_cast<long long>(N) * T);
   }
   H.bg_points = bgoff;
+  // reconstruction workgroups: several radial rows each when every row fits
+  // kReconThreads threads, else one row of up to kReconMaxThreads threads
+  H.recon_groups = 0;
+  H.rec0_lds = 0;
+  for (const SoInstance& I : H.inst) {
+    const int MT = I.M * I.T;
+    const int rpw = H.max_mt <= kReconThreads ? recon_rows_per_group(MT) : 1;
+    H.recon_groups = std::max(H.recon_groups, (I.N + rpw - 1) / rpw);
+    H.rec0_lds = std::max(H.rec0_lds, 16 * kReconLdsRows * MT * rpw);
+  }
+  H.rec1_lds = H.rec0_lds;
   for (auto& v : H.inst_tiles) v.resize(instance_count(), {0, 0});
   if (H.rec1_lds > 160 * 1024 || H.rec0_lds > 160 * 1024)
     throw std::invalid_argument(
@@ -379,10 +388,11 @@ void Plan::so_rhs(int first, int n, const cdouble* in, cdouble* out, bool constr
   };
   linear(0);
   check(so_launch_drad(0, a, H.max_drad, H.schemes, stream_), "reconstruction D");
-  check(so_launch_recon(0, a, H.max_n, H.max_mt, H.rec0_lds, stream_), "reconstruction chain");
+  check(so_launch_recon(0, a, H.recon_groups, H.max_mt, H.rec0_lds, stream_),
+        "reconstruction chain");
   linear(1);
   check(so_launch_drad(1, a, H.max_drad, H.schemes, stream_), "tangent reconstruction D");
-  check(so_launch_recon(1, a, H.max_n, H.max_mt, H.rec1_lds, stream_),
+  check(so_launch_recon(1, a, H.recon_groups, H.max_mt, H.rec1_lds, stream_),
         "tangent reconstruction chain");
   check(so_launch_source(a, H.max_source, stream_), "quadratic source");
   check(so_launch_source_projection(a, H.max_n, H.proj_lds, stream_), "source projection");
