@@ -161,7 +161,9 @@ __global__ __launch_bounds__(kSoThreads) void so_linear_kernel(SoArgs a) {
   cdouble* U = reinterpret_cast<cdouble*>(smem + kSchemeTableDoubles);
   cdouble* QE = U + rows * us;
   cdouble* PT = QE + mrows * T;
-  cdouble* OUT = PT + mrows * T;
+  // the unprojected outputs reuse the staged rows once every thread is done
+  // reading them (orows <= rows)
+  cdouble* OUT = U;
   const double* nodes = node_ptr(a, I);
   const double* cos_th = nodes + 2 * T;
   const double* sin_th = nodes + 3 * T;
@@ -199,7 +201,12 @@ __global__ __launch_bounds__(kSoThreads) void so_linear_kernel(SoArgs a) {
   const bool forced = KIND == 2 && I.op[kCoSecond][mode][kOpProject] >= 0;
   const double a_spin = I.spin, L = I.length, L2 = L * L, L4 = L2 * L2;
   const double* cos_x = nodes;
-  for (int e = tid; e < orows * T; e += kSoThreads) {
+  // each thread keeps its (at most kSoLinearOut) outputs in registers
+  cdouble out_p[kSoLinearOut], out_q[kSoLinearOut], out_s[kSoLinearOut];
+#pragma unroll
+  for (int j = 0; j < kSoLinearOut; ++j) {
+    const int e = tid + j * kSoThreads;
+    if (e >= orows * T) break;
     const int rr = e / T, t = e - rr * T, r = tile.t0 + rr;
     const cdouble* u = U + (r - tile.lo) * us;
     const cdouble dpsi = apply_d<SCHEME>(TB, U + 2 * T + t, us, tile.lo, r, N, I.inv_h);
@@ -238,12 +245,22 @@ __global__ __launch_bounds__(kSoThreads) void so_linear_kernel(SoArgs a) {
       qt = cadd(qt, apply_diss<SCHEME>(TB, U + T + t, us, tile.lo, r, N, I.h, I.eps));
       st = cadd(st, apply_diss<SCHEME>(TB, U + 2 * T + t, us, tile.lo, r, N, I.h, I.eps));
     }
-    cdouble* o = OUT + rr * us;
-    o[t] = pt;
-    o[T + t] = qt;
-    o[2 * T + t] = st;
+    out_p[j] = pt;
+    out_q[j] = qt;
+    out_s[j] = st;
     if constexpr (KIND == 0) a.dpsi[sidx(I, 1, mode, 0, r, t)] = dpsi;
     if constexpr (KIND == 1) a.dpsi_t[sidx(I, 1, mode, 0, r, t)] = dpsi;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kSoLinearOut; ++j) {
+    const int e = tid + j * kSoThreads;
+    if (e >= orows * T) break;
+    const int rr = e / T, t = e - rr * T;
+    cdouble* o = OUT + rr * us;
+    o[t] = out_p[j];
+    o[T + t] = out_q[j];
+    o[2 * T + t] = out_s[j];
   }
   __syncthreads();
   const int proj_off = I.op[coord][mode][kOpProject];

@@ -72,11 +72,13 @@ std::array<std::vector<double>, kSoOps> nodal_operators(int spin, int m, int E, 
   return out;
 }
 
-int linear_lds(int rows, int mrows, int orows, int T) {
-  return 8 * kSchemeTableDoubles + 16 * (rows * 3 * T + 2 * mrows * T + orows * 3 * T);
+// staged rows (reused for the unprojected outputs), Q_eff and psi_t rows
+int linear_lds(int rows, int mrows, int T) {
+  return 8 * kSchemeTableDoubles + 16 * (rows * 3 * T + 2 * mrows * T);
 }
 
-constexpr int kSoLdsBudget = 64 * 1024;
+// three linear-triple workgroups per CU (measured best of 40-80 KiB)
+constexpr int kSoLdsBudget = 48 * 1024;
 
 }  // namespace
 
@@ -207,7 +209,10 @@ void Plan::so_build() {
         }
         t.lo = lo;
         t.hi = hi;
-        worst = std::max(worst, linear_lds(hi - lo, t.mhi - t.mlo, len, T));
+        worst = std::max(worst, linear_lds(hi - lo, t.mhi - t.mlo, T));
+        // a thread holds at most kSoLinearOut outputs (one row of up to
+        // kSoLinearOut * kSoThreads nodes always fits, T <= kReconMaxThreads)
+        if (len * T > kSoLinearOut * kSoThreads) worst = std::max(worst, kSoLdsBudget + 1);
         line.push_back(t);
         t0 += len;
       }
