@@ -737,12 +737,19 @@ hipError_t so_launch_background(const SoArgs& a, long long max_points, hipStream
 //   stage 1: acc = y + dt/6 k, Y2 = y + dt/2 k;   stage 2: acc += dt/3 k,
 //   Y3 = y + dt/2 k;   stage 3: acc += dt/3 k, Y4 = y + dt k;
 //   stage 4: y = acc + dt/6 k.   grid.y = instance.
+// (mode, field) blocks outside the field's mode set are skipped: their
+// state is zero in every buffer (initial data, band-validated imports) and
+// their RHS is exactly zero (the projections of inactive modes are zero
+// operators), so the combination would leave them zero bit for bit.
 __global__ __launch_bounds__(kSoThreads) void so_rk_kernel(SoRkArgs a) {
   const SoInstance& I = a.inst[blockIdx.y];
   const double dt = a.dt[blockIdx.y];
-  const long long n = static_cast<long long>(I.M) * 13 * I.N * I.T;
+  const long long plane = static_cast<long long>(I.N) * I.T;
+  const long long n = static_cast<long long>(I.M) * 13 * plane;
   for (long long e = static_cast<long long>(blockIdx.x) * kSoThreads + threadIdx.x; e < n;
        e += static_cast<long long>(gridDim.x) * kSoThreads) {
+    const int mf = static_cast<int>(e / plane), mode = mf / 13, f = mf - mode * 13;
+    if (!((I.field_mask[mode] >> f) & 1)) continue;
     const long long g = I.base + e;
     const cdouble k = a.k[g];
     if (a.stage == 1) {

@@ -67,6 +67,9 @@ struct SoInstance {
   double mass, spin, length, h, inv_h, gamma, eps;
   int modes[kSoMaxModes];
   int sharp[kSoMaxModes];
+  // bit f: field f evolves on the mode (fields 0..9 of parent modes, 10..12
+  // of target modes); the others are zero for good, see so_rk_kernel
+  int field_mask[kSoMaxModes];
   int target_mode[kSoMaxTargets];
   int pair_begin[kSoMaxTargets + 1];
   int pair_m1[kSoMaxPairs], pair_m2[kSoMaxPairs];

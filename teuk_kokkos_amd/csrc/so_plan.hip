@@ -118,6 +118,8 @@ void Plan::so_build() {
     for (int mi = 0; mi < M; ++mi) {
       I.modes[mi] = modes[mi];
       I.sharp[mi] = in.registry.sharp_index(modes[mi]);
+      I.field_mask[mi] = (in.registry.is_parent(modes[mi]) ? 0x3ff : 0) |
+                         (in.registry.is_target(modes[mi]) ? 0x1c00 : 0);
     }
     std::vector<int> targets = in.registry.targets();
     std::sort(targets.begin(), targets.end());
@@ -593,6 +595,11 @@ void Plan::so_evaluate_rhs(int i, const cplx* d_stage, cplx* d_out) {
   so_rhs(i, 1, H.ya, H.k, false);
   check(hipMemcpyAsync(d_out, H.k + I.base, bytes, hipMemcpyDeviceToDevice, stream_),
         "rhs copy");
+  // the RK combination skips the (mode, field) blocks a field does not
+  // evolve on and relies on their being zero in the staging buffers: hand
+  // the instance's staging block back as a copy of its (zero-padded) state
+  check(hipMemcpyAsync(H.ya + I.base, H.y + I.base, bytes, hipMemcpyDeviceToDevice, stream_),
+        "restore staging");
 }
 
 // EndpointWaveformSampler::sample (waveform_output.hpp:79-160) from the
