@@ -15,7 +15,8 @@ struct Plan::SoHost {
   std::vector<double> nodes;  // per instance: x, sqrt(1-x^2), cos(acos x), sin(acos x), w
   std::vector<SoTile> tiles[3];                      // per radial scheme
   std::vector<std::pair<int, int>> inst_tiles[3];    // per instance: (first, count)
-  int lin_lds = 0, rec0_lds = 0, rec1_lds = 0, max_n = 0, max_mt = 0, max_drad = 0;
+  int lin_lds = 0, rec0_lds = 0, rec1_lds = 0, max_n = 0, max_drad = 0;
+  int max_mt = 0;  // largest reconstruction row: parent modes x theta nodes
   int recon_groups = 0;  // reconstruction workgroups per instance (grid.x)
   long long values = 0, planes = 0, max_block = 0, max_plane = 0;
   std::vector<std::array<double, 14>> norms;

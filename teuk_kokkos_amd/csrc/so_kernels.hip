@@ -154,6 +154,9 @@ __global__ __launch_bounds__(kSoThreads) void so_linear_kernel(SoArgs a) {
   const SoTile tile = a.tiles[blockIdx.x];
   const SoInstance& I = a.inst[tile.inst];
   const int N = I.N, T = I.T, mode = tile.mode, tid = threadIdx.x;
+  // a triple that does not evolve on the mode has a zero RHS (its band
+  // projection is the zero operator): the zero-initialized outputs stand
+  if (!((I.field_mask[mode] >> (KIND == 2 ? 10 : 0)) & 1)) return;
   const int rows = tile.hi - tile.lo, mrows = tile.mhi - tile.mlo, orows = tile.t1 - tile.t0;
   const int us = 3 * T;
   const int coord = KIND == 2 ? kCoSecond : kCoFirst;
@@ -295,6 +298,8 @@ __global__ __launch_bounds__(kSoThreads) void so_drad_kernel(SoArgs a, int which
   const int rt = q % n_rt;
   q /= n_rt;
   const int c = q % 7, mode = q / 7;
+  // reconstruction fields of a non-parent mode are zero for good (their D too)
+  if (!((I.field_mask[mode] >> (3 + c)) & 1)) return;
   const int r0 = rt * kDradRows, r1 = min(N, r0 + kDradRows);
   const int t0 = tc * kDradCols, nt = min(kDradCols, T - t0);
   // rows [lo, hi) cover every stencil of rows [r0, r1): interior taps reach
@@ -346,7 +351,11 @@ __global__ __launch_bounds__(NT) void so_recon_kernel(SoArgs a) {
   extern __shared__ __align__(16) double smem[];
   const int inst = a.first_inst + blockIdx.y;
   const SoInstance& I = a.inst[inst];
-  const int M = I.M, T = I.T, MT = M * T, tid = threadIdx.x;
+  // Only the parent modes carry reconstruction fields (the others are zero
+  // for good), so a radial row has MT = n_parents * N_theta elements:
+  // element e is node t of parent pi, its LDS rows are indexed by e and the
+  // sharp partner's by its parent index.
+  const int T = I.T, MT = I.n_parents * T, tid = threadIdx.x;
   // radial rows of this workgroup: thread tid works on element e of row
   // r0 + sub; rows past the grid and threads past the rows only take part
   // in the barriers
@@ -357,8 +366,10 @@ __global__ __launch_bounds__(NT) void so_recon_kernel(SoArgs a) {
   const int r = r0 + (sub < rpw ? sub : 0);
   const bool live = sub < rpw && r < I.N;
   const int e = live ? tid - sub * MT : 0;
-  const int mode = e / T, t = e - mode * T;
-  const int sm = I.sharp[mode], se = sm * T + t;
+  const int pi = e / T, t = e - pi * T;
+  const int mode = I.parent_mode[pi];
+  const int spi = I.parent_sharp[pi];  // -1: the partner's fields are zero
+  const int sm = I.sharp[mode], se = (spi < 0 ? 0 : spi) * T + t;
   const double R = I.h * r, R2 = R * R, R3 = R2 * R;
   const double mass = I.mass, aspin = I.spin, L = I.length;
   const double* nodes = node_ptr(a, I);
@@ -367,16 +378,19 @@ __global__ __launch_bounds__(NT) void so_recon_kernel(SoArgs a) {
   enum { lF = 0, lG, lB, lPi, lC, ltB, ltC, lRaw };
   auto row = [&](int k) { return LX + k * MT; };
   auto op = [&](int coord, int kind) { return I.op[coord][mode][kind]; };
+  const cdouble czero = cmake(0.0, 0.0);
+  // the sharp partner's value of LDS row k
+  auto sharp_of = [&](int k) { return spi < 0 ? czero : cconj(row(k)[se]); };
   auto eth = [&](int coord, bool prime, cdouble xo, cdouble dto, cdouble ang) {
     return eth_point(prime, xo, dto, ang, I.coord_spin[coord], I.coord_boost[coord], R, sx, cx,
                      aspin, L);
   };
-  // eth / eth' of coordinator `coord` at this element: theta row `xr` of the
-  // field (every mode), its own value `xo` and time derivative `dto`
+  // eth / eth' of coordinator `coord` at this element: theta rows `xr` of the
+  // field (every parent mode), its own value `xo` and time derivative `dto`
   auto ghp = [&](int coord, bool prime, const cdouble* xr, cdouble xo, cdouble dto) {
     const int off = op(coord, prime ? kOpLower : kOpRaise);
     if (off < 0) return cmake(0.0, 0.0);
-    return eth(coord, prime, xo, dto, row_op<CH>(a.ops, off, xr + mode * T, t, T));
+    return eth(coord, prime, xo, dto, row_op<CH>(a.ops, off, xr + pi * T, t, T));
   };
   // eth' of the sharp field conj(X[-m]) from the rows of X and its time
   // derivative: the operators are real, so the angular part is the
@@ -384,11 +398,12 @@ __global__ __launch_bounds__(NT) void so_recon_kernel(SoArgs a) {
   auto ghp_sharp = [&](int coord, const cdouble* xr, const cdouble* dr_row) {
     const int off = op(coord, kOpLower);
     if (off < 0) return cmake(0.0, 0.0);
-    const cdouble ang = cconj(row_op<CH>(a.ops, off, xr + sm * T, t, T));
+    if (spi < 0) return czero;  // the partner's rows are zero
+    const cdouble ang = cconj(row_op<CH>(a.ops, off, xr + spi * T, t, T));
     return eth(coord, true, cconj(xr[se]), cconj(dr_row[se]), ang);
   };
   auto project = [&](int coord, const cdouble* in) {
-    return row_op<CH>(a.ops, op(coord, kOpProject), in + mode * T, t, T);
+    return row_op<CH>(a.ops, op(coord, kOpProject), in + pi * T, t, T);
   };
   const cdouble* src = KIND == 0 ? a.in : a.out;
   cdouble* dst = KIND == 0 ? a.out : a.tan;
@@ -494,7 +509,7 @@ __global__ __launch_bounds__(NT) void so_recon_kernel(SoArgs a) {
     const cdouble ep1BS = ghp_sharp(kCoBSharp, row(lB), row(ltB));
     const cdouble ep2CS = ghp_sharp(kCoCSharp, row(lC), row(ltC));
     const cdouble mu0b = cconj(bg.mu0), pi0b = cconj(bg.pi0), tau0b = cconj(bg.tau0);
-    const cdouble pi_s = cconj(row(lPi)[se]), b_s = cconj(row(lB)[se]), c_s = cconj(row(lC)[se]);
+    const cdouble pi_s = sharp_of(lPi), b_s = sharp_of(lB), c_s = sharp_of(lC);
     cdouble du = cmul(cscale(-R, mu0b), xU);
     du = cadd(du, cmul(cscale(-R, bg.mu0), e2C));
     du = cadd(du, cmul(cmul(cscale(-R2, bg.mu0), cadd(pi0b, cscale(2.0, bg.tau0))), xC));

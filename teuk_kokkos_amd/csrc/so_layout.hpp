@@ -28,11 +28,12 @@ inline int so_drad_blocks(int M, int N, int T) {
   return M * 7 * ((N + kDradRows - 1) / kDradRows) * ((T + kDradCols - 1) / kDradCols);
 }
 // Reconstruction workgroups: kReconThreads threads holding
-// kReconThreads / (M * N_theta) radial rows when M * N_theta <= kReconThreads,
+// kReconThreads / (P * N_theta) radial rows when P * N_theta <= kReconThreads (P:
+// the parent modes, the only ones with reconstruction fields),
 // else one row of up to kReconMaxThreads threads.
 inline constexpr int kReconThreads = 256;
-inline constexpr int kReconMaxThreads = 1024;  // largest M * N_theta a plan accepts
-inline constexpr int kReconLdsRows = 11;       // theta rows of M * N_theta values per radial row
+inline constexpr int kReconMaxThreads = 1024;  // largest P * N_theta a plan accepts
+inline constexpr int kReconLdsRows = 11;       // theta rows of P * N_theta values per radial row
 inline int recon_rows_per_group(int mt) { return mt <= kReconThreads ? kReconThreads / mt : 1; }
 
 // Angular coordinators of the stage graph (spatial_pipeline.hpp:185-218):
@@ -70,6 +71,12 @@ struct SoInstance {
   // bit f: field f evolves on the mode (fields 0..9 of parent modes, 10..12
   // of target modes); the others are zero for good, see so_rk_kernel
   int field_mask[kSoMaxModes];
+  // the parent (first-order) modes, whose reconstruction fields evolve: mode
+  // index of each, and the parent index of its sharp partner (-1 when -m is
+  // not a parent, whose fields 0..9 are then zero)
+  int n_parents;
+  int parent_mode[kSoMaxModes];
+  int parent_sharp[kSoMaxModes];
   int target_mode[kSoMaxTargets];
   int pair_begin[kSoMaxTargets + 1];
   int pair_m1[kSoMaxPairs], pair_m2[kSoMaxPairs];
@@ -166,9 +173,9 @@ hipError_t so_launch_linear(int scheme, int kind, const SoArgs& a, int n_tiles, 
 // k set when some instance uses radial scheme k
 hipError_t so_launch_drad(int which, const SoArgs& a, int blocks, unsigned schemes,
                           hipStream_t s);
-// workgroups of recon_rows_per_group(M * N_theta) radial rows of one
+// workgroups of recon_rows_per_group(P * N_theta) radial rows of one
 // instance: `groups` = the largest count over the plan's instances,
-// max_mt = the largest M * N_theta
+// max_mt = the largest P * N_theta
 hipError_t so_launch_recon(int kind, const SoArgs& a, int groups, int max_mt, int lds,
                            hipStream_t s);
 hipError_t so_launch_source(const SoArgs& a, long long max_points, hipStream_t s);

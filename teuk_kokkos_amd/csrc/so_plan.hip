@@ -121,6 +121,14 @@ void Plan::so_build() {
       I.field_mask[mi] = (in.registry.is_parent(modes[mi]) ? 0x3ff : 0) |
                          (in.registry.is_target(modes[mi]) ? 0x1c00 : 0);
     }
+    I.n_parents = 0;
+    for (int mi = 0; mi < M; ++mi)
+      if (in.registry.is_parent(modes[mi])) I.parent_mode[I.n_parents++] = mi;
+    for (int pi = 0; pi < I.n_parents; ++pi) {
+      I.parent_sharp[pi] = -1;
+      for (int pj = 0; pj < I.n_parents; ++pj)
+        if (I.parent_mode[pj] == I.sharp[I.parent_mode[pi]]) I.parent_sharp[pi] = pj;
+    }
     std::vector<int> targets = in.registry.targets();
     std::sort(targets.begin(), targets.end());
     if (static_cast<int>(targets.size()) > kSoMaxTargets ||
@@ -233,9 +241,10 @@ void Plan::so_build() {
     H.inst_tiles[sc].resize(instance_count(), {0, 0});
     H.inst_tiles[sc][i] = {first, static_cast<int>(H.tiles[sc].size()) - first};
     H.schemes |= 1u << sc;
-    const long long MT = static_cast<long long>(M) * T;
+    // reconstruction elements of a radial row: parent modes x theta nodes
+    const long long MT = static_cast<long long>(I.n_parents) * T;
     if (MT > kReconMaxThreads)
-      throw std::invalid_argument("second-order plans need stored modes x ntheta <= " +
+      throw std::invalid_argument("second-order plans need first-order modes x ntheta <= " +
                                   std::to_string(kReconMaxThreads));
     H.max_mt = std::max(H.max_mt, static_cast<int>(MT));
     H.max_drad = std::max(H.max_drad, so_drad_blocks(M, N, T));
@@ -256,7 +265,7 @@ void Plan::so_build() {
   H.recon_groups = 0;
   H.rec0_lds = 0;
   for (const SoInstance& I : H.inst) {
-    const int MT = I.M * I.T;
+    const int MT = I.n_parents * I.T;
     const int rpw = H.max_mt <= kReconThreads ? recon_rows_per_group(MT) : 1;
     H.recon_groups = std::max(H.recon_groups, (I.N + rpw - 1) / rpw);
     H.rec0_lds = std::max(H.rec0_lds, 16 * kReconLdsRows * MT * rpw);
