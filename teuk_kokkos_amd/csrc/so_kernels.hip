@@ -683,40 +683,48 @@ __global__ __launch_bounds__(kSoThreads) void so_source_kernel(SoArgs a) {
 
 // ------------------------------------------------------------ source projection
 // Band projection of D (spin -2) and T (spin -1) with their tangents and
-// eth'_3 T (spatial_pipeline.hpp:1183-1209): one workgroup per (instance, r).
-__global__ __launch_bounds__(64) void so_source_projection_kernel(SoArgs a) {
+// eth'_3 T (spatial_pipeline.hpp:1183-1209): one workgroup per
+// kProjRows radial rows of an instance, so one round of phase latencies
+// serves several rows.
+__global__ __launch_bounds__(kSoThreads) void so_source_projection_kernel(SoArgs a) {
   extern __shared__ __align__(16) double smem[];
   const SoInstance& I = a.inst[a.first_inst + blockIdx.y];
-  const int r = blockIdx.x;
-  if (r >= I.N) return;
+  const int r0 = blockIdx.x * kProjRows;
+  if (r0 >= I.N) return;
   const int T = I.T, NT = I.n_targets, tid = threadIdx.x;
-  const double R = I.h * r;
+  const int per_row = NT * 4 * T;  // raw sums of one row: [targets][4][T]
+  const int rows = min(kProjRows, I.N - r0);
   const double* nodes = node_ptr(a, I);
-  cdouble* RW = reinterpret_cast<cdouble*>(smem);  // [targets][4][T] raw sums
-  cdouble* PRJ = RW + 4 * NT * T;                   // [targets][4][T] projected
-  for (int e = tid; e < NT * 4 * T; e += 64) {
-    const int ts = e / (4 * T), rem = e - ts * 4 * T, c = rem / T, t = rem - c * T;
-    RW[e] = a.raw[sidx(I, kSlotsRaw, I.target_mode[ts], c, r, t)];
+  cdouble* RW = reinterpret_cast<cdouble*>(smem);    // [rows][targets][4][T] raw sums
+  cdouble* PRJ = RW + kProjRows * per_row;            // the same, projected
+  for (int e = tid; e < rows * per_row; e += kSoThreads) {
+    const int rr = e / per_row, q = e - rr * per_row;
+    const int ts = q / (4 * T), rem = q - ts * 4 * T, c = rem / T, t = rem - c * T;
+    RW[e] = a.raw[sidx(I, kSlotsRaw, I.target_mode[ts], c, r0 + rr, t)];
   }
   __syncthreads();
-  for (int e = tid; e < NT * 4 * T; e += 64) {
-    const int ts = e / (4 * T), rem = e - ts * 4 * T, c = rem / T, t = rem - c * T;
+  for (int e = tid; e < rows * per_row; e += kSoThreads) {
+    const int rr = e / per_row, q = e - rr * per_row;
+    const int ts = q / (4 * T), rem = q - ts * 4 * T, c = rem / T, t = rem - c * T;
     const int coord = c < 2 ? kCoSecond : kCoT;
-    PRJ[e] = row_op(a.ops, I.op[coord][I.target_mode[ts]][kOpProject], RW + (ts * 4 + c) * T, t, T);
+    PRJ[e] = row_op(a.ops, I.op[coord][I.target_mode[ts]][kOpProject],
+                    RW + rr * per_row + (ts * 4 + c) * T, t, T);
   }
   __syncthreads();
-  for (int e = tid; e < NT * T; e += 64) {
-    const int ts = e / T, t = e - ts * T;
+  for (int e = tid; e < rows * NT * T; e += kSoThreads) {
+    const int rr = e / (NT * T), q = e - rr * NT * T;
+    const int ts = q / T, t = q - ts * T, r = r0 + rr;
     const int mode = I.target_mode[ts];
     const int off = I.op[kCoT][mode][kOpLower];
-    const cdouble* tv = PRJ + (ts * 4 + 2) * T;
-    const cdouble* tt = PRJ + (ts * 4 + 3) * T;
+    const cdouble* pr = PRJ + rr * per_row + ts * 4 * T;
+    const cdouble* tv = pr + 2 * T;
+    const cdouble* tt = pr + 3 * T;
     cdouble ethT = cmake(0.0, 0.0);
     if (off >= 0)
       ethT = eth_point(true, tv[t], tt[t], row_op(a.ops, off, tv, t, T), I.coord_spin[kCoT],
-                       I.coord_boost[kCoT], R, nodes[T + t], nodes[t], I.spin, I.length);
-    a.src[sidx(I, kSlotsSrc, mode, 0, r, t)] = PRJ[(ts * 4 + 0) * T + t];
-    a.src[sidx(I, kSlotsSrc, mode, 1, r, t)] = PRJ[(ts * 4 + 1) * T + t];
+                       I.coord_boost[kCoT], I.h * r, nodes[T + t], nodes[t], I.spin, I.length);
+    a.src[sidx(I, kSlotsSrc, mode, 0, r, t)] = pr[t];
+    a.src[sidx(I, kSlotsSrc, mode, 1, r, t)] = pr[T + t];
     a.src[sidx(I, kSlotsSrc, mode, 2, r, t)] = tv[t];
     a.src[sidx(I, kSlotsSrc, mode, 3, r, t)] = ethT;
   }
@@ -926,7 +934,8 @@ hipError_t so_launch_source(const SoArgs& a, long long max_points, hipStream_t s
 
 hipError_t so_launch_source_projection(const SoArgs& a, int rows, int lds, hipStream_t s) {
   if (a.n_inst <= 0) return hipSuccess;
-  hipLaunchKernelGGL(so_source_projection_kernel, dim3(rows, a.n_inst), dim3(64), lds, s, a);
+  const dim3 g((rows + kProjRows - 1) / kProjRows, a.n_inst);
+  hipLaunchKernelGGL(so_source_projection_kernel, g, dim3(kSoThreads), lds, s, a);
   return hipGetLastError();
 }
 

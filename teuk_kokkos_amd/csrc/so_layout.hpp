@@ -19,6 +19,8 @@ inline constexpr int kSoMaxPairs = 64;
 inline constexpr int kSoThreads = 256;
 // largest output points (rows x N_theta) of a linear-triple tile per thread
 inline constexpr int kSoLinearOut = 4;
+// radial rows per source-projection workgroup
+inline constexpr int kProjRows = 4;
 // radial D workgroups: kDradRows output rows x kDradCols theta columns, with
 // the stencil reach (at most one closure width each side) staged in LDS
 inline constexpr int kDradRows = 64;

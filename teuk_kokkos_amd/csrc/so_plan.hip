@@ -248,7 +248,7 @@ void Plan::so_build() {
                                   std::to_string(kReconMaxThreads));
     H.max_mt = std::max(H.max_mt, static_cast<int>(MT));
     H.max_drad = std::max(H.max_drad, so_drad_blocks(M, N, T));
-    H.proj_lds = std::max(H.proj_lds, static_cast<int>(16 * 8LL * I.n_targets * T));
+    H.proj_lds = std::max(H.proj_lds, static_cast<int>(16 * 8LL * kProjRows * I.n_targets * T));
     H.max_source = std::max(H.max_source, static_cast<long long>(I.n_targets) * N * T);
     H.max_n = std::max(H.max_n, N);
     const long long block = static_cast<long long>(M) * 13 * N * T;
