@@ -3,7 +3,9 @@
 // per (line, radius) assembles the n_ell x n_ell system
 //   (A diag(1/C_T) S) p = A (C_T^-1 (-2K Q + G_m Psi))
 // in LDS, solves it by Gaussian elimination with partial pivoting, checks
-// the relative residual and writes the modal coefficients A x of P, Q and Psi.
+// the relative residual and writes the modal coefficients A x of P, Q and Psi,
+// or for a nodal state their band projections S A x
+// (pipeline_initial_data.hpp:334-335).
 #include <hip/hip_runtime.h>
 
 #include "init_layout.hpp"
@@ -240,6 +242,33 @@ __global__ __launch_bounds__(kInitThreads) void teuk_galerkin_init(InitArgs a) {
     pn[t] = cmake(re, im);
   }
   __syncthreads();
+  if (a.nodal) {
+    // rhs, b0 and x are free now and contiguous: the 3 n coefficients A v
+    cdouble* coef = rhs;
+    for (int e = tid; e < 3 * n; e += kInitThreads) {
+      const int f = e / n, l = e - f * n;
+      const cdouble* v = f == 0 ? pn : (f == 1 ? q : psi);
+      double re = 0.0, im = 0.0;
+      for (int t = 0; t < T; ++t) {
+        re += A[l * T + t] * v[t].re;
+        im += A[l * T + t] * v[t].im;
+      }
+      coef[e] = cmake(re, im);
+    }
+    __syncthreads();
+    cdouble* out = a.nodal + il.nodal_base + static_cast<long long>(r) * T;
+    const long long field_stride = static_cast<long long>(N) * T;
+    for (int e = tid; e < 3 * T; e += kInitThreads) {
+      const int f = e / T, t = e - f * T;
+      double re = 0.0, im = 0.0;
+      for (int l = 0; l < n; ++l) {
+        re += S[t * n + l] * coef[f * n + l].re;
+        im += S[t * n + l] * coef[f * n + l].im;
+      }
+      out[f * field_stride + t] = cmake(re, im);
+    }
+    return;
+  }
   cdouble* out = a.y + L.base + static_cast<long long>(r) * 3 * n;
   for (int e = tid; e < 3 * n; e += kInitThreads) {
     const int f = e / n, l = e - f * n;

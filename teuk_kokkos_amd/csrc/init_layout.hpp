@@ -14,6 +14,7 @@ struct InitLine {
   int compact;    // compact-support bump instead of a Gaussian
   int scheme;     // radial scheme of Q = D Psi
   double center, width;
+  long long nodal_base;  // nodal output: cdouble offset of the (mode, field 0) block
 };
 
 struct InitArgs {
@@ -24,6 +25,10 @@ struct InitArgs {
   const SchemeTable* tables;
   const cdouble* seeds;
   cdouble* y;                        // modal state
+  // When set, the band-projected fields S A P, S A Q, S A Psi are written
+  // instead as nodal rows (field, r, theta) at nodal + nodal_base, and y is
+  // not touched (second-order nodal state).
+  cdouble* nodal;
   unsigned long long* residual;      // per line: max relative residual (double bits)
   int n_lines;
 };

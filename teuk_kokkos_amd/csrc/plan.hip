@@ -11,7 +11,6 @@
 #include <cstring>
 #include <map>
 #include <stdexcept>
-#include <thread>
 #include <tuple>
 
 namespace teuk {
@@ -451,72 +450,64 @@ double Plan::modal_points_per_step() const {
   return s;
 }
 
-// Gaussian pulse in Psi, Q = D Psi with the selected scheme and the Galerkin
-// P of one first-order mode, nodal (N x T each)
-// (pipeline_initial_data.hpp:142-203, 211-340).  Returns the largest
-// relative residual of the per-radius Galerkin solves.
-double Plan::first_order_initial(const Instance& in, int mi, std::vector<cplx>& pn,
-                                 std::vector<cplx>& q, std::vector<cplx>& psi) {
+// Gaussian pulse geometry and expanded seeds of one instance, checked as
+// in pipeline_initial_data.hpp:211-257.
+std::vector<SeedMode> Plan::gaussian_seeds(const Instance& in, double& center, double& width) {
   const RunParameters& p = in.params;
-  const int m = in.registry.modes()[mi];
-  const int N = in.N, T = in.T;
-  const double center = p.center_fraction * in.horizon;
-  const double width = p.width_fraction * in.horizon;
+  center = p.center_fraction * in.horizon;
+  width = p.width_fraction * in.horizon;
   if (!(width > 0.0) || center < 0.0 || center > in.horizon)
     throw std::invalid_argument("Gaussian center must be on-grid and width must be positive");
-  const auto seeds = expanded_seeds(p);
-  for (const auto& s : seeds) {
+  std::vector<SeedMode> expanded = expanded_seeds(p);
+  for (const auto& s : expanded) {
     if (!in.registry.is_parent(s.m))
       throw std::invalid_argument("Gaussian seed m is not a first-order mode");
     if (s.ell < std::max(2, std::abs(s.m)) || s.ell > p.ell_max_first)
       throw std::invalid_argument("Gaussian seed ell is outside the band");
   }
-  const BandTransform bt = make_band_transform(-2, m, p.ell_max_first, T);
-  const int nl = bt.modes();
-  std::vector<cplx> seed(nl, cplx(0.0, 0.0));
-  for (const auto& s : seeds)
-    if (s.m == m) seed[s.ell - bt.ell_min] += s.amplitude;
-  const std::vector<cplx> ang = bt.synthesize(seed);
-  psi.assign(static_cast<size_t>(N) * T, cplx(0.0, 0.0));
-  q.assign(psi.size(), cplx(0.0, 0.0));
-  pn.assign(psi.size(), cplx(0.0, 0.0));
-  for (int r = 0; r < N; ++r) {
-    const double z = (0.0 + in.h * r - center) / width;
-    double prof;
-    if (!p.compact_support) prof = std::exp(-z * z);
-    else if (std::abs(0.0 + in.h * r - center) >= width) prof = 0.0;
-    else prof = std::exp(1.0 - 1.0 / (1.0 - z * z));
-    for (int j = 0; j < T; ++j) psi[r * T + j] = prof * ang[j];
+  return expanded;
+}
+
+void Plan::run_initial_data(InitArgs a, const std::vector<InitLine>& il,
+                            const std::vector<cdouble>& seeds, int max_nr, int lds) {
+  InitLine* d_il = device_alloc<InitLine>(il.size());
+  cdouble* d_seeds = device_alloc<cdouble>(seeds.size());
+  unsigned long long* d_res = device_alloc<unsigned long long>(il.size());
+  std::vector<unsigned long long> res(il.size(), 0ULL);
+  try {
+    check(hipMemcpy(d_il, il.data(), il.size() * sizeof(InitLine), hipMemcpyHostToDevice),
+          "upload initial data lines");
+    check(hipMemcpy(d_seeds, seeds.data(), seeds.size() * sizeof(cdouble), hipMemcpyHostToDevice),
+          "upload seeds");
+    check(hipMemcpy(d_res, res.data(), res.size() * sizeof(unsigned long long),
+                    hipMemcpyHostToDevice),
+          "clear residuals");
+    a.init = d_il;
+    a.seeds = d_seeds;
+    a.residual = d_res;
+    a.n_lines = static_cast<int>(il.size());
+    check(launch_galerkin_init(a, max_nr, lds, stream_), "Galerkin initial data");
+    check(hipMemcpyAsync(res.data(), d_res, res.size() * sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost, stream_),
+          "read residuals");
+    check(hipStreamSynchronize(stream_), "initialize");
+  } catch (...) {
+    (void)hipFree(d_il);
+    (void)hipFree(d_seeds);
+    (void)hipFree(d_res);
+    throw;
   }
-  for (int j = 0; j < T; ++j)
-    sbp_derivative_line(p.scheme, N, 1.0 / in.h, &psi[j], T, &q[j], T);
-  double maxres = 0.0;
-  std::vector<cplx> mat(static_cast<size_t>(nl) * nl), rhs(nl), sol;
-  for (int r = 0; r < N; ++r) {
-    std::fill(mat.begin(), mat.end(), cplx(0.0, 0.0));
-    std::fill(rhs.begin(), rhs.end(), cplx(0.0, 0.0));
-    const RadialCoefficients rc =
-        radial_coefficients(p.mass, p.spin, p.compactification_length, -2, m, 0.0 + in.h * r);
-    for (int j = 0; j < T; ++j) {
-      const double ct = time_coefficient(rc, p.spin, in.sin_theta[j]);
-      const cdouble g = definition_coefficient(rc, p.spin, -2, in.cos_theta[j]);
-      const cplx inv_time = 1.0 / ct;
-      const cplx nodal =
-          inv_time * (-2.0 * rc.adv * q[r * T + j] + cplx(g.re, g.im) * psi[r * T + j]);
-      for (int a = 0; a < nl; ++a) {
-        const double aa = bt.A[static_cast<size_t>(a) * T + j];
-        rhs[a] += aa * nodal;
-        for (int b = 0; b < nl; ++b)
-          mat[a * nl + b] += aa * inv_time * bt.S[static_cast<size_t>(j) * nl + b];
-      }
-    }
-    if (!dense_complex_solve(nl, mat, rhs, sol))
-      throw std::runtime_error("singular Galerkin initialization matrix");
-    maxres = std::max(maxres, dense_complex_residual(nl, mat, sol, rhs));
-    const std::vector<cplx> pnod = bt.synthesize(sol);
-    for (int j = 0; j < T; ++j) pn[r * T + j] = pnod[j];
+  (void)hipFree(d_il);
+  (void)hipFree(d_seeds);
+  (void)hipFree(d_res);
+  // singular matrix, then the relative residual bar (dense_solve.hpp:27-90)
+  for (unsigned long long bits : res) {
+    double r;
+    std::memcpy(&r, &bits, sizeof(r));
+    if (std::isinf(r)) throw std::runtime_error("singular Galerkin initialization matrix");
+    if (!std::isfinite(r) || r > 2.0e-12)
+      throw std::runtime_error("Galerkin initial-time-derivative solve failed its residual check");
   }
-  return maxres;
 }
 
 // initialize_pipeline_state (initial_data_factory.hpp:82-110): Gaussian data
@@ -541,17 +532,8 @@ void Plan::initialize_gaussian() {
     for (int i = 0; i < instance_count(); ++i) {
       const Instance& in = inst_[i];
       const RunParameters& p = in.params;
-      const double center = p.center_fraction * in.horizon;
-      const double width = p.width_fraction * in.horizon;
-      if (!(width > 0.0) || center < 0.0 || center > in.horizon)
-        throw std::invalid_argument("Gaussian center must be on-grid and width must be positive");
-      const auto expanded = expanded_seeds(p);
-      for (const auto& s : expanded) {
-        if (!in.registry.is_parent(s.m))
-          throw std::invalid_argument("Gaussian seed m is not a first-order mode");
-        if (s.ell < std::max(2, std::abs(s.m)) || s.ell > p.ell_max_first)
-          throw std::invalid_argument("Gaussian seed ell is outside the band");
-      }
+      double center, width;
+      const auto expanded = gaussian_seeds(in, center, width);
       for (int mi = 0; mi < in.M; ++mi) {
         const int li = in.line_of_mode[mi];
         if (li < 0) continue;
@@ -574,51 +556,13 @@ void Plan::initialize_gaussian() {
         max_nr = std::max(max_nr, L.nr);
       }
     }
-    InitLine* d_il = device_alloc<InitLine>(il.size());
-    cdouble* d_seeds = device_alloc<cdouble>(seeds.size());
-    unsigned long long* d_res = device_alloc<unsigned long long>(il.size());
-    std::vector<unsigned long long> res(il.size(), 0ULL);
-    try {
-      check(hipMemcpy(d_il, il.data(), il.size() * sizeof(InitLine), hipMemcpyHostToDevice),
-            "upload initial data lines");
-      check(hipMemcpy(d_seeds, seeds.data(), seeds.size() * sizeof(cdouble),
-                      hipMemcpyHostToDevice),
-            "upload seeds");
-      check(hipMemcpy(d_res, res.data(), res.size() * sizeof(unsigned long long),
-                      hipMemcpyHostToDevice),
-            "clear residuals");
-      InitArgs a{};
-      a.lines = d_lines_;
-      a.init = d_il;
-      a.mats = d_mats_;
-      a.nodes = d_nodes_;
-      a.tables = d_tables_;
-      a.seeds = d_seeds;
-      a.y = d_y_;
-      a.residual = d_res;
-      a.n_lines = static_cast<int>(il.size());
-      check(launch_galerkin_init(a, max_nr, lds, stream_), "Galerkin initial data");
-      check(hipMemcpyAsync(res.data(), d_res, res.size() * sizeof(unsigned long long),
-                           hipMemcpyDeviceToHost, stream_),
-            "read residuals");
-      check(hipStreamSynchronize(stream_), "initialize");
-    } catch (...) {
-      hipFree(d_il);
-      hipFree(d_seeds);
-      hipFree(d_res);
-      throw;
-    }
-    hipFree(d_il);
-    hipFree(d_seeds);
-    hipFree(d_res);
-    for (unsigned long long bits : res) {
-      double r;
-      std::memcpy(&r, &bits, sizeof(r));
-      if (std::isinf(r)) throw std::runtime_error("singular Galerkin initialization matrix");
-      if (!std::isfinite(r) || r > 2.0e-12)
-        throw std::runtime_error(
-            "Galerkin initial-time-derivative solve failed its residual check");
-    }
+    InitArgs a{};
+    a.lines = d_lines_;
+    a.mats = d_mats_;
+    a.nodes = d_nodes_;
+    a.tables = d_tables_;
+    a.y = d_y_;
+    run_initial_data(a, il, seeds, max_nr, lds);
   }
   for (auto& in : inst_) {
     in.passive.clear();

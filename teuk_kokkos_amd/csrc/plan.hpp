@@ -13,6 +13,7 @@
 #include "config.hpp"
 #include "device_layout.hpp"
 #include "host_math.hpp"
+#include "init_layout.hpp"
 
 namespace teuk {
 
@@ -115,8 +116,13 @@ class Plan {
   static SourceActivation initial_activation(const RunParameters& p);
 
  private:
-  static double first_order_initial(const Instance& in, int mi, std::vector<cplx>& pn,
-                                    std::vector<cplx>& q, std::vector<cplx>& psi);
+  // Validated Gaussian geometry and expanded seeds of one instance.
+  static std::vector<SeedMode> gaussian_seeds(const Instance& in, double& center, double& width);
+  // Uploads the per-line initial-data descriptors, launches the Galerkin
+  // initial data with the line, matrix, node and table pointers in `a` and
+  // checks every line's solve residual.
+  void run_initial_data(InitArgs a, const std::vector<InitLine>& il,
+                        const std::vector<cdouble>& seeds, int max_nr, int lds);
   // second-order graph (so_plan.hip)
   void so_build();
   void so_free();

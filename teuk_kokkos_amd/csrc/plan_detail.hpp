@@ -4,10 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <exception>
 #include <stdexcept>
 #include <string>
-#include <thread>
 #include <vector>
 
 #include "config.hpp"
@@ -28,29 +26,6 @@ inline T* device_alloc(size_t count) {
   void* p = nullptr;
   check(hipMalloc(&p, count * sizeof(T)), "hipMalloc");
   return static_cast<T*>(p);
-}
-
-template <class F>
-inline void parallel_for(int n, F&& f) {
-  const int hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-  const int nt = std::min(hw, std::max(1, n));
-  if (nt <= 1) {
-    for (int i = 0; i < n; ++i) f(i);
-    return;
-  }
-  std::vector<std::thread> pool;
-  std::vector<std::exception_ptr> errors(nt);
-  for (int t = 0; t < nt; ++t)
-    pool.emplace_back([&, t] {
-      try {
-        for (int i = t; i < n; i += nt) f(i);
-      } catch (...) {
-        errors[t] = std::current_exception();
-      }
-    });
-  for (auto& th : pool) th.join();
-  for (auto& e : errors)
-    if (e) std::rethrow_exception(e);
 }
 
 inline SchemeTable make_table(RadialScheme s) {

@@ -539,36 +539,99 @@ void Plan::so_step(const std::vector<double>& t, const std::vector<double>& dt) 
   so_upload_active();
 }
 
+// Gaussian data with the Galerkin P (pipeline_initial_data.hpp:142-340) on
+// the GPU: one wavefront per (instance, parent mode, radius) forms Psi,
+// Q = D Psi and P, and writes the band projections S A of the three fields as
+// nodal rows; every other block of the state is zero.
 void Plan::so_initialize() {
   SoHost& H = *so_;
-  std::vector<cdouble> host(static_cast<size_t>(H.values), cdouble{0.0, 0.0});
-  std::vector<std::pair<int, int>> work;
-  for (int i = 0; i < instance_count(); ++i)
-    for (int mi = 0; mi < inst_[i].M; ++mi)
-      if (inst_[i].registry.is_parent(inst_[i].registry.modes()[mi])) work.push_back({i, mi});
-  std::vector<double> residual(work.size(), 0.0);
-  parallel_for(static_cast<int>(work.size()), [&](int w) {
-    const Instance& in = inst_[work[w].first];
-    const int mi = work[w].second, N = in.N, T = in.T;
-    std::vector<cplx> pn, q, psi;
-    residual[w] = first_order_initial(in, mi, pn, q, psi);
-    // final band projection of the evolved fields (pipeline_initial_data.hpp:334-335)
-    const BandTransform bt =
-        make_band_transform(-2, in.registry.modes()[mi], in.params.ell_max_first, T);
-    const SoInstance& I = H.inst[work[w].first];
-    const std::vector<cplx>* fields[3] = {&pn, &q, &psi};
-    for (int f = 0; f < 3; ++f)
-      for (int r = 0; r < N; ++r) {
-        const std::vector<cplx> v = bt.synthesize(bt.analyze(&(*fields[f])[static_cast<size_t>(r) * T]));
-        cdouble* o = host.data() + I.base + ((static_cast<long long>(mi) * 13 + f) * N + r) * T;
-        for (int t = 0; t < T; ++t) o[t] = {v[t].real(), v[t].imag()};
-      }
-  });
-  for (double r : residual)
-    if (!std::isfinite(r) || r > 2.0e-12)
-      throw std::runtime_error("Galerkin initial-time-derivative solve failed its residual check");
-  check(hipMemcpy(H.y, host.data(), host.size() * sizeof(cdouble), hipMemcpyHostToDevice),
-        "upload initial state");
+  std::vector<LineDesc> ld;
+  std::vector<InitLine> il;
+  std::vector<double> mats, nodes;
+  std::vector<cdouble> seeds;
+  int lds = 0, max_nr = 0;
+  for (int i = 0; i < instance_count(); ++i) {
+    const Instance& in = inst_[i];
+    const RunParameters& p = in.params;
+    double center, width;
+    const auto expanded = gaussian_seeds(in, center, width);
+    for (int mi = 0; mi < in.M; ++mi) {
+      const int m = in.registry.modes()[mi];
+      if (!in.registry.is_parent(m)) continue;
+      const BandTransform bt = make_band_transform(-2, m, p.ell_max_first, in.T);
+      LineDesc L{};
+      L.nr = in.N;
+      L.n_ell = bt.modes();
+      L.ntheta = in.T;
+      L.ell_min = bt.ell_min;
+      L.m = m;
+      L.mat_off = static_cast<int>(mats.size());
+      mats.insert(mats.end(), bt.S.begin(), bt.S.end());
+      mats.insert(mats.end(), bt.A.begin(), bt.A.end());
+      L.node_off = static_cast<int>(nodes.size());
+      nodes.insert(nodes.end(), in.sin_theta.begin(), in.sin_theta.end());
+      nodes.insert(nodes.end(), in.cos_theta.begin(), in.cos_theta.end());
+      L.mass = p.mass;
+      L.spin = p.spin;
+      L.length = p.compactification_length;
+      L.h = in.h;
+      L.inv_h = 1.0 / in.h;
+      ld.push_back(L);
+      InitLine d{};
+      d.seed_off = static_cast<int>(seeds.size());
+      d.compact = p.compact_support ? 1 : 0;
+      d.scheme = static_cast<int>(p.scheme);
+      d.center = center;
+      d.width = width;
+      d.nodal_base = H.inst[i].base + static_cast<long long>(mi) * 13 * in.N * in.T;
+      il.push_back(d);
+      seeds.resize(seeds.size() + L.n_ell, cdouble{0.0, 0.0});
+      for (const auto& s : expanded)
+        if (s.m == m) {
+          cdouble& c = seeds[d.seed_off + s.ell - L.ell_min];
+          c.re += s.amplitude.real();
+          c.im += s.amplitude.imag();
+        }
+      lds = std::max(lds, galerkin_init_lds_bytes(L.n_ell, L.ntheta));
+      max_nr = std::max(max_nr, L.nr);
+    }
+  }
+  const SchemeTable tabs[3] = {make_table(RadialScheme::D42), make_table(RadialScheme::D84),
+                               make_table(RadialScheme::D105)};
+  LineDesc* d_ld = nullptr;
+  double* d_mats = nullptr;
+  double* d_nodes = nullptr;
+  SchemeTable* d_tabs = nullptr;
+  auto release = [&] {
+    for (void* q : {static_cast<void*>(d_ld), static_cast<void*>(d_mats),
+                    static_cast<void*>(d_nodes), static_cast<void*>(d_tabs)})
+      if (q) (void)hipFree(q);
+  };
+  try {
+    d_ld = device_alloc<LineDesc>(ld.size());
+    d_mats = device_alloc<double>(mats.size());
+    d_nodes = device_alloc<double>(nodes.size());
+    d_tabs = device_alloc<SchemeTable>(3);
+    check(hipMemcpy(d_ld, ld.data(), ld.size() * sizeof(LineDesc), hipMemcpyHostToDevice),
+          "upload initial data lines");
+    check(hipMemcpy(d_mats, mats.data(), mats.size() * sizeof(double), hipMemcpyHostToDevice),
+          "upload band matrices");
+    check(hipMemcpy(d_nodes, nodes.data(), nodes.size() * sizeof(double), hipMemcpyHostToDevice),
+          "upload nodes");
+    check(hipMemcpy(d_tabs, tabs, sizeof(tabs), hipMemcpyHostToDevice), "upload tables");
+    check(launch_fill_zero(H.y, H.values, stream_), "zero state");
+    InitArgs a{};
+    a.lines = d_ld;
+    a.mats = d_mats;
+    a.nodes = d_nodes;
+    a.tables = d_tabs;
+    a.nodal = H.y;
+    run_initial_data(a, il, seeds, max_nr, lds);
+  } catch (...) {
+    release();
+    throw;
+  }
+  release();
   for (int i = 0; i < instance_count(); ++i) inst_[i].activation = initial_activation(inst_[i].params);
   so_upload_active();
 }
