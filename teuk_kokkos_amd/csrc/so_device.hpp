@@ -14,13 +14,21 @@ __device__ __forceinline__ cdouble cneg(cdouble a) { return {-a.re, -a.im}; }
 __device__ __forceinline__ cdouble cmulr(cdouble a, double s) { return {a.re * s, a.im * s}; }
 
 // Complex quotient by Smith's algorithm (Smith 1962, CACM 5:435).
+// The branch on the larger component of b is taken with selects on swapped
+// operands, so the quotient is branch free (no divergence, and loads can be
+// scheduled across it); each branch's terms round exactly as written out:
+//   |b.re| >= |b.im|: r = b.im/b.re, d = b.re + b.im r,
+//                     ((a.re + a.im r) / d, (a.im - a.re r) / d)
+//   otherwise:        r = b.re/b.im, d = b.re r + b.im,
+//                     ((a.re r + a.im) / d, (a.im r - a.re) / d)
+// (the last term as -((a.re - a.im r) / d), equal under round-to-nearest).
 __device__ __forceinline__ cdouble cdiv(cdouble a, cdouble b) {
-  if (fabs(b.re) >= fabs(b.im)) {
-    const double r = b.im / b.re, d = b.re + b.im * r;
-    return {(a.re + a.im * r) / d, (a.im - a.re * r) / d};
-  }
-  const double r = b.re / b.im, d = b.re * r + b.im;
-  return {(a.re * r + a.im) / d, (a.im * r - a.re) / d};
+  const bool big = fabs(b.re) >= fabs(b.im);
+  const double p = big ? b.re : b.im, q = big ? b.im : b.re;
+  const double x = big ? a.re : a.im, y = big ? a.im : a.re;
+  const double r = q / p, d = p + q * r;
+  const double e = (x + y * r) / d, f = (y - x * r) / d;
+  return {e, big ? f : -f};
 }
 
 // Rescaled rotated-Kinnersley background (background.hpp:29-78).

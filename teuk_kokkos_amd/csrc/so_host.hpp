@@ -37,7 +37,8 @@ struct Plan::SoHost {
   cdouble *rang = nullptr, *src = nullptr, *cons = nullptr, *terms = nullptr;
   cdouble *rang_t = nullptr, *ex = nullptr, *raw = nullptr, *rangx = nullptr;
   int proj_lds = 0;
-  long long max_source = 0;
+  long long max_source = 0;  // largest N x ntheta (points of one target)
+  int max_targets = 0;
   unsigned schemes = 0;  // bit k: some instance uses radial scheme k
   // per-pair source values of a diagnostics evaluation (one instance)
   cdouble* pairs = nullptr;

@@ -91,6 +91,16 @@ __device__ __forceinline__ void pair_source(double R, const Bg& bg, FJ f, DJ d, 
   T = t;
 }
 
+// A value every lane of the wave holds, moved to a scalar register.
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ long long uniform64(long long v) {
+  const unsigned long long u = static_cast<unsigned long long>(v);
+  const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(u)));
+  const unsigned hi =
+      static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(u >> 32)));
+  return static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo);
+}
+
 __device__ __forceinline__ const double* node_ptr(const SoArgs& a, const SoInstance& I) {
   return a.nodes + I.node_off;
 }
@@ -583,33 +593,62 @@ __global__ __launch_bounds__(NT) void so_recon_kernel(SoArgs a) {
 // tangents (source_tangent_spatial.hpp:137-219, second_order.hpp:74-142) from
 // the stage, its RHS and tangent and the derivative inputs in HBM
 // (spatial_pipeline.hpp:934-1181).  grid.y = instance.
-__global__ __launch_bounds__(kSoThreads) void so_source_kernel(SoArgs a) {
+// Workgroup order: runs of eight point blocks per target, the targets of
+// the same eight blocks next to each other (block x -> target (x / 8) %
+// targets, point block 8 (x / (8 targets)) + x % 8).  Workgroups x and x + 8
+// go to the same XCD, so the targets of one point block, which read the same
+// parent values, meet in one L2 at about the same time.
+__global__ __launch_bounds__(kSoThreads) void so_source_kernel(SoArgs a, int max_targets) {
   const SoInstance& I = a.inst[a.first_inst + blockIdx.y];
-  const int N = I.N, T = I.T;
-  const long long total = static_cast<long long>(I.n_targets) * N * T;
+  // The instance record is read through a pointer the compiler cannot prove
+  // unclobbered, so its fields would stay in vector registers; taking them
+  // as wave-uniform makes every plane start a scalar address.
+  const int N = uniform(I.N), T = uniform(I.T);
+  const int run = 8 * max_targets;
+  const int within = static_cast<int>(blockIdx.x) % run;
+  const int ts = within / 8;
+  const long long p0 = (static_cast<long long>(blockIdx.x / run) * 8 + within % 8) * kSoThreads;
+  if (ts >= uniform(I.n_targets) || p0 >= static_cast<long long>(N) * T) return;
   const double mass = I.mass, L = I.length;
+  const long long plane = static_cast<long long>(N) * T;
+  const long long base = uniform64(I.base), pbase = uniform64(I.pbase);
   auto slot_of = [](int k) { return 2 + k; };
   enum { xF = 0, xG, xLa, xH, xB, xPi, xC, xU };
-  for (long long e = static_cast<long long>(blockIdx.x) * kSoThreads + threadIdx.x; e < total;
-       e += static_cast<long long>(gridDim.x) * kSoThreads) {
-    const int t = static_cast<int>(e % T);
-    const int r = static_cast<int>((e / T) % N);
-    const int ts = static_cast<int>(e / (static_cast<long long>(T) * N));
+  {
+    const int pt = static_cast<int>(p0) + static_cast<int>(threadIdx.x);
+    if (pt >= N * T) return;
+    const int r = pt / T, t = pt - r * T;
     const double R = I.h * r;
-    auto S = [&](int mode, int k) { return a.in[fidx(I, mode, slot_of(k), r, t)]; };
-    auto X = [&](int mode, int k) { return a.out[fidx(I, mode, slot_of(k), r, t)]; };
-    auto XT = [&](int mode, int k) { return a.tan[sidx(I, kSlotsTan, mode, slot_of(k), r, t)]; };
-    auto rd = [&](int mode, int c) { return a.rdr[sidx(I, kSlotsRdr, mode, c, r, t)]; };
-    auto rdt = [&](int mode, int c) { return a.rdr_t[sidx(I, kSlotsRdr, mode, c, r, t)]; };
-    auto rg = [&](int mode, int k) { return a.rang[sidx(I, kSlotsRang, mode, k, r, t)]; };
-    auto rgt = [&](int mode, int k) { return a.rang_t[sidx(I, kSlotsRang, mode, k, r, t)]; };
-    auto ex = [&](int mode, int k) { return a.ex[sidx(I, kSlotsEx, mode, k, r, t)]; };
-    const Bg bg = bg_at(a, I, false, r, t);
+    // Every access is a plane start plus the one in-plane byte offset, a
+    // 32-bit value (planes hold fewer than 2^27 points), so the per-thread
+    // part of every address is one register shared by all the loads.
+    const unsigned off = static_cast<unsigned>(pt) * static_cast<unsigned>(sizeof(cdouble));
+    auto at = [&](const cdouble* p) {
+      return *reinterpret_cast<const cdouble*>(reinterpret_cast<const char*>(p) + off);
+    };
+    auto put = [&](cdouble* p, cdouble v) {
+      *reinterpret_cast<cdouble*>(reinterpret_cast<char*>(p) + off) = v;
+    };
+    auto fplane = [&](int mode, int f) { return base + (mode * 13LL + f) * plane; };
+    auto splane = [&](int slots, int mode, int sl) {
+      return pbase * slots + (static_cast<long long>(mode) * slots + sl) * plane;
+    };
+    auto S = [&](int mode, int k) { return at(a.in + fplane(mode, slot_of(k))); };
+    auto X = [&](int mode, int k) { return at(a.out + fplane(mode, slot_of(k))); };
+    auto XT = [&](int mode, int k) { return at(a.tan + splane(kSlotsTan, mode, slot_of(k))); };
+    auto rd = [&](int mode, int c) { return at(a.rdr + splane(kSlotsRdr, mode, c)); };
+    auto rdt = [&](int mode, int c) { return at(a.rdr_t + splane(kSlotsRdr, mode, c)); };
+    auto rg = [&](int mode, int k) { return at(a.rang + splane(kSlotsRang, mode, k)); };
+    auto rgt = [&](int mode, int k) { return at(a.rang_t + splane(kSlotsRang, mode, k)); };
+    auto ex = [&](int mode, int k) { return at(a.ex + splane(kSlotsEx, mode, k)); };
+    const Bg bg = *reinterpret_cast<const Bg*>(reinterpret_cast<const char*>(
+        a.bg + uniform64(I.bgoff) * 2) + static_cast<unsigned>(pt) * static_cast<unsigned>(sizeof(Bg)));
     Jet totD{cmake(0.0, 0.0), cmake(0.0, 0.0)}, totT = totD;
-    for (int k = I.pair_begin[ts]; k < I.pair_begin[ts + 1]; ++k) {
-      const int m1 = I.pair_m1[k], m2 = I.pair_m2[k];
-      const int s1 = I.sharp[m1], s2 = I.sharp[m2];
-      auto fj = [&](int mode, int k2) { return Jet{S(mode, k2), X(mode, k2)}; };
+    const int k_end = uniform(I.pair_begin[ts + 1]);
+    for (int k = uniform(I.pair_begin[ts]); k < k_end; ++k) {
+      const int m1 = uniform(I.pair_m1[k]), m2 = uniform(I.pair_m2[k]);
+      const int s1 = uniform(I.sharp[m1]), s2 = uniform(I.sharp[m2]);
+    auto fj = [&](int mode, int k2) { return Jet{S(mode, k2), X(mode, k2)}; };
       // field jets: value from the stage, tangent from its RHS
       auto f = [&](int i) -> Jet {
         switch (i) {
@@ -644,8 +683,9 @@ __global__ __launch_bounds__(kSoThreads) void so_source_kernel(SoArgs a) {
         switch (i) {
           case 0: {
             const cdouble xf = X(m2, xF);
-            return {delta_n(S(m2, xF), xf, a.dpsi[sidx(I, 1, m2, 0, r, t)], 1, R, mass, L),
-                    delta_n(xf, XT(m2, xF), a.dpsi_t[sidx(I, 1, m2, 0, r, t)], 1, R, mass, L)};
+            return {delta_n(S(m2, xF), xf, at(a.dpsi + splane(1, m2, 0)), 1, R, mass, L),
+                    delta_n(xf, XT(m2, xF), at(a.dpsi_t + splane(1, m2, 0)), 1, R, mass,
+                            L)};
           }
           case 1: return dl(m2, xU, 3, 6);
           case 2: return {rg(m2, kRangEth2C), rgt(m2, kRangEth2C)};
@@ -666,18 +706,17 @@ __global__ __launch_bounds__(kSoThreads) void so_source_kernel(SoArgs a) {
       Jet Dp, Tp;
       pair_source(R, bg, f, d, Dp, Tp);
       if (a.pairs) {
-        const long long plane = static_cast<long long>(N) * T, p = r * T + t;
-        a.pairs[2 * k * plane + p] = Dp.v;
-        a.pairs[(2 * k + 1) * plane + p] = Tp.v;
+        put(a.pairs + 2 * k * plane, Dp.v);
+        put(a.pairs + (2 * k + 1) * plane, Tp.v);
       }
       totD = jadd(totD, Dp);
       totT = jadd(totT, Tp);
     }
-    const int mode = I.target_mode[ts];
-    a.raw[sidx(I, kSlotsRaw, mode, 0, r, t)] = totD.v;
-    a.raw[sidx(I, kSlotsRaw, mode, 1, r, t)] = totD.d;
-    a.raw[sidx(I, kSlotsRaw, mode, 2, r, t)] = totT.v;
-    a.raw[sidx(I, kSlotsRaw, mode, 3, r, t)] = totT.d;
+    const int mode = uniform(I.target_mode[ts]);
+    put(a.raw + splane(kSlotsRaw, mode, 0), totD.v);
+    put(a.raw + splane(kSlotsRaw, mode, 1), totD.d);
+    put(a.raw + splane(kSlotsRaw, mode, 2), totT.v);
+    put(a.raw + splane(kSlotsRaw, mode, 3), totT.d);
   }
 }
 
@@ -924,11 +963,14 @@ hipError_t so_launch_recon(int kind, const SoArgs& a, int groups, int max_mt, in
   return hipGetLastError();
 }
 
-hipError_t so_launch_source(const SoArgs& a, long long max_points, hipStream_t s) {
-  if (a.n_inst <= 0) return hipSuccess;
-  const long long want = (max_points + kSoThreads - 1) / kSoThreads;
-  const dim3 g(static_cast<unsigned>(want < 1 ? 1 : (want > 8192 ? 8192 : want)), a.n_inst);
-  hipLaunchKernelGGL(so_source_kernel, g, dim3(kSoThreads), 0, s, a);
+hipError_t so_launch_source(const SoArgs& a, long long max_points, int max_targets,
+                            hipStream_t s) {
+  if (a.n_inst <= 0 || max_targets <= 0 || max_points <= 0) return hipSuccess;
+  const long long blocks = (max_points + kSoThreads - 1) / kSoThreads;
+  const long long gx = (blocks + 7) / 8 * 8 * max_targets;
+  if (gx > 0x7fffffffLL) return hipErrorInvalidConfiguration;
+  const dim3 g(static_cast<unsigned>(gx), a.n_inst);
+  hipLaunchKernelGGL(so_source_kernel, g, dim3(kSoThreads), 0, s, a, max_targets);
   return hipGetLastError();
 }
 

@@ -180,7 +180,8 @@ hipError_t so_launch_drad(int which, const SoArgs& a, int blocks, unsigned schem
 // max_mt = the largest P * N_theta
 hipError_t so_launch_recon(int kind, const SoArgs& a, int groups, int max_mt, int lds,
                            hipStream_t s);
-hipError_t so_launch_source(const SoArgs& a, long long max_points, hipStream_t s);
+hipError_t so_launch_source(const SoArgs& a, long long max_points, int max_targets,
+                            hipStream_t s);
 hipError_t so_launch_source_projection(const SoArgs& a, int rows, int lds, hipStream_t s);
 hipError_t so_launch_rk(const SoRkArgs& a, hipStream_t s);
 hipError_t so_launch_reduce(const SoReduceArgs& a, hipStream_t s);

@@ -249,7 +249,8 @@ void Plan::so_build() {
     H.max_mt = std::max(H.max_mt, static_cast<int>(MT));
     H.max_drad = std::max(H.max_drad, so_drad_blocks(M, N, T));
     H.proj_lds = std::max(H.proj_lds, static_cast<int>(16 * 8LL * kProjRows * I.n_targets * T));
-    H.max_source = std::max(H.max_source, static_cast<long long>(I.n_targets) * N * T);
+    H.max_source = std::max(H.max_source, static_cast<long long>(N) * T);
+    H.max_targets = std::max(H.max_targets, I.n_targets);
     H.max_n = std::max(H.max_n, N);
     const long long block = static_cast<long long>(M) * 13 * N * T;
     H.max_block = std::max(H.max_block, block);
@@ -410,7 +411,7 @@ void Plan::so_rhs(int first, int n, const cdouble* in, cdouble* out, bool constr
   check(so_launch_drad(1, a, H.max_drad, H.schemes, stream_), "tangent reconstruction D");
   check(so_launch_recon(1, a, H.recon_groups, H.max_mt, H.rec1_lds, stream_),
         "tangent reconstruction chain");
-  check(so_launch_source(a, H.max_source, stream_), "quadratic source");
+  check(so_launch_source(a, H.max_source, H.max_targets, stream_), "quadratic source");
   check(so_launch_source_projection(a, H.max_n, H.proj_lds, stream_), "source projection");
   linear(2);
 }
