@@ -104,6 +104,38 @@ def test_second_order_steps_match_oracle(name):
     assert errs.max() <= TOL, errs
 
 
+INIT_CASES = {
+    "compact_support": so_case(initial_data__compact_support=True, initial_data__width=0.2),
+    "no_sharp_partner": so_case(initial_data__add_sharp_partner=False,
+                                seeds=[(2, 2, 1e-3), (4, -2, 3e-4j)]),
+    "unseeded_parent": so_case(initial_data__add_sharp_partner=False, seeds=[(3, 2, 1e-3)]),
+    "wide_band": so_case(ellmax_first=8, ellmax_second=12, nr=49, spin=0.95,
+                         seeds=[(2, 2, 1e-3), (8, 2, 2e-4), (5, -2, 1e-4 + 3e-4j)]),
+}
+
+
+@pytest.mark.parametrize("name", sorted(INIT_CASES))
+def test_second_order_initial_data_matches_oracle(name):
+    """Gaussian data with the Galerkin P (pipeline_initial_data.hpp:142-340)
+    on the GPU: the band-projected P, Q, Psi of every parent mode at 1e-12 of
+    each field's scale, and every other block exactly zero."""
+    params = INIT_CASES[name]
+    pipe = H.oracle_initialized(params)
+    ref, modes = pipe.get_state(), list(pipe.setup.modes)
+    with _plan(params) as plan:
+        plan.initialize()
+        got = plan.export_state()
+    errs = field_errors(got, ref)
+    print(f"[gpu] {name} initial data per-field rel err max {errs.max():.3e}", flush=True)
+    assert errs[:3].max() <= TOL, errs
+    assert not np.any(got[:, 3:]) and not np.any(ref[:, 3:])
+    for mi, m in enumerate(modes):
+        if m not in params["first_order_modes"]:
+            assert not np.any(got[mi]), m
+    if name == "unseeded_parent":
+        assert not np.any(got[modes.index(-2)])
+
+
 def test_source_event_split_and_latch_match_oracle():
     params = so_case()
     dt = params["final_time"] / params["steps"]
