@@ -13,9 +13,14 @@ Prints one JSON line.
 from __future__ import annotations
 
 import argparse
+import csv
+import glob
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 METRIC = ("grid-point-updates/s, mixed vs FP64 on GPU, at ≤1e-9 waveform err; "
@@ -73,6 +78,63 @@ def cpu_baseline(cases, target_seconds: float):
                       f"{sec:.1f} s"}
 
 
+def _under_profiler() -> bool:
+    """True inside a rocprofv3 run (its tool library is preloaded)."""
+    return ("rocprofiler" in os.environ.get("LD_PRELOAD", "")
+            or any(k.startswith(("ROCPROF", "ROCP_")) for k in os.environ))
+
+
+def _timed_kernel(name: str, second_order: bool) -> bool:
+    if second_order:
+        return "so_" in name and "so_background" not in name and "so_reduce" not in name
+    return "teuk_stage_kernel" in name or "teuk_march_kernel" in name
+
+
+def measure_traffic(workload_name: str, second_order: bool, algorithmic_stage_bytes: float):
+    """HBM bytes per RK stage of the timed kernels: two child runs of one
+    step of the same workload under `rocprofv3 --pmc` (FETCH_SIZE, then
+    WRITE_SIZE, KiB per dispatch), each scaled by what the same counter reads
+    for a 512 MiB device-to-device copy made in the same run.  None when
+    rocprofv3 is absent or a pass yields no usable counters."""
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        return None
+    copy_bytes = 512 << 20
+    total = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        with tempfile.TemporaryDirectory() as d:
+            cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run", "--",
+                   sys.executable, os.path.abspath(__file__), "--workload", workload_name,
+                   "--steps", "1", "--warmup", "0", "--no-cpu", "--no-traffic",
+                   "--calibrate-copy-mib", str(copy_bytes >> 20)]
+            try:
+                # the profiler may exit non-zero after a complete run; the
+                # counter file decides
+                subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                               timeout=240, check=False)
+            except subprocess.TimeoutExpired:
+                return None
+            files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+            if not files:
+                return None
+            per = {}
+            for row in csv.DictReader(open(files[0])):
+                key = (row["Kernel_Name"], row["Dispatch_Id"])
+                per[key] = per.get(key, 0.0) + float(row["Counter_Value"])
+        copies = [v for (k, _), v in per.items() if "copyBuffer" in k]
+        if not copies or max(copies) <= 0:
+            return None
+        scale = copy_bytes / max(copies)  # bytes per counter unit
+        total[counter] = sum(v for (k, _), v in per.items() if _timed_kernel(k, second_order))
+        total[counter] *= scale / 4.0  # one step = four RK stages
+        total[counter + "_scale"] = scale / 1024.0
+    stage = total["FETCH_SIZE"] + total["WRITE_SIZE"]
+    return {"bytes_per_stage": stage, "read": total["FETCH_SIZE"], "written": total["WRITE_SIZE"],
+            "over_algorithmic": stage / algorithmic_stage_bytes,
+            "calibration": {"fetch": total["FETCH_SIZE_scale"],
+                            "write": total["WRITE_SIZE_scale"]}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +149,8 @@ def main() -> int:
                          "default: the plan's)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 --pmc child runs that measure HBM traffic")
     ap.add_argument("--autotune", action="store_true",
                     help="sweep LDS budget x tile rows before timing (cached per workload)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -188,6 +252,14 @@ def main() -> int:
         "cpu_baseline": None,
     }
     plan.close()
+    if roofline is not None and not args.no_traffic and not _under_profiler():
+        algorithmic = (BYTES_PER_GRID_STEP_13 * grid if second_order
+                       else BYTES_PER_MODAL_STEP * modal) / 4.0
+        tr = measure_traffic(args.workload, second_order, algorithmic)
+        if tr is not None:
+            # HBM bytes of one RK stage (all its launches), as `achieved`
+            roofline["traffic"] = tr["bytes_per_stage"]
+            roofline["traffic_detail"] = tr
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cases, args.cpu_seconds)
     print(json.dumps(out))
