@@ -721,33 +721,105 @@ __global__ __launch_bounds__(kSoThreads) void so_source_kernel(SoArgs a, int max
 }
 
 // ------------------------------------------------------------ source projection
+typedef double mfma4 __attribute__((ext_vector_type(4)));
+
+// Out[m][t] = sum_u A[m][u] Op[t][u] for one 16-row block on FP64 MFMA
+// (v_mfma_f64_16x16x4_f64): the rows are the real and imaginary parts of up
+// to eight complex theta rows (row m: complex row m >> 1, part m & 1; the
+// operators are real, so each part is contracted on its own).  Operand
+// lanes: A row lr, u = 4 kb + lg; B column t = 16 tile + lr, u = 4 kb + lg
+// (Op column-major: entry (t, u) at u T + t); accumulator element w: row
+// lg + 4 w, column t = 16 tile + lr.
+template <class AF>
+__device__ __forceinline__ mfma4 row_block_op(const double* op, int T, int tile, int lane,
+                                              AF a_of) {
+  const int lr = lane & 15, lg = lane >> 4;
+  const int tcol = 16 * tile + lr;
+  mfma4 acc = {0.0, 0.0, 0.0, 0.0};
+  // operands of four k blocks (16 u) requested before the products
+  for (int k0 = 0; k0 < T; k0 += 16) {
+    double av[4], bv[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int u = k0 + 4 * q + lg;
+      bv[q] = (u < T && tcol < T) ? op[u * T + tcol] : 0.0;
+      av[q] = u < T ? a_of(lr, u) : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (k0 + 4 * q < T) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q], bv[q], acc, 0, 0, 0);
+  }
+  return acc;
+}
+
 // Band projection of D (spin -2) and T (spin -1) with their tangents and
-// eth'_3 T (spatial_pipeline.hpp:1183-1209): one workgroup per
-// kProjRows radial rows of an instance, so one round of phase latencies
-// serves several rows.
+// eth'_3 T (spatial_pipeline.hpp:1183-1209): one workgroup per kProjRows (4)
+// radial rows of an instance.  Each operator application is a 16-row MFMA
+// block: the projection of target ts and coordinator (D pair or T pair)
+// takes rows (radial row, component of the pair, real/imag), eth'_3 T rows
+// (radial row, real/imag); the blocks go round the four waves.
 __global__ __launch_bounds__(kSoThreads) void so_source_projection_kernel(SoArgs a) {
+  static_assert(kProjRows == 4, "a projection block is 4 rows x 2 components x 2 parts");
   extern __shared__ __align__(16) double smem[];
   const SoInstance& I = a.inst[a.first_inst + blockIdx.y];
   const int r0 = blockIdx.x * kProjRows;
   if (r0 >= I.N) return;
   const int T = I.T, NT = I.n_targets, tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6, lg = lane >> 4, lr = lane & 15;
+  constexpr int kWaves = kSoThreads / 64;
   const int per_row = NT * 4 * T;  // raw sums of one row: [targets][4][T]
   const int rows = min(kProjRows, I.N - r0);
+  const int ttiles = (T + 15) / 16;
   const double* nodes = node_ptr(a, I);
   cdouble* RW = reinterpret_cast<cdouble*>(smem);    // [rows][targets][4][T] raw sums
   cdouble* PRJ = RW + kProjRows * per_row;            // the same, projected
+  cdouble* ANG = PRJ + kProjRows * per_row;           // [rows][targets][T]: angular part of eth'_3 T
   for (int e = tid; e < rows * per_row; e += kSoThreads) {
     const int rr = e / per_row, q = e - rr * per_row;
     const int ts = q / (4 * T), rem = q - ts * 4 * T, c = rem / T, t = rem - c * T;
     RW[e] = a.raw[sidx(I, kSlotsRaw, I.target_mode[ts], c, r0 + rr, t)];
   }
   __syncthreads();
-  for (int e = tid; e < rows * per_row; e += kSoThreads) {
-    const int rr = e / per_row, q = e - rr * per_row;
-    const int ts = q / (4 * T), rem = q - ts * 4 * T, c = rem / T, t = rem - c * T;
-    const int coord = c < 2 ? kCoSecond : kCoT;
-    PRJ[e] = row_op(a.ops, I.op[coord][I.target_mode[ts]][kOpProject],
-                    RW + rr * per_row + (ts * 4 + c) * T, t, T);
+  // projections: block (ts, pair, tile), pair 0 = (D, D_t), 1 = (T, T_t)
+  for (int blk = wave; blk < NT * 2 * ttiles; blk += kWaves) {
+    const int ts = blk / (2 * ttiles), pr = (blk / ttiles) % 2, tile = blk % ttiles;
+    const int off = I.op[pr == 0 ? kCoSecond : kCoT][I.target_mode[ts]][kOpProject];
+    auto a_of = [&](int m, int u) {
+      const int rr = m >> 2, c = 2 * pr + ((m >> 1) & 1);
+      if (rr >= rows) return 0.0;
+      const cdouble v = RW[rr * per_row + (ts * 4 + c) * T + u];
+      return (m & 1) ? v.im : v.re;
+    };
+    const mfma4 acc = off < 0 ? mfma4{0.0, 0.0, 0.0, 0.0}
+                              : row_block_op(a.ops + off, T, tile, lane, a_of);
+    const int t = 16 * tile + lr;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      const int m = lg + 4 * w, rr = m >> 2, c = 2 * pr + ((m >> 1) & 1);
+      if (rr < rows && t < T)
+        reinterpret_cast<double*>(PRJ + rr * per_row + (ts * 4 + c) * T + t)[m & 1] = acc[w];
+    }
+  }
+  __syncthreads();
+  // eth'_3 T: block (ts, tile) over rows (radial row, real/imag), 8 of 16 used
+  for (int blk = wave; blk < NT * ttiles; blk += kWaves) {
+    const int ts = blk / ttiles, tile = blk % ttiles;
+    const int off = I.op[kCoT][I.target_mode[ts]][kOpLower];
+    if (off < 0) continue;
+    auto a_of = [&](int m, int u) {
+      const int rr = m >> 1;
+      if (rr >= rows) return 0.0;
+      const cdouble v = PRJ[rr * per_row + (ts * 4 + 2) * T + u];
+      return (m & 1) ? v.im : v.re;
+    };
+    const mfma4 acc = row_block_op(a.ops + off, T, tile, lane, a_of);
+    const int t = 16 * tile + lr;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      const int m = lg + 4 * w, rr = m >> 1;
+      if (rr < rows && t < T)
+        reinterpret_cast<double*>(ANG + (rr * NT + ts) * T + t)[m & 1] = acc[w];
+    }
   }
   __syncthreads();
   for (int e = tid; e < rows * NT * T; e += kSoThreads) {
@@ -760,7 +832,7 @@ __global__ __launch_bounds__(kSoThreads) void so_source_projection_kernel(SoArgs
     const cdouble* tt = pr + 3 * T;
     cdouble ethT = cmake(0.0, 0.0);
     if (off >= 0)
-      ethT = eth_point(true, tv[t], tt[t], row_op(a.ops, off, tv, t, T), I.coord_spin[kCoT],
+      ethT = eth_point(true, tv[t], tt[t], ANG[(rr * NT + ts) * T + t], I.coord_spin[kCoT],
                        I.coord_boost[kCoT], I.h * r, nodes[T + t], nodes[t], I.spin, I.length);
     a.src[sidx(I, kSlotsSrc, mode, 0, r, t)] = pr[t];
     a.src[sidx(I, kSlotsSrc, mode, 1, r, t)] = pr[T + t];
@@ -976,6 +1048,12 @@ hipError_t so_launch_source(const SoArgs& a, long long max_points, int max_targe
 
 hipError_t so_launch_source_projection(const SoArgs& a, int rows, int lds, hipStream_t s) {
   if (a.n_inst <= 0) return hipSuccess;
+  if (lds > 64 * 1024) {
+    const hipError_t e =
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&so_source_projection_kernel),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    if (e != hipSuccess) return e;
+  }
   const dim3 g((rows + kProjRows - 1) / kProjRows, a.n_inst);
   hipLaunchKernelGGL(so_source_projection_kernel, g, dim3(kSoThreads), lds, s, a);
   return hipGetLastError();

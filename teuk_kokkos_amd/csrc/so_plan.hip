@@ -248,7 +248,9 @@ void Plan::so_build() {
                                   std::to_string(kReconMaxThreads));
     H.max_mt = std::max(H.max_mt, static_cast<int>(MT));
     H.max_drad = std::max(H.max_drad, so_drad_blocks(M, N, T));
-    H.proj_lds = std::max(H.proj_lds, static_cast<int>(16 * 8LL * kProjRows * I.n_targets * T));
+    // raw sums and their projections (4 x T per target and row), and the
+    // angular part of eth'_3 T (T per target and row)
+    H.proj_lds = std::max(H.proj_lds, static_cast<int>(16 * 9LL * kProjRows * I.n_targets * T));
     H.max_source = std::max(H.max_source, static_cast<long long>(N) * T);
     H.max_targets = std::max(H.max_targets, I.n_targets);
     H.max_n = std::max(H.max_n, N);
