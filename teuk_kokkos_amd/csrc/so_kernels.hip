@@ -151,6 +151,37 @@ __device__ __forceinline__ Bg bg_at(const SoArgs& a, const SoInstance& I, bool t
 
 }  // namespace
 
+typedef double mfma4 __attribute__((ext_vector_type(4)));
+
+// Out[m][t] = sum_u A[m][u] Op[t][u] for one 16-row block on FP64 MFMA
+// (v_mfma_f64_16x16x4_f64): the rows are the real and imaginary parts of up
+// to eight complex theta rows (row m: complex row m >> 1, part m & 1; the
+// operators are real, so each part is contracted on its own).  Operand
+// lanes: A row lr, u = 4 kb + lg; B column t = 16 tile + lr, u = 4 kb + lg
+// (Op column-major: entry (t, u) at u T + t); accumulator element w: row
+// lg + 4 w, column t = 16 tile + lr.
+template <class AF>
+__device__ __forceinline__ mfma4 row_block_op(const double* op, int T, int tile, int lane,
+                                              AF a_of) {
+  const int lr = lane & 15, lg = lane >> 4;
+  const int tcol = 16 * tile + lr;
+  mfma4 acc = {0.0, 0.0, 0.0, 0.0};
+  // operands of four k blocks (16 u) requested before the products
+  for (int k0 = 0; k0 < T; k0 += 16) {
+    double av[4], bv[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int u = k0 + 4 * q + lg;
+      bv[q] = (u < T && tcol < T) ? op[u * T + tcol] : 0.0;
+      av[q] = u < T ? a_of(lr, u) : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (k0 + 4 * q < T) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q], bv[q], acc, 0, 0, 0);
+  }
+  return acc;
+}
+
 // ------------------------------------------------------------ linear triple
 // KIND 0: stage fields 0..2 -> out 0..2, D Psi -> dpsi      (first order)
 // KIND 1: out fields 0..2 -> tan 0..2, D Psi -> dpsi_t      (its tangent)
@@ -276,13 +307,34 @@ __global__ __launch_bounds__(kSoThreads) void so_linear_kernel(SoArgs a) {
     o[2 * T + t] = out_s[j];
   }
   __syncthreads();
+  // band projection of the outputs on FP64 MFMA: the rows (output row,
+  // field) are contiguous theta rows of OUT; a block takes eight of them,
+  // real and imaginary parts, and the blocks go round the waves
   const int proj_off = I.op[coord][mode][kOpProject];
-  for (int e = tid; e < orows * us; e += kSoThreads) {
-    const int rr = e / us, rem = e - rr * us, f = rem / T, t = rem - f * T, r = tile.t0 + rr;
-    const cdouble v = row_op(a.ops, proj_off, OUT + rr * us + f * T, t, T);
-    if constexpr (KIND == 0) a.out[fidx(I, mode, f, r, t)] = v;
-    if constexpr (KIND == 1) a.tan[sidx(I, kSlotsTan, mode, f, r, t)] = v;
-    if constexpr (KIND == 2) a.out[fidx(I, mode, 10 + f, r, t)] = v;
+  const int lane = tid & 63, lg = lane >> 4, lr = lane & 15;
+  const int crows = orows * 3, ttiles = (T + 15) / 16;
+  for (int blk = tid >> 6; blk < ((crows + 7) / 8) * ttiles; blk += kSoThreads / 64) {
+    const int b8 = 8 * (blk / ttiles), tile_t = blk % ttiles;
+    auto a_of = [&](int mm, int u) {
+      const int cr = b8 + (mm >> 1);
+      if (cr >= crows) return 0.0;
+      const cdouble v = OUT[cr * T + u];
+      return (mm & 1) ? v.im : v.re;
+    };
+    const mfma4 acc = proj_off < 0 ? mfma4{0.0, 0.0, 0.0, 0.0}
+                                   : row_block_op(a.ops + proj_off, T, tile_t, lane, a_of);
+    const int t = 16 * tile_t + lr;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      const int mm = lg + 4 * w, cr = b8 + (mm >> 1);
+      if (cr >= crows || t >= T) continue;
+      const int rr = cr / 3, f = cr - 3 * rr, r = tile.t0 + rr;
+      cdouble* dst;
+      if constexpr (KIND == 0) dst = a.out + fidx(I, mode, f, r, t);
+      if constexpr (KIND == 1) dst = a.tan + sidx(I, kSlotsTan, mode, f, r, t);
+      if constexpr (KIND == 2) dst = a.out + fidx(I, mode, 10 + f, r, t);
+      reinterpret_cast<double*>(dst)[mm & 1] = acc[w];
+    }
   }
 }
 
@@ -721,37 +773,6 @@ __global__ __launch_bounds__(kSoThreads) void so_source_kernel(SoArgs a, int max
 }
 
 // ------------------------------------------------------------ source projection
-typedef double mfma4 __attribute__((ext_vector_type(4)));
-
-// Out[m][t] = sum_u A[m][u] Op[t][u] for one 16-row block on FP64 MFMA
-// (v_mfma_f64_16x16x4_f64): the rows are the real and imaginary parts of up
-// to eight complex theta rows (row m: complex row m >> 1, part m & 1; the
-// operators are real, so each part is contracted on its own).  Operand
-// lanes: A row lr, u = 4 kb + lg; B column t = 16 tile + lr, u = 4 kb + lg
-// (Op column-major: entry (t, u) at u T + t); accumulator element w: row
-// lg + 4 w, column t = 16 tile + lr.
-template <class AF>
-__device__ __forceinline__ mfma4 row_block_op(const double* op, int T, int tile, int lane,
-                                              AF a_of) {
-  const int lr = lane & 15, lg = lane >> 4;
-  const int tcol = 16 * tile + lr;
-  mfma4 acc = {0.0, 0.0, 0.0, 0.0};
-  // operands of four k blocks (16 u) requested before the products
-  for (int k0 = 0; k0 < T; k0 += 16) {
-    double av[4], bv[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int u = k0 + 4 * q + lg;
-      bv[q] = (u < T && tcol < T) ? op[u * T + tcol] : 0.0;
-      av[q] = u < T ? a_of(lr, u) : 0.0;
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-      if (k0 + 4 * q < T) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q], bv[q], acc, 0, 0, 0);
-  }
-  return acc;
-}
-
 // Band projection of D (spin -2) and T (spin -1) with their tangents and
 // eth'_3 T (spatial_pipeline.hpp:1183-1209): one workgroup per kProjRows (4)
 // radial rows of an instance.  Each operator application is a 16-row MFMA
