@@ -44,6 +44,8 @@ CASES = {
     "d105_constrained": so_case(reduction_mode="stage_constrained", spin=0.9, nr=45),
     "m0_parent": so_case(first_order_modes=[-1, 0, 1], second_order_modes=[-2, -1, 0, 1, 2],
                          seeds=[(2, 1, 1e-3), (3, 0, 5e-4)], ellmax_first=4, ellmax_second=5),
+    # theta rows wider than one 16-node tile: the per-thread projections
+    "wide_theta": so_case(ellmax_first=6, ellmax_second=14, nr=37, ntheta=20),
 }
 
 

@@ -467,6 +467,39 @@ __global__ __launch_bounds__(NT) void so_recon_kernel(SoArgs a) {
   auto project = [&](int coord, const cdouble* in) {
     return row_op<CH>(a.ops, op(coord, kOpProject), in + pi * T, t, T);
   };
+  // Band projections of whole LDS rows, in place, on FP64 MFMA when a
+  // theta row fits one 16-wide tile: a block is (projection j, parent mode,
+  // eight radial rows), its sixteen MFMA rows the real and imaginary parts.
+  // A block reads and writes only its own rows, so every wave takes blocks
+  // independently; the caller syncs before and after.  Wider rows keep the
+  // per-thread contraction (`project`).
+  const bool mfma_proj = T <= 16;
+  const int rows_here = min(rpw, I.N - r0);
+  auto project_rows = [&](int nproj, const int* coords, const int* krows) {
+    const int lane = tid & 63, lr = lane & 15, lg = lane >> 4;
+    const int np = I.n_parents, nsb = (rows_here + 7) / 8;
+    const int waves = static_cast<int>(blockDim.x) / 64;
+    for (int blk = tid >> 6; blk < nproj * np * nsb; blk += waves) {
+      const int j = blk / (np * nsb), rem = blk - j * (np * nsb), p = rem / nsb;
+      const int sb = rem - p * nsb;
+      const int off = I.op[coords[j]][I.parent_mode[p]][kOpProject];
+      cdouble* base = reinterpret_cast<cdouble*>(smem) + krows[j] * MT + p * T;
+      auto a_of = [&](int m, int u) {
+        const int s = 8 * sb + (m >> 1);
+        if (s >= rows_here) return 0.0;
+        const cdouble v = base[s * kReconLdsRows * MT + u];
+        return (m & 1) ? v.im : v.re;
+      };
+      const mfma4 acc =
+          off < 0 ? mfma4{0.0, 0.0, 0.0, 0.0} : row_block_op(a.ops + off, T, 0, lane, a_of);
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        const int m = lg + 4 * w, s = 8 * sb + (m >> 1);
+        if (s < rows_here && lr < T)
+          reinterpret_cast<double*>(base + s * kReconLdsRows * MT + lr)[m & 1] = acc[w];
+      }
+    }
+  };
   const cdouble* src = KIND == 0 ? a.in : a.out;
   cdouble* dst = KIND == 0 ? a.out : a.tan;
   const cdouble* rdr = KIND == 0 ? a.rdr : a.rdr_t;
@@ -518,14 +551,19 @@ __global__ __launch_bounds__(NT) void so_recon_kernel(SoArgs a) {
     row(lRaw + 1)[e] = recon_dt(xLa, d1, dl, 1, R, mass, L);
   }
   __syncthreads();
+  if (mfma_proj) {
+    const int coords[2] = {kCoG, kCoFirst}, krows[2] = {lRaw, lRaw + 1};
+    project_rows(2, coords, krows);
+    __syncthreads();
+  }
   if (live) {
     d2 = dr(2);
     d3 = dr(3);
     d4 = dr(4);
     d5 = dr(5);
-    tG = project(kCoG, row(lRaw));
+    tG = mfma_proj ? row(lRaw)[e] : project(kCoG, row(lRaw));
     store(1, tG);
-    store(2, project(kCoFirst, row(lRaw + 1)));
+    store(2, mfma_proj ? row(lRaw + 1)[e] : project(kCoFirst, row(lRaw + 1)));
     e2G = ghp(kCoG, false, row(lG), xG, tG);
   }
   __syncthreads();
@@ -546,12 +584,18 @@ __global__ __launch_bounds__(NT) void so_recon_kernel(SoArgs a) {
     row(lRaw + 3)[e] = recon_dt(xC, d5, dc, 2, R, mass, L);
   }
   __syncthreads();
+  if (mfma_proj) {
+    const int coords[4] = {kCoScalar, kCoB, kCoPi, kCoC};
+    const int krows[4] = {lRaw, lRaw + 1, lRaw + 2, lRaw + 3};
+    project_rows(4, coords, krows);
+    __syncthreads();
+  }
   if (live) {
     d6 = dr(6);
-    tH = project(kCoScalar, row(lRaw));
-    tB = project(kCoB, row(lRaw + 1));
-    tPi = project(kCoPi, row(lRaw + 2));
-    tC = project(kCoC, row(lRaw + 3));
+    tH = mfma_proj ? row(lRaw)[e] : project(kCoScalar, row(lRaw));
+    tB = mfma_proj ? row(lRaw + 1)[e] : project(kCoB, row(lRaw + 1));
+    tPi = mfma_proj ? row(lRaw + 2)[e] : project(kCoPi, row(lRaw + 2));
+    tC = mfma_proj ? row(lRaw + 3)[e] : project(kCoC, row(lRaw + 3));
     store(3, tH);
     store(4, tB);
     store(5, tPi);
@@ -598,8 +642,13 @@ __global__ __launch_bounds__(NT) void so_recon_kernel(SoArgs a) {
     }
   }
   __syncthreads();
+  if (mfma_proj) {
+    const int coords[1] = {kCoScalar}, krows[1] = {lRaw};
+    project_rows(1, coords, krows);
+    __syncthreads();
+  }
   if (!live) return;
-  store(7, project(kCoScalar, row(lRaw)));
+  store(7, mfma_proj ? row(lRaw)[e] : project(kCoScalar, row(lRaw)));
   if constexpr (KIND == 0) {
     if (!a.want_constraints) return;
     // reloaded rather than held in registers through the chain
