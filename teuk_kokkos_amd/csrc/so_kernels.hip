@@ -399,14 +399,16 @@ __global__ __launch_bounds__(kSoThreads) void so_drad_kernel(SoArgs a, int which
 // reconstruction_constraints.hpp:24-56; source inputs: 934-1181,
 // source_tangent_spatial.hpp:65-224; projection: 1183-1209.
 //
-// One workgroup per (instance, r) and one thread per (mode, theta) element,
-// so every value a thread only needs at its own element stays in registers
-// across the phases.  LDS holds just the rows other threads read: the theta
-// rows the angular operators contract over and the sharp partners
-// conj(X[-m]).  Rows: F, G, B, Pi, C of the input, the time derivatives of
-// B and C, and four rows of reconstructed time derivatives awaiting their
-// band projection: 11 * M * N_theta * 16 bytes (13.2 KiB for 5 modes x 15
-// nodes, where register use rather than LDS then bounds the rows per CU).
+// One thread per (parent mode, theta) element of a radial row, several
+// radial rows per workgroup, so every value a thread only needs at its own
+// element stays in registers across the phases.  LDS holds just the rows
+// other threads read: the theta rows the angular operators contract over
+// and the sharp partners conj(X[-m]).  Rows: F, G, B, Pi, C of the input,
+// the time derivatives of B and C, four rows of reconstructed time
+// derivatives awaiting their band projection, and one for an angular part:
+// 12 * P * N_theta * 16 bytes per radial row (P parent modes).  The angular
+// operators run on FP64 MFMA over the workgroup's rows (apply_rows) and
+// leave their results in these rows.
 template <int KIND, int NT>
 __global__ __launch_bounds__(NT) void so_recon_kernel(SoArgs a) {
   constexpr int CH = 8;  // operator loads in flight per row_op chunk
@@ -437,7 +439,7 @@ __global__ __launch_bounds__(NT) void so_recon_kernel(SoArgs a) {
   const double* nodes = node_ptr(a, I);
   const double sx = nodes[T + t], cx = nodes[t];
   cdouble* LX = reinterpret_cast<cdouble*>(smem) + (live ? sub : 0) * kReconLdsRows * MT;
-  enum { lF = 0, lG, lB, lPi, lC, ltB, ltC, lRaw };
+  enum { lF = 0, lG, lB, lPi, lC, ltB, ltC, lRaw, lAng = lRaw + 4 };
   auto row = [&](int k) { return LX + k * MT; };
   auto op = [&](int coord, int kind) { return I.op[coord][mode][kind]; };
   const cdouble czero = cmake(0.0, 0.0);
@@ -467,39 +469,55 @@ __global__ __launch_bounds__(NT) void so_recon_kernel(SoArgs a) {
   auto project = [&](int coord, const cdouble* in) {
     return row_op<CH>(a.ops, op(coord, kOpProject), in + pi * T, t, T);
   };
-  // Band projections of whole LDS rows, in place, on FP64 MFMA when a
-  // theta row fits one 16-wide tile: a block is (projection j, parent mode,
-  // eight radial rows), its sixteen MFMA rows the real and imaginary parts.
-  // A block reads and writes only its own rows, so every wave takes blocks
-  // independently; the caller syncs before and after.  Wider rows keep the
-  // per-thread contraction (`project`).
-  const bool mfma_proj = T <= 16;
+  // Angular operators applied to whole LDS rows on FP64 MFMA when a theta
+  // row fits one 16-wide tile: a block is (operator j, parent mode, eight
+  // radial rows), its sixteen MFMA rows the real and imaginary parts.
+  // Operator j (coordinator coords[j], kind kinds[j], of the block's mode)
+  // reads row srow[j] of that mode, or of its sharp partner when sharp[j],
+  // and writes row drow[j] of the mode; a block reads and writes only its
+  // own rows (in place when srow == drow), so the waves take blocks
+  // independently, and the caller syncs before and after.  Wider rows keep
+  // the per-thread contractions (`ghp`, `ghp_sharp`, `project`).
+  const bool mfma_rows = T <= 16;
   const int rows_here = min(rpw, I.N - r0);
-  auto project_rows = [&](int nproj, const int* coords, const int* krows) {
+  auto apply_rows = [&](int nops, const int* coords, const int* kinds, const int* srow,
+                        const int* drow, const bool* sharp) {
     const int lane = tid & 63, lr = lane & 15, lg = lane >> 4;
     const int np = I.n_parents, nsb = (rows_here + 7) / 8;
     const int waves = static_cast<int>(blockDim.x) / 64;
-    for (int blk = tid >> 6; blk < nproj * np * nsb; blk += waves) {
+    for (int blk = tid >> 6; blk < nops * np * nsb; blk += waves) {
       const int j = blk / (np * nsb), rem = blk - j * (np * nsb), p = rem / nsb;
       const int sb = rem - p * nsb;
-      const int off = I.op[coords[j]][I.parent_mode[p]][kOpProject];
-      cdouble* base = reinterpret_cast<cdouble*>(smem) + krows[j] * MT + p * T;
+      const int off = I.op[coords[j]][I.parent_mode[p]][kinds[j]];
+      const int ps = sharp[j] ? I.parent_sharp[p] : p;
+      const cdouble* in = reinterpret_cast<const cdouble*>(smem) + srow[j] * MT + (ps < 0 ? 0 : ps) * T;
+      cdouble* out = reinterpret_cast<cdouble*>(smem) + drow[j] * MT + p * T;
       auto a_of = [&](int m, int u) {
         const int s = 8 * sb + (m >> 1);
         if (s >= rows_here) return 0.0;
-        const cdouble v = base[s * kReconLdsRows * MT + u];
+        const cdouble v = in[s * kReconLdsRows * MT + u];
         return (m & 1) ? v.im : v.re;
       };
-      const mfma4 acc =
-          off < 0 ? mfma4{0.0, 0.0, 0.0, 0.0} : row_block_op(a.ops + off, T, 0, lane, a_of);
+      const mfma4 acc = off < 0 || ps < 0 ? mfma4{0.0, 0.0, 0.0, 0.0}
+                                          : row_block_op(a.ops + off, T, 0, lane, a_of);
 #pragma unroll
       for (int w = 0; w < 4; ++w) {
         const int m = lg + 4 * w, s = 8 * sb + (m >> 1);
         if (s < rows_here && lr < T)
-          reinterpret_cast<double*>(base + s * kReconLdsRows * MT + lr)[m & 1] = acc[w];
+          reinterpret_cast<double*>(out + s * kReconLdsRows * MT + lr)[m & 1] = acc[w];
       }
     }
   };
+  // eth / eth' of this element from an angular part computed by apply_rows
+  auto ghp_from = [&](int coord, bool prime, cdouble xo, cdouble dto, cdouble ang) {
+    if (op(coord, prime ? kOpLower : kOpRaise) < 0) return czero;
+    return eth(coord, prime, xo, dto, ang);
+  };
+  auto ghp_sharp_from = [&](int coord, const cdouble* xr, const cdouble* dr_row, cdouble ang) {
+    if (op(coord, kOpLower) < 0 || spi < 0) return czero;
+    return eth(coord, true, cconj(xr[se]), cconj(dr_row[se]), cconj(ang));
+  };
+
   const cdouble* src = KIND == 0 ? a.in : a.out;
   cdouble* dst = KIND == 0 ? a.out : a.tan;
   const cdouble* rdr = KIND == 0 ? a.rdr : a.rdr_t;
@@ -540,10 +558,19 @@ __global__ __launch_bounds__(NT) void so_recon_kernel(SoArgs a) {
     row(lC)[e] = xC;
   }
   __syncthreads();
+  if (mfma_rows) {
+    const int co[2] = {kCoFirst, kCoFirst}, ki[2] = {kOpRaise, kOpLower};
+    const int sr[2] = {lF, lF}, dr_[2] = {lRaw + 2, lRaw + 3};
+    const bool sh[2] = {false, false};
+    apply_rows(2, co, ki, sr, dr_, sh);
+    __syncthreads();
+  }
   // pass 1: G, Lambda
   if (live) {
-    e1F = ghp(kCoFirst, false, row(lF), xF, tF);
-    ex(1, ghp(kCoFirst, true, row(lF), xF, tF));
+    e1F = mfma_rows ? ghp_from(kCoFirst, false, xF, tF, row(lRaw + 2)[e])
+                    : ghp(kCoFirst, false, row(lF), xF, tF);
+    ex(1, mfma_rows ? ghp_from(kCoFirst, true, xF, tF, row(lRaw + 3)[e])
+                    : ghp(kCoFirst, true, row(lF), xF, tF));
     const cdouble dg = csub(cadd(cmul(cscale(-4.0 * R, bg.mu0), xG), e1F),
                             cmul(cscale(R, bg.tau0), xF));
     const cdouble dl = csub(cmul(cscale(-R, cadd(bg.mu0, cconj(bg.mu0))), xLa), xF);
@@ -551,9 +578,11 @@ __global__ __launch_bounds__(NT) void so_recon_kernel(SoArgs a) {
     row(lRaw + 1)[e] = recon_dt(xLa, d1, dl, 1, R, mass, L);
   }
   __syncthreads();
-  if (mfma_proj) {
-    const int coords[2] = {kCoG, kCoFirst}, krows[2] = {lRaw, lRaw + 1};
-    project_rows(2, coords, krows);
+  if (mfma_rows) {
+    const int co[3] = {kCoG, kCoFirst, kCoG}, ki[3] = {kOpProject, kOpProject, kOpRaise};
+    const int sr[3] = {lRaw, lRaw + 1, lG}, dr_[3] = {lRaw, lRaw + 1, lRaw + 2};
+    const bool sh[3] = {false, false, false};
+    apply_rows(3, co, ki, sr, dr_, sh);
     __syncthreads();
   }
   if (live) {
@@ -561,10 +590,11 @@ __global__ __launch_bounds__(NT) void so_recon_kernel(SoArgs a) {
     d3 = dr(3);
     d4 = dr(4);
     d5 = dr(5);
-    tG = mfma_proj ? row(lRaw)[e] : project(kCoG, row(lRaw));
+    tG = mfma_rows ? row(lRaw)[e] : project(kCoG, row(lRaw));
     store(1, tG);
-    store(2, mfma_proj ? row(lRaw + 1)[e] : project(kCoFirst, row(lRaw + 1)));
-    e2G = ghp(kCoG, false, row(lG), xG, tG);
+    store(2, mfma_rows ? row(lRaw + 1)[e] : project(kCoFirst, row(lRaw + 1)));
+    e2G = mfma_rows ? ghp_from(kCoG, false, xG, tG, row(lRaw + 2)[e])
+                    : ghp(kCoG, false, row(lG), xG, tG);
   }
   __syncthreads();
   // pass 2: H, B, Pi, C
@@ -584,36 +614,64 @@ __global__ __launch_bounds__(NT) void so_recon_kernel(SoArgs a) {
     row(lRaw + 3)[e] = recon_dt(xC, d5, dc, 2, R, mass, L);
   }
   __syncthreads();
-  if (mfma_proj) {
-    const int coords[4] = {kCoScalar, kCoB, kCoPi, kCoC};
-    const int krows[4] = {lRaw, lRaw + 1, lRaw + 2, lRaw + 3};
-    project_rows(4, coords, krows);
+  if (mfma_rows) {
+    // the angular parts of eth C, eth B go to the rows that take the
+    // projected C_t, B_t afterwards (each element is read by its thread
+    // before it writes there), eth Pi's to the spare row
+    const int co[7] = {kCoScalar, kCoB, kCoPi, kCoC, kCoC, kCoPi, kCoB};
+    const int ki[7] = {kOpProject, kOpProject, kOpProject, kOpProject,
+                       kOpRaise,   kOpRaise,   kOpRaise};
+    const int sr[7] = {lRaw, lRaw + 1, lRaw + 2, lRaw + 3, lC, lPi, lB};
+    const int dr_[7] = {lRaw, lRaw + 1, lRaw + 2, lRaw + 3, ltC, lAng, ltB};
+    const bool sh[7] = {false, false, false, false, false, false, false};
+    apply_rows(7, co, ki, sr, dr_, sh);
     __syncthreads();
   }
   if (live) {
     d6 = dr(6);
-    tH = mfma_proj ? row(lRaw)[e] : project(kCoScalar, row(lRaw));
-    tB = mfma_proj ? row(lRaw + 1)[e] : project(kCoB, row(lRaw + 1));
-    tPi = mfma_proj ? row(lRaw + 2)[e] : project(kCoPi, row(lRaw + 2));
-    tC = mfma_proj ? row(lRaw + 3)[e] : project(kCoC, row(lRaw + 3));
+    if (mfma_rows) {
+      tH = row(lRaw)[e];
+      tB = row(lRaw + 1)[e];
+      tPi = row(lRaw + 2)[e];
+      tC = row(lRaw + 3)[e];
+      e2C = ghp_from(kCoC, false, xC, tC, row(ltC)[e]);
+      e2Pi = ghp_from(kCoPi, false, xPi, tPi, row(lAng)[e]);
+      ex(0, ghp_from(kCoB, false, xB, tB, row(ltB)[e]));
+    } else {
+      tH = project(kCoScalar, row(lRaw));
+      tB = project(kCoB, row(lRaw + 1));
+      tPi = project(kCoPi, row(lRaw + 2));
+      tC = project(kCoC, row(lRaw + 3));
+      e2C = ghp(kCoC, false, row(lC), xC, tC);
+      e2Pi = ghp(kCoPi, false, row(lPi), xPi, tPi);
+      ex(0, ghp(kCoB, false, row(lB), xB, tB));
+    }
     store(3, tH);
     store(4, tB);
     store(5, tPi);
     store(6, tC);
     row(ltB)[e] = tB;
     row(ltC)[e] = tC;
-    e2C = ghp(kCoC, false, row(lC), xC, tC);
-    e2Pi = ghp(kCoPi, false, row(lPi), xPi, tPi);
-    ex(0, ghp(kCoB, false, row(lB), xB, tB));
   }
   __syncthreads();
+  if (mfma_rows) {
+    const int co[2] = {kCoBSharp, kCoCSharp}, ki[2] = {kOpLower, kOpLower};
+    const int sr[2] = {lB, lC}, dr_[2] = {lRaw + 2, lRaw + 3};
+    const bool sh[2] = {true, true};
+    apply_rows(2, co, ki, sr, dr_, sh);
+    __syncthreads();
+  }
   // pass 3: U, with the sharp fields conj(X[-m]) (spatial_pipeline.hpp:
   // 895-932) read from their partners' rows.  The raw rows are free from
   // here on: the first takes the raw U derivative, the second the H row
   // (for the constraint eth').
   if (live) {
-    const cdouble ep1BS = ghp_sharp(kCoBSharp, row(lB), row(ltB));
-    const cdouble ep2CS = ghp_sharp(kCoCSharp, row(lC), row(ltC));
+    const cdouble ep1BS = mfma_rows
+                              ? ghp_sharp_from(kCoBSharp, row(lB), row(ltB), row(lRaw + 2)[e])
+                              : ghp_sharp(kCoBSharp, row(lB), row(ltB));
+    const cdouble ep2CS = mfma_rows
+                              ? ghp_sharp_from(kCoCSharp, row(lC), row(ltC), row(lRaw + 3)[e])
+                              : ghp_sharp(kCoCSharp, row(lC), row(ltC));
     const cdouble mu0b = cconj(bg.mu0), pi0b = cconj(bg.pi0), tau0b = cconj(bg.tau0);
     const cdouble pi_s = sharp_of(lPi), b_s = sharp_of(lB), c_s = sharp_of(lC);
     cdouble du = cmul(cscale(-R, mu0b), xU);
@@ -642,13 +700,14 @@ __global__ __launch_bounds__(NT) void so_recon_kernel(SoArgs a) {
     }
   }
   __syncthreads();
-  if (mfma_proj) {
-    const int coords[1] = {kCoScalar}, krows[1] = {lRaw};
-    project_rows(1, coords, krows);
+  if (mfma_rows) {
+    const int co[1] = {kCoScalar}, ki[1] = {kOpProject}, sr[1] = {lRaw}, dr_[1] = {lRaw};
+    const bool sh[1] = {false};
+    apply_rows(1, co, ki, sr, dr_, sh);
     __syncthreads();
   }
   if (!live) return;
-  store(7, mfma_proj ? row(lRaw)[e] : project(kCoScalar, row(lRaw)));
+  store(7, mfma_rows ? row(lRaw)[e] : project(kCoScalar, row(lRaw)));
   if constexpr (KIND == 0) {
     if (!a.want_constraints) return;
     // reloaded rather than held in registers through the chain

@@ -35,7 +35,7 @@ inline int so_drad_blocks(int M, int N, int T) {
 // else one row of up to kReconMaxThreads threads.
 inline constexpr int kReconThreads = 256;
 inline constexpr int kReconMaxThreads = 1024;  // largest P * N_theta a plan accepts
-inline constexpr int kReconLdsRows = 11;       // theta rows of P * N_theta values per radial row
+inline constexpr int kReconLdsRows = 12;       // theta rows of P * N_theta values per radial row
 inline int recon_rows_per_group(int mt) { return mt <= kReconThreads ? kReconThreads / mt : 1; }
 
 // Angular coordinators of the stage graph (spatial_pipeline.hpp:185-218):
