@@ -774,110 +774,108 @@ __global__ __launch_bounds__(kSoThreads) void so_source_kernel(SoArgs a, int max
   const long long base = uniform64(I.base), pbase = uniform64(I.pbase);
   auto slot_of = [](int k) { return 2 + k; };
   enum { xF = 0, xG, xLa, xH, xB, xPi, xC, xU };
-  {
-    const int pt = static_cast<int>(p0) + static_cast<int>(threadIdx.x);
-    if (pt >= N * T) return;
-    const int r = pt / T, t = pt - r * T;
-    const double R = I.h * r;
-    // Every access is a plane start plus the one in-plane byte offset, a
-    // 32-bit value (planes hold fewer than 2^27 points), so the per-thread
-    // part of every address is one register shared by all the loads.
-    const unsigned off = static_cast<unsigned>(pt) * static_cast<unsigned>(sizeof(cdouble));
-    auto at = [&](const cdouble* p) {
-      return *reinterpret_cast<const cdouble*>(reinterpret_cast<const char*>(p) + off);
-    };
-    auto put = [&](cdouble* p, cdouble v) {
-      *reinterpret_cast<cdouble*>(reinterpret_cast<char*>(p) + off) = v;
-    };
-    auto fplane = [&](int mode, int f) { return base + (mode * 13LL + f) * plane; };
-    auto splane = [&](int slots, int mode, int sl) {
-      return pbase * slots + (static_cast<long long>(mode) * slots + sl) * plane;
-    };
-    auto S = [&](int mode, int k) { return at(a.in + fplane(mode, slot_of(k))); };
-    auto X = [&](int mode, int k) { return at(a.out + fplane(mode, slot_of(k))); };
-    auto XT = [&](int mode, int k) { return at(a.tan + splane(kSlotsTan, mode, slot_of(k))); };
-    auto rd = [&](int mode, int c) { return at(a.rdr + splane(kSlotsRdr, mode, c)); };
-    auto rdt = [&](int mode, int c) { return at(a.rdr_t + splane(kSlotsRdr, mode, c)); };
-    auto rg = [&](int mode, int k) { return at(a.rang + splane(kSlotsRang, mode, k)); };
-    auto rgt = [&](int mode, int k) { return at(a.rang_t + splane(kSlotsRang, mode, k)); };
-    auto ex = [&](int mode, int k) { return at(a.ex + splane(kSlotsEx, mode, k)); };
-    const Bg bg = *reinterpret_cast<const Bg*>(reinterpret_cast<const char*>(
-        a.bg + uniform64(I.bgoff) * 2) + static_cast<unsigned>(pt) * static_cast<unsigned>(sizeof(Bg)));
-    Jet totD{cmake(0.0, 0.0), cmake(0.0, 0.0)}, totT = totD;
-    const int k_end = uniform(I.pair_begin[ts + 1]);
-    for (int k = uniform(I.pair_begin[ts]); k < k_end; ++k) {
-      const int m1 = uniform(I.pair_m1[k]), m2 = uniform(I.pair_m2[k]);
-      const int s1 = uniform(I.sharp[m1]), s2 = uniform(I.sharp[m2]);
-    auto fj = [&](int mode, int k2) { return Jet{S(mode, k2), X(mode, k2)}; };
-      // field jets: value from the stage, tangent from its RHS
-      auto f = [&](int i) -> Jet {
-        switch (i) {
-          case 0: return fj(m1, xF);
-          case 1: return fj(m1, xG);
-          case 2: return fj(m1, xLa);
-          case 3: return fj(m1, xPi);
-          case 4: return fj(m1, xB);
-          case 5: return fj(m1, xC);
-          case 6: return fj(m1, xU);
-          case 7: return fj(m2, xF);
-          case 8: return fj(m2, xG);
-          case 9: return fj(m2, xH);
-          case 10: return fj(m2, xB);
-          case 11: return fj(m2, xC);
-          case 12: return fj(m2, xU);
-          case 13: return jconj(fj(s2, xU));
-          case 14: return jconj(fj(s2, xC));
-          case 15: return jconj(fj(s1, xC));
-          case 16: return jconj(fj(s1, xB));
-          case 17: return jconj(fj(s2, xPi));
-          default: return jconj(fj(s2, xB));
-        }
-      };
-      // the eleven derivative jets of m2 (spatial_pipeline.hpp:1019-1180)
-      auto dl = [&](int mode, int k2, int n, int c) {
-        const cdouble x = X(mode, k2);
-        return Jet{delta_n(S(mode, k2), x, rd(mode, c), n, R, mass, L),
-                   delta_n(x, XT(mode, k2), rdt(mode, c), n, R, mass, L)};
-      };
-      auto d = [&](int i) -> Jet {
-        switch (i) {
-          case 0: {
-            const cdouble xf = X(m2, xF);
-            return {delta_n(S(m2, xF), xf, at(a.dpsi + splane(1, m2, 0)), 1, R, mass, L),
-                    delta_n(xf, XT(m2, xF), at(a.dpsi_t + splane(1, m2, 0)), 1, R, mass,
-                            L)};
-          }
-          case 1: return dl(m2, xU, 3, 6);
-          case 2: return {rg(m2, kRangEth2C), rgt(m2, kRangEth2C)};
-          case 3: return {rg(m2, kRangEthP2CSharp), rgt(m2, kRangEthP2CSharp)};
-          case 4: return {ex(m2, 0), ex(m2, 2)};
-          case 5: return dl(m2, xC, 2, 5);
-          case 6: return dl(m2, xG, 2, 0);
-          case 7: return {rg(m2, kRangEth2G), rgt(m2, kRangEth2G)};
-          case 8: return {ex(m2, 1), ex(m2, 3)};
-          case 9: {
-            const cdouble xcs = cconj(X(s2, xC));
-            return {delta_n(cconj(S(s2, xC)), xcs, cconj(rd(s2, 5)), 2, R, mass, L),
-                    delta_n(xcs, cconj(XT(s2, xC)), cconj(rdt(s2, 5)), 2, R, mass, L)};
-          }
-          default: return {rg(m2, kRangEthP1BSharp), rgt(m2, kRangEthP1BSharp)};
-        }
-      };
-      Jet Dp, Tp;
-      pair_source(R, bg, f, d, Dp, Tp);
-      if (a.pairs) {
-        put(a.pairs + 2 * k * plane, Dp.v);
-        put(a.pairs + (2 * k + 1) * plane, Tp.v);
+  const int pt = static_cast<int>(p0) + static_cast<int>(threadIdx.x);
+  if (pt >= N * T) return;
+  const int r = pt / T, t = pt - r * T;
+  const double R = I.h * r;
+  // Every access is a plane start plus the one in-plane byte offset, a
+  // 32-bit value (planes hold fewer than 2^27 points), so the per-thread
+  // part of every address is one register shared by all the loads.
+  const unsigned off = static_cast<unsigned>(pt) * static_cast<unsigned>(sizeof(cdouble));
+  auto at = [&](const cdouble* p) {
+    return *reinterpret_cast<const cdouble*>(reinterpret_cast<const char*>(p) + off);
+  };
+  auto put = [&](cdouble* p, cdouble v) {
+    *reinterpret_cast<cdouble*>(reinterpret_cast<char*>(p) + off) = v;
+  };
+  auto fplane = [&](int mode, int f) { return base + (mode * 13LL + f) * plane; };
+  auto splane = [&](int slots, int mode, int sl) {
+    return pbase * slots + (static_cast<long long>(mode) * slots + sl) * plane;
+  };
+  auto S = [&](int mode, int k) { return at(a.in + fplane(mode, slot_of(k))); };
+  auto X = [&](int mode, int k) { return at(a.out + fplane(mode, slot_of(k))); };
+  auto XT = [&](int mode, int k) { return at(a.tan + splane(kSlotsTan, mode, slot_of(k))); };
+  auto rd = [&](int mode, int c) { return at(a.rdr + splane(kSlotsRdr, mode, c)); };
+  auto rdt = [&](int mode, int c) { return at(a.rdr_t + splane(kSlotsRdr, mode, c)); };
+  auto rg = [&](int mode, int k) { return at(a.rang + splane(kSlotsRang, mode, k)); };
+  auto rgt = [&](int mode, int k) { return at(a.rang_t + splane(kSlotsRang, mode, k)); };
+  auto ex = [&](int mode, int k) { return at(a.ex + splane(kSlotsEx, mode, k)); };
+  const Bg bg = *reinterpret_cast<const Bg*>(reinterpret_cast<const char*>(
+      a.bg + uniform64(I.bgoff) * 2) + static_cast<unsigned>(pt) * static_cast<unsigned>(sizeof(Bg)));
+  Jet totD{cmake(0.0, 0.0), cmake(0.0, 0.0)}, totT = totD;
+  const int k_end = uniform(I.pair_begin[ts + 1]);
+  for (int k = uniform(I.pair_begin[ts]); k < k_end; ++k) {
+    const int m1 = uniform(I.pair_m1[k]), m2 = uniform(I.pair_m2[k]);
+    const int s1 = uniform(I.sharp[m1]), s2 = uniform(I.sharp[m2]);
+  auto fj = [&](int mode, int k2) { return Jet{S(mode, k2), X(mode, k2)}; };
+    // field jets: value from the stage, tangent from its RHS
+    auto f = [&](int i) -> Jet {
+      switch (i) {
+        case 0: return fj(m1, xF);
+        case 1: return fj(m1, xG);
+        case 2: return fj(m1, xLa);
+        case 3: return fj(m1, xPi);
+        case 4: return fj(m1, xB);
+        case 5: return fj(m1, xC);
+        case 6: return fj(m1, xU);
+        case 7: return fj(m2, xF);
+        case 8: return fj(m2, xG);
+        case 9: return fj(m2, xH);
+        case 10: return fj(m2, xB);
+        case 11: return fj(m2, xC);
+        case 12: return fj(m2, xU);
+        case 13: return jconj(fj(s2, xU));
+        case 14: return jconj(fj(s2, xC));
+        case 15: return jconj(fj(s1, xC));
+        case 16: return jconj(fj(s1, xB));
+        case 17: return jconj(fj(s2, xPi));
+        default: return jconj(fj(s2, xB));
       }
-      totD = jadd(totD, Dp);
-      totT = jadd(totT, Tp);
+    };
+    // the eleven derivative jets of m2 (spatial_pipeline.hpp:1019-1180)
+    auto dl = [&](int mode, int k2, int n, int c) {
+      const cdouble x = X(mode, k2);
+      return Jet{delta_n(S(mode, k2), x, rd(mode, c), n, R, mass, L),
+                 delta_n(x, XT(mode, k2), rdt(mode, c), n, R, mass, L)};
+    };
+    auto d = [&](int i) -> Jet {
+      switch (i) {
+        case 0: {
+          const cdouble xf = X(m2, xF);
+          return {delta_n(S(m2, xF), xf, at(a.dpsi + splane(1, m2, 0)), 1, R, mass, L),
+                  delta_n(xf, XT(m2, xF), at(a.dpsi_t + splane(1, m2, 0)), 1, R, mass,
+                          L)};
+        }
+        case 1: return dl(m2, xU, 3, 6);
+        case 2: return {rg(m2, kRangEth2C), rgt(m2, kRangEth2C)};
+        case 3: return {rg(m2, kRangEthP2CSharp), rgt(m2, kRangEthP2CSharp)};
+        case 4: return {ex(m2, 0), ex(m2, 2)};
+        case 5: return dl(m2, xC, 2, 5);
+        case 6: return dl(m2, xG, 2, 0);
+        case 7: return {rg(m2, kRangEth2G), rgt(m2, kRangEth2G)};
+        case 8: return {ex(m2, 1), ex(m2, 3)};
+        case 9: {
+          const cdouble xcs = cconj(X(s2, xC));
+          return {delta_n(cconj(S(s2, xC)), xcs, cconj(rd(s2, 5)), 2, R, mass, L),
+                  delta_n(xcs, cconj(XT(s2, xC)), cconj(rdt(s2, 5)), 2, R, mass, L)};
+        }
+        default: return {rg(m2, kRangEthP1BSharp), rgt(m2, kRangEthP1BSharp)};
+      }
+    };
+    Jet Dp, Tp;
+    pair_source(R, bg, f, d, Dp, Tp);
+    if (a.pairs) {
+      put(a.pairs + 2 * k * plane, Dp.v);
+      put(a.pairs + (2 * k + 1) * plane, Tp.v);
     }
-    const int mode = uniform(I.target_mode[ts]);
-    put(a.raw + splane(kSlotsRaw, mode, 0), totD.v);
-    put(a.raw + splane(kSlotsRaw, mode, 1), totD.d);
-    put(a.raw + splane(kSlotsRaw, mode, 2), totT.v);
-    put(a.raw + splane(kSlotsRaw, mode, 3), totT.d);
+    totD = jadd(totD, Dp);
+    totT = jadd(totT, Tp);
   }
+  const int mode = uniform(I.target_mode[ts]);
+  put(a.raw + splane(kSlotsRaw, mode, 0), totD.v);
+  put(a.raw + splane(kSlotsRaw, mode, 1), totD.d);
+  put(a.raw + splane(kSlotsRaw, mode, 2), totT.v);
+  put(a.raw + splane(kSlotsRaw, mode, 3), totT.d);
 }
 
 // ------------------------------------------------------------ source projection

@@ -251,6 +251,10 @@ void Plan::so_build() {
     // raw sums and their projections (4 x T per target and row), and the
     // angular part of eth'_3 T (T per target and row)
     H.proj_lds = std::max(H.proj_lds, static_cast<int>(16 * 9LL * kProjRows * I.n_targets * T));
+    // the source kernel addresses a radial x theta plane with 32-bit byte
+    // offsets
+    if (static_cast<long long>(N) * T >= (1LL << 27))
+      throw std::invalid_argument("second-order plans need nr x ntheta < 2^27");
     H.max_source = std::max(H.max_source, static_cast<long long>(N) * T);
     H.max_targets = std::max(H.max_targets, I.n_targets);
     H.max_n = std::max(H.max_n, N);
