@@ -185,7 +185,9 @@ def test_constraint_aware_gate_matches_oracle():
 
 
 def test_second_order_batch_equals_independent_runs():
-    cases = [CASES["d105_kerr"], CASES["d42_kerr_diss"], CASES["m0_parent"]]
+    # different target counts, schemes and theta widths (MFMA and per-thread
+    # angular operators) in one batch
+    cases = [CASES["d105_kerr"], CASES["d42_kerr_diss"], CASES["m0_parent"], CASES["wide_theta"]]
     with T.Plan([W.config_text(**dict(c)) for c in cases]) as plan:
         plan.initialize()
         plan.advance(4)
