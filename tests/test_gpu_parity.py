@@ -1,5 +1,5 @@
 """HIP path vs CPU oracle on identical inputs (FP64 bar: <= 1e-12 relative
-on the state, north_star).  All calls go through the C ABI."""
+on the state).  All calls go through the C ABI."""
 from __future__ import annotations
 
 import numpy as np
