@@ -105,7 +105,7 @@ def measure_traffic(workload_name: str, second_order: bool, algorithmic_stage_by
         with tempfile.TemporaryDirectory() as d:
             cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run", "--",
                    sys.executable, os.path.abspath(__file__), "--workload", workload_name,
-                   "--steps", "1", "--warmup", "0", "--no-cpu", "--no-traffic",
+                   "--steps", "1", "--warmup", "0", "--no-cpu", "--no-traffic", "--eager",
                    "--calibrate-copy-mib", str(copy_bytes >> 20)]
             try:
                 # the profiler may exit non-zero after a complete run; the
@@ -141,7 +141,10 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="sweep512")
-    ap.add_argument("--graph", action="store_true", help="replay each step as a hipGraph")
+    ap.add_argument("--eager", action="store_true",
+                    help="launch every stage kernel directly instead of replaying the step as "
+                         "a captured hipGraph (the default for first-order plans)")
+    ap.add_argument("--graph", action="store_true", help=argparse.SUPPRESS)  # the default
     ap.add_argument("--lds-budget", type=int, default=0)
     ap.add_argument("--tile-rows", type=int, default=0)
     ap.add_argument("--segment-rows", type=int, default=-1,
@@ -195,11 +198,17 @@ def main() -> int:
     plan.initialize()
     print(f"[bench] initial data {time.perf_counter() - t0:.1f} s; tiles {plan.tiles()}",
           file=sys.stderr, flush=True)
-    plan.set_graph(args.graph)
+    # first-order plans replay each RK4 step as one captured hipGraph; the
+    # second-order graph launches its kernels directly (its latch decisions
+    # are taken on the host between steps)
+    use_graph = not args.eager and plan.modal_points > 0
+    plan.set_graph(use_graph)
     plan.advance(args.warmup)
     plan.sync()
 
-    plan.set_kernel_timing(not args.graph)
+    # HIP events around every step (around every replay in graph mode) and,
+    # in eager mode, around every stage launch
+    plan.set_kernel_timing(True)
     plan.sync()
     t0 = time.perf_counter()
     plan.advance(args.steps)
@@ -235,9 +244,12 @@ def main() -> int:
                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                     "kernel": "teuk_stage_kernel (4 RK stages)",
                     "avg_stage_ms": stage_s * 1e3,
-                    "launches_per_stage": launches / (4 * args.steps),
-                    "sum_of_launches_per_stage_ms": kernel_ms / (4 * args.steps),
-                    "groups": [{k: (round(v / (4 * args.steps), 4) if k == "kernel_ms" else v)
+                    "launches_per_stage": (float(sum(1 for g in groups if g["tiles"] > 0))
+                                           if use_graph else launches / (4 * args.steps)),
+                    "sum_of_launches_per_stage_ms": (None if use_graph
+                                                     else kernel_ms / (4 * args.steps)),
+                    "groups": [{k: (None if use_graph else round(v / (4 * args.steps), 4))
+                                if k == "kernel_ms" else v
                                 for k, v in g.items()} for g in groups]}
     out = {
         "metric": METRIC, "value": value, "unit": "grid-point-updates/s", "n_gpus": 1,
@@ -246,7 +258,7 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": desc, "instances": len(cases), "grid_points_per_step": grid,
                    "modal_points_per_step": modal, "precision": "fp64",
-                   "graph": bool(args.graph)},
+                   "graph": bool(use_graph)},
         "modal_updates_per_s": modal * args.steps / seconds,
         "roofline": roofline,
         "cpu_baseline": None,

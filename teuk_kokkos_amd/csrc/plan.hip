@@ -927,7 +927,19 @@ void Plan::advance(int steps) {
       hipGraphDestroy(graph);
     }
     for (int s = 0; s < steps; ++s) {
+      // with timing on, HIP events around each replay (recorded outside the
+      // graph) give the GPU time of the step
+      hipEvent_t e0 = nullptr;
+      if (timing_) {
+        e0 = timing_event();
+        check(hipEventRecord(e0, stream_), "event");
+      }
       check(hipGraphLaunch(graph_exec_, stream_), "graph launch");
+      if (timing_) {
+        hipEvent_t e1 = timing_event();
+        check(hipEventRecord(e1, stream_), "event");
+        step_events_.push_back({e0, e1});
+      }
       record_time_step(0.0);
     }
     return;
