@@ -282,3 +282,26 @@ def test_step_and_group_timing_account_for_every_launch():
     assert launches == 3 * 4 * len(groups)
     assert abs(sum(g["kernel_ms"] for g in per_group) - total) <= 1e-9 * max(total, 1.0)
     assert sum(g["tiles"] for g in groups) == sum(g["tiles"] for g in per_group)
+
+
+def test_graph_replays_are_timed_and_match_eager_steps():
+    """With timing on, each graph replay is bracketed by HIP events on the
+    plan's stream (no per-launch events inside the graph), and the replayed
+    steps leave the state the eager steps leave."""
+    cases = _two_group_cases()
+    texts = [W.config_text(**dict(c)) for c in cases]
+    with T.Plan(texts) as plan:
+        plan.initialize()
+        plan.set_graph(True)
+        plan.set_kernel_timing(True)
+        plan.advance(3)
+        ms, steps = plan.step_timing()
+        _, launches = plan.kernel_timing()
+        plan.set_kernel_timing(False)
+        graph_state = [plan.export_state(i) for i in range(len(cases))]
+    assert steps == 3 and ms > 0.0 and launches == 0
+    with T.Plan(texts) as plan:
+        plan.initialize()
+        plan.advance(3)
+        for i, g in enumerate(graph_state):
+            assert np.array_equal(plan.export_state(i), g)
