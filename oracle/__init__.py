@@ -102,6 +102,13 @@ def lib():
         L.oracle_ghp_row.argtypes = [i, i, i, i, i, i, d, d, d, dp, dp, dp]
         L.oracle_delta_n.argtypes = [dp, dp, dp, i, d, d, d, dp]
         L.oracle_recon_dt.argtypes = [dp, dp, dp, i, d, d, d, dp]
+        L.oracle_field_weights.argtypes = [ip]
+        L.oracle_eth_point.argtypes = [i, i, i, d, d, d, d, d, dp, dp, dp, dp]
+        L.oracle_recon_rhs.argtypes = [d, dp, dp, dp]
+        L.oracle_outer_source_point.argtypes = [d, dp, dp, dp]
+        L.oracle_coordinate_forcing.argtypes = [d, d, d, d, dp, dp]
+        L.oracle_constraint_terms.argtypes = [d, dp, dp, dp]
+        L.oracle_thorn.argtypes = [dp, dp, dp, i, i, i, i, d, d, d, d, d, dp, dp]
         L.oracle_endpoint_modal.restype = i
         L.oracle_endpoint_modal.argtypes = [p, i, dp, dp]
         L.oracle_gauss_legendre.argtypes = [i, dp, dp]
@@ -246,6 +253,84 @@ def ghp_row(spin, boost, m, ell_max, x, dt, *, prime=False, kerr_spin=0.0, lengt
     if rc < 0:
         raise ValueError(lib().oracle_last_error().decode())
     return out.view(np.complex128)
+
+
+def _c1(z) -> np.ndarray:
+    return np.array([complex(z)], dtype=np.complex128).view(np.float64).copy()
+
+
+def _cn(values) -> np.ndarray:
+    return np.array([complex(z) for z in values], dtype=np.complex128).view(np.float64).copy()
+
+
+WEIGHT_FIELDS = ("F", "G", "H", "Lambda", "Pi", "B", "C", "U", "D", "T")
+
+
+def field_weights() -> dict:
+    """{field: (spin, boost, falloff)} as the oracle's pipeline uses them."""
+    out = (ctypes.c_int * 30)()
+    lib().oracle_field_weights(out)
+    return {f: (out[3 * k], out[3 * k + 1], out[3 * k + 2]) for k, f in enumerate(WEIGHT_FIELDS)}
+
+
+def eth_point(prime, spin, boost, kerr_spin, length, radius, cos_theta, sin_theta, x, dt,
+              ang) -> complex:
+    out = np.zeros(2)
+    lib().oracle_eth_point(int(prime), spin, boost, kerr_spin, length, radius, cos_theta,
+                           sin_theta, _dp(_c1(x)), _dp(_c1(dt)), _dp(_c1(ang)), _dp(out))
+    return complex(out[0], out[1])
+
+
+RECON_INPUTS = ("F", "G", "Lambda", "H", "B", "Pi", "C", "U", "Pi_s", "B_s", "C_s", "eth1F",
+                "eth2G", "eth2C", "eth2Pi", "ethp1B_s", "ethp2C_s")
+RECON_OUTPUTS = ("G", "Lambda", "H", "B", "Pi", "C", "U")
+
+
+def recon_rhs(radius, bg: dict, inputs: dict) -> dict:
+    """R_X of Delta_n X = R_X for the seven reconstruction fields."""
+    b = _cn([bg["mu0"], bg["tau0"], bg["pi0"]])
+    x = _cn([inputs.get(k, 0j) for k in RECON_INPUTS])
+    out = np.zeros(14)
+    lib().oracle_recon_rhs(radius, _dp(b), _dp(x), _dp(out))
+    return dict(zip(RECON_OUTPUTS, out.view(np.complex128)))
+
+
+def outer_source_point(radius, bg: dict, delta3_d, d, ethp3_t, t) -> complex:
+    b = _cn([bg["mu0"], bg["tau0"], bg["pi0"]])
+    x = _cn([delta3_d, d, ethp3_t, t])
+    out = np.zeros(2)
+    lib().oracle_outer_source_point(radius, _dp(b), _dp(x), _dp(out))
+    return complex(out[0], out[1])
+
+
+def coordinate_forcing(length, kerr_spin, radius, cos_theta, source) -> complex:
+    out = np.zeros(2)
+    lib().oracle_coordinate_forcing(length, kerr_spin, radius, cos_theta, _dp(_c1(source)),
+                                    _dp(out))
+    return complex(out[0], out[1])
+
+
+CONSTRAINT_INPUTS = ("F", "G", "H", "Lambda", "Pi", "B", "C", "U", "U_s", "thorn1F", "thorn2G",
+                     "ethp2G", "ethp3H")
+
+
+def constraint_terms(radius, bg: dict, inputs: dict):
+    """(left, right) pairs of the psi3 Bianchi, psi2 Bianchi and h_ll
+    reality constraints."""
+    b = _cn([bg[k] for k in BACKGROUND_NAMES])
+    x = _cn([inputs.get(k, 0j) for k in CONSTRAINT_INPUTS])
+    out = np.zeros(12)
+    lib().oracle_constraint_terms(radius, _dp(b), _dp(x), _dp(out))
+    v = out.view(np.complex128)
+    return [(v[0], v[1]), (v[2], v[3]), (v[4], v[5])]
+
+
+def thorn(value, dt, dr, n, spin, boost, m, radius, cos_theta, mass, kerr_spin, length,
+          epsilon0) -> complex:
+    out = np.zeros(2)
+    lib().oracle_thorn(_dp(_c1(value)), _dp(_c1(dt)), _dp(_c1(dr)), n, spin, boost, m, radius,
+                       cos_theta, mass, kerr_spin, length, _dp(_c1(epsilon0)), _dp(out))
+    return complex(out[0], out[1])
 
 
 def delta_n(value, dt, dr, n, radius, mass, length) -> complex:

@@ -537,16 +537,37 @@ void laplacian(const OraclePipeline* p, const Coord& c, Plane in, PlaneW out) {
     }
 }
 
+// eth_n X (prime = false) or eth'_n X (prime = true) at one point, from the
+// angular part `ang` of X (its spin-raised, or lowered, band synthesis), its
+// coordinate-time derivative and the point's (R, cos, sin): the coordinate
+// forms of ghp.hpp:60-116 with the connection weight p = s + b (q = -s + b).
+cplx eth_point(bool prime, int spin, int boost, double a, double L, double R, double ct,
+               double st, cplx x, cplx dt, cplx ang) {
+  const double L2 = L * L;
+  const double sqrt_two = std::sqrt(2.0);
+  const cplx I(0.0, 1.0);
+  if (!prime) {
+    const int pw = spin + boost;
+    const cplx den(L2, -a * R * ct);
+    const cplx deriv = (-I * a * st * dt + ang) / (sqrt_two * den);
+    const cplx conn = -I * static_cast<double>(pw) * a * R * st * x / (sqrt_two * den * den);
+    return deriv + conn;
+  }
+  const int qw = -spin + boost;
+  const cplx den(L2, a * R * ct);
+  const cplx deriv = (I * a * st * dt + ang) / (sqrt_two * den);
+  const cplx conn = I * static_cast<double>(qw) * a * R * st * x / (sqrt_two * den * den);
+  return deriv + conn;
+}
+
 // eth_n X and eth'_n X (ghp.hpp:60-116) with the angular part from the
 // spin-raised (lowered) band: analyze, multiply by the raising factor
 // sqrt((l - s)(l + s + 1)) (lowering factor -sqrt((l + s)(l - s + 1))),
 // align ell to the shifted band, synthesize (ghp_device.hpp:174-196,
-// angular.hpp:129-137).
+// angular.hpp:129-137), then the point formula above.
 void ghp(const OraclePipeline* p, const Coord& c, bool prime, Plane x, Plane dt, PlaneW out) {
   const int N = p->N, T = p->T;
-  const double a = p->p.spin, L = p->p.length, L2 = L * L;
-  const double sqrt_two = std::sqrt(2.0);
-  const cplx I(0.0, 1.0);
+  const double a = p->p.spin, L = p->p.length;
 #pragma omp parallel for collapse(2) schedule(static)
   for (int mode = 0; mode < p->M; ++mode)
     for (int r = 0; r < N; ++r) {
@@ -574,24 +595,9 @@ void ghp(const OraclePipeline* p, const Coord& c, bool prime, Plane x, Plane dt,
       std::vector<cplx> ang(T);
       synthesize(dst, shifted, T, ang.data());
       const double R = p->h * r;
-      for (int t = 0; t < T; ++t) {
-        const double st = p->sin_t[t], ct = p->cos_t[t];
-        if (!prime) {
-          const int pw = c.spin + c.boost;
-          const cplx den(L2, -a * R * ct);
-          const cplx deriv = (-I * a * st * dv[t] + ang[t]) / (sqrt_two * den);
-          const cplx conn = -I * static_cast<double>(pw) * a * R * st * xv[t] /
-                            (sqrt_two * den * den);
-          o[t] = deriv + conn;
-        } else {
-          const int qw = -c.spin + c.boost;
-          const cplx den(L2, a * R * ct);
-          const cplx deriv = (I * a * st * dv[t] + ang[t]) / (sqrt_two * den);
-          const cplx conn = I * static_cast<double>(qw) * a * R * st * xv[t] /
-                            (sqrt_two * den * den);
-          o[t] = deriv + conn;
-        }
-      }
+      for (int t = 0; t < T; ++t)
+        o[t] = eth_point(prime, c.spin, c.boost, a, L, R, p->cos_t[t], p->sin_t[t], xv[t], dv[t],
+                         ang[t]);
     }
 }
 
@@ -857,7 +863,7 @@ OraclePipeline* oracle_create(const OracleParams* params) {
         q.constraint_tolerance < 0.0 || q.required_passes < 1)
       throw std::invalid_argument("source policy is invalid");
     p->second = q.source_mode != 0;
-    p->c_first = make_coord(p, -2, -2, q.ell_max_first, p->parents, p->second);
+    p->c_first = make_coord(p, weight::F.spin, weight::F.boost, q.ell_max_first, p->parents, p->second);
     p->sharp.assign(p->M, -1);
     for (int i = 0; i < p->M; ++i) {
       const auto it = std::find(p->modes.begin(), p->modes.end(), -p->modes[i]);
@@ -870,15 +876,15 @@ OraclePipeline* oracle_create(const OracleParams* params) {
       if (E1 < 3 || E2 < 3 || q.ntheta < std::max({E1 + 1, E2 + 1, (2 * E1 + E2 + 2) / 2}))
         throw std::invalid_argument("angular bands require exact-product padding");
       // Coordinators of spatial_pipeline.hpp:185-218.
-      p->c_second = make_coord(p, -2, -2, E2, p->targets, false);
-      p->c_g = make_coord(p, -1, -1, E1, p->parents, true);
-      p->c_b = make_coord(p, -2, 0, E1, p->parents, true);
-      p->c_pi = make_coord(p, -1, 0, E1, p->parents, true);
-      p->c_c = make_coord(p, -1, 1, E1, p->parents, true);
-      p->c_scalar = make_coord(p, 0, 0, E1, p->parents, true);
-      p->c_bsharp = make_coord(p, 2, 0, E1, p->parents, true);
-      p->c_csharp = make_coord(p, 1, 1, E1, p->parents, true);
-      p->c_t = make_coord(p, -1, -2, E2, p->targets, true);
+      p->c_second = make_coord(p, weight::F.spin, weight::F.boost, E2, p->targets, false);
+      p->c_g = make_coord(p, weight::G.spin, weight::G.boost, E1, p->parents, true);
+      p->c_b = make_coord(p, weight::B.spin, weight::B.boost, E1, p->parents, true);
+      p->c_pi = make_coord(p, weight::Pi.spin, weight::Pi.boost, E1, p->parents, true);
+      p->c_c = make_coord(p, weight::C.spin, weight::C.boost, E1, p->parents, true);
+      p->c_scalar = make_coord(p, weight::H.spin, weight::H.boost, E1, p->parents, true);
+      p->c_bsharp = make_coord(p, weight::sharp(weight::B).spin, weight::sharp(weight::B).boost, E1, p->parents, true);
+      p->c_csharp = make_coord(p, weight::sharp(weight::C).spin, weight::sharp(weight::C).boost, E1, p->parents, true);
+      p->c_t = make_coord(p, weight::T.spin, weight::T.boost, E2, p->targets, true);
       // Ordered pairs (m1, m2) -> m1 + m2, target-major then m1 ascending,
       // both factors parents (modes.hpp:32-213).
       std::vector<int> sorted_targets = p->targets;
@@ -1331,6 +1337,77 @@ int oracle_ghp_row(int spin, int boost, int m, int ell_max, int ntheta, int prim
   } catch (const std::exception& e) {
     g_error = e.what();
     return -1;
+  }
+}
+
+namespace {
+cplx cz(const double* v, int k = 0) { return cplx(v[2 * k], v[2 * k + 1]); }
+void put(double* out, int k, cplx v) {
+  out[2 * k] = v.real();
+  out[2 * k + 1] = v.imag();
+}
+Background background_from(const double* bg, int count) {
+  Background b{};
+  cplx* slots[6] = {&b.mu0, &b.tau0, &b.pi0, &b.psi20, &b.rho0, &b.epsilon0};
+  for (int k = 0; k < count; ++k) *slots[k] = cz(bg, k);
+  return b;
+}
+}  // namespace
+
+void oracle_eth_point(int prime, int spin, int boost, double kerr_spin, double length,
+                      double radius, double cos_theta, double sin_theta, const double* x,
+                      const double* dt, const double* ang, double* out) {
+  put(out, 0, eth_point(prime != 0, spin, boost, kerr_spin, length, radius, cos_theta, sin_theta,
+                        cz(x), cz(dt), cz(ang)));
+}
+
+void oracle_recon_rhs(double radius, const double* bg, const double* in, double* out) {
+  const Background b = background_from(bg, 3);
+  enum { F, G, Lam, H, B, Pi, C, U, Pis, Bs, Cs, e1F, e2G, e2C, e2Pi, ep1Bs, ep2Cs };
+  auto v = [&](int k) { return cz(in, k); };
+  put(out, 0, recon_rhs_G(radius, b, v(G), v(e1F), v(F)));
+  put(out, 1, recon_rhs_Lambda(radius, b, v(Lam), v(F)));
+  put(out, 2, recon_rhs_H(radius, b, v(H), v(e2G), v(G)));
+  put(out, 3, recon_rhs_B(radius, b, v(B), v(Lam)));
+  put(out, 4, recon_rhs_Pi(radius, b, v(G), v(Lam), v(B)));
+  put(out, 5, recon_rhs_C(radius, b, v(C), v(Pi), v(B)));
+  put(out, 6, recon_rhs_U(radius, b, v(U), v(H), v(Pi), v(C), v(Pis), v(Bs), v(Cs), v(e2C),
+                          v(e2Pi), v(ep1Bs), v(ep2Cs)));
+}
+
+void oracle_outer_source_point(double radius, const double* bg, const double* in, double* out) {
+  put(out, 0, outer_source_point(radius, background_from(bg, 3), cz(in, 0), cz(in, 1), cz(in, 2),
+                                 cz(in, 3)));
+}
+
+void oracle_coordinate_forcing(double length, double kerr_spin, double radius, double cos_theta,
+                               const double* source, double* out) {
+  put(out, 0, coordinate_forcing_point(length, kerr_spin, radius, cos_theta, cz(source)));
+}
+
+void oracle_constraint_terms(double radius, const double* bg, const double* in, double* out) {
+  const Background b = background_from(bg, 6);
+  cplx tv[6];
+  constraint_terms_point(radius, b, cz(in, 0), cz(in, 1), cz(in, 2), cz(in, 3), cz(in, 4),
+                         cz(in, 5), cz(in, 6), cz(in, 7), cz(in, 8), cz(in, 9), cz(in, 10),
+                         cz(in, 11), cz(in, 12), tv);
+  for (int k = 0; k < 6; ++k) put(out, k, tv[k]);
+}
+
+void oracle_thorn(const double* value, const double* dt, const double* dr, int n, int spin,
+                  int boost, int m, double radius, double cos_theta, double mass,
+                  double kerr_spin, double length, const double* epsilon0, double* out) {
+  put(out, 0, thorn_n(cz(value), cz(dt), cz(dr), n, spin, boost, m, radius, cos_theta, mass,
+                      kerr_spin, length, cz(epsilon0)));
+}
+
+void oracle_field_weights(int* out) {
+  const FieldWeight w[10] = {weight::F, weight::G,  weight::H, weight::Lambda, weight::Pi,
+                             weight::B, weight::C, weight::U, weight::D,      weight::T};
+  for (int k = 0; k < 10; ++k) {
+    out[3 * k] = w[k].spin;
+    out[3 * k + 1] = w[k].boost;
+    out[3 * k + 2] = w[k].falloff;
   }
 }
 

@@ -172,6 +172,32 @@ void oracle_pair_source(double radius, const double* bg, const double* fields,
 int oracle_ghp_row(int spin, int boost, int m, int ell_max, int ntheta, int prime,
                    double kerr_spin, double length, double radius, const double* x,
                    const double* dt, double* out);
+/* Point forms used by the pipeline (complex values as (re, im) pairs).
+ * eth_n (prime = 0) / eth'_n (prime = 1) of X at one point from X, its time
+ * derivative and its angular part `ang` (raised / lowered synthesis). */
+void oracle_eth_point(int prime, int spin, int boost, double kerr_spin, double length,
+                      double radius, double cos_theta, double sin_theta, const double* x,
+                      const double* dt, const double* ang, double* out);
+/* R_X of Delta_n X = R_X for G, Lambda, H, B, Pi, C, U (out: 7 values);
+ * bg: mu0, tau0, pi0; in: F, G, Lambda, H, B, Pi, C, U, Pi#, B#, C#,
+ * eth_1 F, eth_2 G, eth_2 C, eth_2 Pi, eth'_1 B#, eth'_2 C# (17 values). */
+void oracle_recon_rhs(double radius, const double* bg, const double* in, double* out);
+/* S / R^3 from in: Delta_3 D, D, eth'_3 T, T; bg: mu0, tau0, pi0. */
+void oracle_outer_source_point(double radius, const double* bg, const double* in, double* out);
+/* 2 (L^4 + a^2 R^2 cos^2) * source. */
+void oracle_coordinate_forcing(double length, double kerr_spin, double radius, double cos_theta,
+                               const double* source, double* out);
+/* Constraint terms (left, right) x (psi3 Bianchi, psi2 Bianchi, h_ll
+ * reality); bg: the six background values of oracle_background; in: F, G,
+ * H, Lambda, Pi, B, C, U, U#, thorn_1 F, thorn_2 G, eth'_2 G, eth'_3 H. */
+void oracle_constraint_terms(double radius, const double* bg, const double* in, double* out);
+/* thorn_n at one point (ghp.hpp:24-58). */
+void oracle_thorn(const double* value, const double* dt, const double* dr, int n, int spin,
+                  int boost, int m, double radius, double cos_theta, double mass,
+                  double kerr_spin, double length, const double* epsilon0, double* out);
+/* (spin, boost, falloff) of F, G, H, Lambda, Pi, B, C, U, D, T (30 ints):
+ * the weights the pipeline's coordinators, Delta_n, eth_n and thorn_n use. */
+void oracle_field_weights(int* out);
 /* Delta_n and its inverse for the time derivative (complex in/out). */
 void oracle_delta_n(const double* value, const double* dt, const double* dr, int n,
                     double radius, double mass, double length, double* out);
