@@ -87,7 +87,7 @@ def _under_profiler() -> bool:
 def _timed_kernel(name: str, second_order: bool) -> bool:
     if second_order:
         return "so_" in name and "so_background" not in name and "so_reduce" not in name
-    return "teuk_stage_kernel" in name or "teuk_march_kernel" in name
+    return "teuk_stage_kernel" in name
 
 
 def measure_traffic(workload_name: str, second_order: bool, algorithmic_stage_bytes: float):
@@ -147,9 +147,6 @@ def main() -> int:
     ap.add_argument("--graph", action="store_true", help=argparse.SUPPRESS)  # the default
     ap.add_argument("--lds-budget", type=int, default=0)
     ap.add_argument("--tile-rows", type=int, default=0)
-    ap.add_argument("--segment-rows", type=int, default=-1,
-                    help="marching-kernel rows per workgroup (0: tile kernel only; "
-                         "default: the plan's)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-traffic", action="store_true",
@@ -186,8 +183,6 @@ def main() -> int:
         plan.set_lds_budget(args.lds_budget)
     if args.tile_rows:
         plan.set_tile_rows(args.tile_rows)
-    if args.segment_rows >= 0:
-        plan.set_segment_rows(args.segment_rows)
     if args.autotune:
         from teuk_kokkos_amd import autotune
         plan.initialize()

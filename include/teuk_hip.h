@@ -216,11 +216,10 @@ int teuk_plan_set_graph(teuk_plan* plan, int enable);
  * both re-tile the plan. */
 int teuk_plan_set_lds_budget(teuk_plan* plan, int bytes);
 int teuk_plan_set_tile_rows(teuk_plan* plan, int rows);
-/* Output rows per workgroup of the marching kernel, which streams radial
- * segments of the instances whose lines all have at most 16 ell and neither
- * dissipation nor the stage constraint (0, the default: the tile kernel for
- * every instance); re-tiles the plan. */
-int teuk_plan_set_segment_rows(teuk_plan* plan, int rows);
+/* Stage kernels specialised per line variant (default 1); 0 runs every line
+ * through the general kernel, which gives the same results bit for bit.
+ * Re-tiles the plan. */
+int teuk_plan_set_specialized(teuk_plan* plan, int enable);
 int teuk_plan_tiles(const teuk_plan* plan, int* tiles, int* max_lds_bytes);
 /* HIP-event timing of every stage-kernel launch on the plan's stream
  * (eager mode only); reset by enabling. */
@@ -229,14 +228,18 @@ int teuk_plan_kernel_timing(const teuk_plan* plan, double* total_ms, long long* 
 /* GPU time of whole RK4 steps (all execution groups, which run on streams of
  * their own between a fork from and a join into the plan's stream). */
 int teuk_plan_step_timing(const teuk_plan* plan, double* total_ms, long long* steps);
-/* Execution groups: the tiles of one radial scheme and ell-tile class
- * (ell_tiles = 1, 2 or 4 tiles of 16 held by the MFMA routines, 0 for the
- * segments of the marching kernel), one
+/* Execution groups: the tiles of one radial scheme, ell-tile class
+ * (ell_tiles = 1, 2 or 4 tiles of 16 held by the MFMA routines) and line
+ * variant (Kerr lines without dissipation in free-damped mode, with and
+ * without the Gram projection, run kernels specialised for them), one
  * stage-kernel launch per group and RK stage.  kernel_ms: the group's timed
  * launches since teuk_plan_set_kernel_timing(plan, 1). */
 int teuk_plan_group_count(const teuk_plan* plan, int* count);
 int teuk_plan_group_info(const teuk_plan* plan, int group, int* scheme, int* ell_tiles,
                          int* tiles, int* lds_bytes, double* kernel_ms);
+/* The group's line variant: 0 general, 1 Kerr with G = I, 2 Kerr with the
+ * Gram projection (both without dissipation, free-damped). */
+int teuk_plan_group_variant(const teuk_plan* plan, int group, int* variant);
 
 /* ---------------------------------------------------------------------
  * Device buffers for callers without a HIP runtime of their own (no

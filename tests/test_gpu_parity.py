@@ -147,29 +147,29 @@ def test_tiling_does_not_change_results():
         assert H.rel_err(o, out[0]) <= 1e-14
 
 
-@pytest.mark.parametrize("scheme", ["d4-2", "d8-4", "d10-5"])
-def test_marching_segments_match_tile_kernel_and_oracle(scheme):
-    """Narrow bands without dissipation stream through the marching kernel:
-    any segment length, from one output row per workgroup to whole lines,
-    must reproduce the tile kernel bit for bit (each output value comes from
-    the same inputs in the same order) and the oracle to the FP64 bar."""
-    params = H.small_case(spin=0.8, nr=229, radial_discretization=scheme, ellmax_first=16,
-                          ellmax_second=16, seeds=[(2, 2, 1e-3), (5, 2, 3e-4j)])
+@pytest.mark.parametrize("scheme,ell", [("d10-5", 16), ("d10-5", 24), ("d8-4", 16), ("d4-2", 24)])
+def test_specialised_stage_kernels_match_general_kernel_and_oracle(scheme, ell):
+    """Kerr lines without dissipation in free-damped mode run stage kernels
+    with the line flags fixed at compile time (G = I for ell <= 16, the Gram
+    projection at ell 24).  They must reproduce the general kernel bit for
+    bit (the same arithmetic in the same order) and the oracle to the FP64
+    bar."""
+    params = H.small_case(spin=0.8, nr=229, radial_discretization=scheme, ellmax_first=ell,
+                          ellmax_second=ell, seeds=[(2, 2, 1e-3), (5, 2, 3e-4j)])
     out = {}
-    for rows in (0, 1, 13, 64, 100, 256):
+    for special in (True, False):
         with _plan(params) as plan:
-            plan.set_segment_rows(rows)
-            kinds = {g["ell_tiles"] for g in plan.groups()}
-            assert kinds == ({0} if rows else {1})
+            plan.set_specialized(special)
+            variants = {g["variant"] for g in plan.groups()}
+            assert variants == ({2 if ell > 16 else 1} if special else {0}), variants
             plan.initialize()
             plan.advance(3)
-            out[rows] = plan.export_state()
-    for rows, o in out.items():
-        assert np.array_equal(o, out[0]), f"segment rows {rows}"
+            out[special] = plan.export_state()
+    assert np.array_equal(out[True], out[False])
     ref = H.oracle_initialized(params)
     ref.advance(3, params["final_time"] / params["steps"])
-    err = H.rel_err(out[256], ref.get_state())
-    print(f"[gpu] marching {scheme} 3 steps rel err {err:.3e}", flush=True)
+    err = H.rel_err(out[True], ref.get_state())
+    print(f"[gpu] specialised {scheme} ell {ell} 3 steps rel err {err:.3e}", flush=True)
     assert err <= TOL
 
 

@@ -81,13 +81,14 @@ SIGNATURES = {
     "teuk_plan_set_graph": (_I, [_P, _I]),
     "teuk_plan_set_lds_budget": (_I, [_P, _I]),
     "teuk_plan_set_tile_rows": (_I, [_P, _I]),
-    "teuk_plan_set_segment_rows": (_I, [_P, _I]),
     "teuk_plan_tiles": (_I, [_P, _IP, _IP]),
     "teuk_plan_set_kernel_timing": (_I, [_P, _I]),
     "teuk_plan_kernel_timing": (_I, [_P, _DP, ctypes.POINTER(ctypes.c_longlong)]),
     "teuk_plan_step_timing": (_I, [_P, _DP, ctypes.POINTER(ctypes.c_longlong)]),
     "teuk_plan_group_count": (_I, [_P, _IP]),
     "teuk_plan_group_info": (_I, [_P, _I, _IP, _IP, _IP, _IP, _DP]),
+    "teuk_plan_group_variant": (_I, [_P, _I, _IP]),
+    "teuk_plan_set_specialized": (_I, [_P, _I]),
     "teuk_device_alloc": (_I, [ctypes.c_size_t, ctypes.POINTER(_P)]),
     "teuk_device_free": (_I, [_P]),
     "teuk_memcpy": (_I, [_P, _P, ctypes.c_size_t, _I]),
@@ -398,9 +399,10 @@ class Plan:
     def set_tile_rows(self, rows: int):
         self._check(lib().teuk_plan_set_tile_rows(self._h, rows))
 
-    def set_segment_rows(self, rows: int):
-        """Output rows per marching-kernel workgroup (0: tile kernel only)."""
-        self._check(lib().teuk_plan_set_segment_rows(self._h, rows))
+    def set_specialized(self, enable: bool):
+        """Stage kernels specialised per line variant (the default), or every
+        line through the general kernel."""
+        self._check(lib().teuk_plan_set_specialized(self._h, int(enable)))
 
     def tiles(self):
         n = ctypes.c_int()
@@ -426,8 +428,10 @@ class Plan:
         return ms.value, n.value
 
     def groups(self):
-        """Execution groups (one stage launch each): scheme, ell_tiles, tiles,
-        lds_bytes and the group's timed kernel milliseconds."""
+        """Execution groups (one stage launch each): scheme, ell_tiles, line
+        variant (0 general, 1 Kerr with G = I, 2 Kerr with the Gram
+        projection), tiles, lds_bytes and the group's timed kernel
+        milliseconds."""
         n = ctypes.c_int()
         self._check(lib().teuk_plan_group_count(self._h, ctypes.byref(n)))
         out = []
@@ -436,6 +440,8 @@ class Plan:
             ms = ctypes.c_double()
             self._check(lib().teuk_plan_group_info(self._h, g, *[ctypes.byref(x) for x in v],
                                                    ctypes.byref(ms)))
-            out.append({"scheme": v[0].value, "ell_tiles": v[1].value, "tiles": v[2].value,
-                        "lds_bytes": v[3].value, "kernel_ms": ms.value})
+            var = ctypes.c_int()
+            self._check(lib().teuk_plan_group_variant(self._h, g, ctypes.byref(var)))
+            out.append({"scheme": v[0].value, "ell_tiles": v[1].value, "variant": var.value,
+                        "tiles": v[2].value, "lds_bytes": v[3].value, "kernel_ms": ms.value})
         return out

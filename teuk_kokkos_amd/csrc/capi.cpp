@@ -355,9 +355,9 @@ int teuk_plan_set_tile_rows(teuk_plan* plan, int rows) {
   return guarded(plan ? plan->error : dummy, [&] { plan_of(plan).set_tile_rows(rows); });
 }
 
-int teuk_plan_set_segment_rows(teuk_plan* plan, int rows) {
+int teuk_plan_set_specialized(teuk_plan* plan, int enable) {
   std::string dummy;
-  return guarded(plan ? plan->error : dummy, [&] { plan_of(plan).set_segment_rows(rows); });
+  return guarded(plan ? plan->error : dummy, [&] { plan_of(plan).set_specialized(enable != 0); });
 }
 
 int teuk_plan_tiles(const teuk_plan* plan, int* tiles, int* max_lds_bytes) {
@@ -422,6 +422,15 @@ int teuk_plan_group_info(const teuk_plan* plan, int group, int* scheme, int* ell
     if (tiles) *tiles = t;
     if (lds_bytes) *lds_bytes = l;
     if (kernel_ms) *kernel_ms = ms;
+  });
+}
+
+int teuk_plan_group_variant(const teuk_plan* plan, int group, int* variant) {
+  auto* p = const_cast<teuk_plan*>(plan);
+  std::string dummy;
+  return guarded(p ? p->error : dummy, [&] {
+    if (!variant) throw std::invalid_argument("null output pointer");
+    *variant = plan_of(p).group_variant(group);
   });
 }
 

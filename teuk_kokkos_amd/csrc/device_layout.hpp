@@ -117,30 +117,16 @@ struct SchemeShape {
   static constexpr int O = SCHEME == 2 ? 6 : (SCHEME == 1 ? 5 : 3);
 };
 
-// Marching stage kernel: one workgroup streams the output rows of a radial
-// segment, kMarchRows new rows per step, keeping the last kMarchKeep staged
-// rows (the stencil reach to the left) in LDS for the next step.  It serves
-// lines of one ell tile without dissipation in free-damped mode.  The rows of
-// a step are requested from HBM one step ahead into registers: kMarchPrefetch
-// complex values per thread cover kMarchRows rows of 3 x 16 coefficients.
-inline constexpr int kMarchRows = 64;
-inline constexpr int kMarchKeep = 16;
-inline constexpr int kMarchPrefetch = kMarchRows * 3 * 16 / kStageThreads;
-static_assert(kMarchPrefetch * kStageThreads == kMarchRows * 3 * 16, "whole prefetch rounds");
-static_assert(kMarchKeep >= SchemeShape<2>::C && kMarchKeep >= 2 * SchemeShape<2>::B,
-              "kept rows cover every stencil's reach to the left");
+// Stage-kernel variants: line flags fixed at compile time, or read from the
+// line descriptor (kStageGeneral: any mix of Schwarzschild / Kerr, Gram,
+// dissipation and the stage constraint).
+inline constexpr int kStageGeneral = 0;
+inline constexpr int kStageKerr = 1;      // Kerr, G = I, no dissipation, free-damped
+inline constexpr int kStageKerrGram = 2;  // Kerr, Gram projection, no dissipation, free-damped
 
-__host__ __device__ inline bool march_line(const LineDesc& L) {
-  return L.n_ell <= 16 && !L.dissipate && !L.constrained;
-}
-
-// LDS of the marching kernel: the scheme table, per-row coefficients and
-// staged rows of kMarchKeep + kMarchRows rows, and the Kerr node table.
-__host__ __device__ inline int march_lds_bytes(int n_ell, int ntheta, int kerr) {
-  const int rows = kMarchKeep + kMarchRows;
-  long long b = 8LL * kSchemeTableDoubles + 80LL * rows + 16LL * rows * 3 * n_ell;
-  if (kerr) b += 16LL * ntheta;
-  return static_cast<int>(b);
+__host__ __device__ inline int stage_variant(const LineDesc& L) {
+  if (!L.kerr || L.dissipate || L.constrained) return kStageGeneral;
+  return L.gram ? kStageKerrGram : kStageKerr;
 }
 
 // Kernel arguments of one stage launch.

@@ -15,8 +15,8 @@
 
 namespace teuk {
 
-hipError_t launch_stage(int scheme, int stage, int ub, const StageArgs& a, int n_tiles, int lds,
-                        hipStream_t s);
+hipError_t launch_stage(int scheme, int stage, int ub, int variant, const StageArgs& a,
+                        int n_tiles, int lds, hipStream_t s);
 hipError_t set_stage_lds_limit(int bytes);
 hipError_t launch_nodal_to_modal(const NodalArgs& a, const cdouble* nodal, cdouble* modal,
                                  hipStream_t s);
@@ -348,48 +348,25 @@ void Plan::build_tiles() {
         per[i].push_back(t);
       }
     }
-    // Instances whose lines all suit the marching kernel use it instead:
-    // segments of about segment_rows_ output rows per workgroup.
-    bool march = segment_rows_ > 0 && in.M > 0;
-    for (int mi = 0; mi < in.M && march; ++mi) {
-      const int li = in.line_of_mode[mi];
-      if (li >= 0 && !march_line(lines_[li])) march = false;
-    }
-    if (march && !per[i].empty()) {
-      per[i].clear();
-      inst_lds[i] = 0;
-      inst_ub[i] = 0;
-      for (int mi = 0; mi < in.M; ++mi) {
-        const int li = in.line_of_mode[mi];
-        if (li < 0) continue;
-        const LineDesc& L = lines_[li];
-        const int n = L.nr;
-        const int count = std::max(1, (n + segment_rows_ - 1) / segment_rows_);
-        const int basec = n / count, rem = n % count;
-        int t0 = 0;
-        for (int k = 0; k < count; ++k) {
-          const int len = basec + (k < rem ? 1 : 0);
-          TileDesc t{};
-          t.line = li;
-          t.t0 = t0;
-          t.t1 = t0 + len;
-          t.lo = t.mlo = std::max(0, t0 - kMarchKeep);
-          t.hi = t.mhi = std::min(n, t0 + len + kMarchKeep);
-          t.lds_bytes = march_lds_bytes(L.n_ell, L.ntheta, L.kerr);
-          inst_lds[i] = std::max(inst_lds[i], t.lds_bytes);
-          per[i].push_back(t);
-          t0 += len;
-        }
-      }
-    }
   }
-  // One launch per (scheme, ell-tile class or 0 for the marching kernel); a
-  // launch reserves its largest tile's LDS for every workgroup.
+  // One launch per (scheme, ell-tile class, line variant); a launch reserves
+  // its largest tile's LDS for every workgroup.  An instance takes a
+  // compile-time variant when all its lines share it (stage_variant).
   std::map<std::tuple<int, int, int>, std::vector<int>> buckets;
-  for (int i = 0; i < instance_count(); ++i)
-    buckets[{static_cast<int>(inst_[i].params.scheme), inst_ub[i], 0}].push_back(i);
+  for (int i = 0; i < instance_count(); ++i) {
+    const Instance& in = inst_[i];
+    int variant = -1;
+    for (int mi = 0; mi < in.M; ++mi) {
+      const int li = in.line_of_mode[mi];
+      if (li < 0) continue;
+      const int v = specialized_ ? stage_variant(lines_[li]) : kStageGeneral;
+      variant = variant < 0 || variant == v ? v : kStageGeneral;
+    }
+    buckets[{static_cast<int>(in.params.scheme), inst_ub[i], std::max(variant, 0)}].push_back(i);
+  }
   for (const auto& [key, members] : buckets) {
-    Group g{std::get<0>(key), std::get<1>(key), static_cast<int>(tiles_.size()), 0, 0};
+    Group g{std::get<0>(key), std::get<1>(key), std::get<2>(key),
+            static_cast<int>(tiles_.size()), 0, 0};
     for (int i : members) {
       Instance& in = inst_[i];
       in.scheme_group = static_cast<int>(groups_.size());
@@ -840,7 +817,8 @@ void Plan::launch_step(double dt_override) {
           e0 = timing_event();
           check(hipEventRecord(e0, st), "event");
         }
-        check(teuk::launch_stage(g.scheme, s, g.ub, a, g.count, g.lds, st), "stage kernel");
+        check(teuk::launch_stage(g.scheme, s, g.ub, g.variant, a, g.count, g.lds, st),
+              "stage kernel");
         if (timed) {
           e1 = timing_event();
           check(hipEventRecord(e1, st), "event");
@@ -978,7 +956,8 @@ void Plan::evaluate_rhs(int i, const cplx* d_stage, cplx* d_out, double t) {
   a.table = d_tables_ + g.scheme;
   a.yin = d_ya_;
   a.kout = d_k_;
-  check(teuk::launch_stage(g.scheme, 0, g.ub, a, in.tile_count, g.lds, stream_), "rhs kernel");
+  check(teuk::launch_stage(g.scheme, 0, g.ub, g.variant, a, in.tile_count, g.lds, stream_),
+        "rhs kernel");
   check(launch_fill_zero(reinterpret_cast<cdouble*>(d_out), nodal_value_count(i), stream_), "zero");
   check(launch_modal_to_nodal(na, d_k_, reinterpret_cast<cdouble*>(d_out), stream_), "synthesis");
 }
@@ -1117,17 +1096,6 @@ void Plan::set_lds_budget(int bytes) {
   }
 }
 
-void Plan::set_segment_rows(int rows) {
-  if (rows < 0) throw std::invalid_argument("segment rows must be nonnegative");
-  if (so_mode_) return;
-  segment_rows_ = rows;
-  build_tiles();
-  upload_tables();
-  if (graph_exec_) {
-    hipGraphExecDestroy(graph_exec_);
-    graph_exec_ = nullptr;
-  }
-}
 
 void Plan::set_tile_rows(int rows) {
   if (rows < 0) throw std::invalid_argument("tile rows must be nonnegative");
@@ -1139,6 +1107,22 @@ void Plan::set_tile_rows(int rows) {
     hipGraphExecDestroy(graph_exec_);
     graph_exec_ = nullptr;
   }
+}
+
+void Plan::set_specialized(bool enable) {
+  if (so_mode_) return;
+  specialized_ = enable;
+  build_tiles();
+  upload_tables();
+  if (graph_exec_) {
+    hipGraphExecDestroy(graph_exec_);
+    graph_exec_ = nullptr;
+  }
+}
+
+int Plan::group_variant(int g) const {
+  if (g < 0 || g >= group_count()) throw std::invalid_argument("group index out of range");
+  return groups_[g].variant;
 }
 
 void Plan::set_activation(int i, const SourceActivation& a) {

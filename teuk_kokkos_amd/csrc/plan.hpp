@@ -99,7 +99,10 @@ class Plan {
   void group_info(int g, int* scheme, int* ub, int* tiles, int* lds_bytes, double* kernel_ms);
   void set_lds_budget(int bytes);   // re-tiles
   void set_tile_rows(int rows);     // 0 = automatic
-  void set_segment_rows(int rows);  // marching-kernel segment length, 0 = off
+  // Compile-time line variants for the stage kernels (default on); off runs
+  // every line through the general kernel.  Re-tiles.
+  void set_specialized(bool enable);
+  int group_variant(int g) const;
   SourceActivation activation(int i) const { return inst_.at(i).activation; }
   void set_activation(int i, const SourceActivation& a);
   bool second_order() const { return so_mode_; }
@@ -155,15 +158,16 @@ class Plan {
   std::vector<double> mats_, nodes_;
   std::vector<TileDesc> tiles_;
   struct Group {
-    // ub: ell tiles of 16 per MFMA pass, 0 for the marching kernel
-    int scheme, ub, first, count, lds;
+    // ub: ell tiles of 16 per MFMA pass; variant: kStageGeneral, kStageKerr
+    // or kStageKerrGram (stage_variant)
+    int scheme, ub, variant, first, count, lds;
   };
   std::vector<Group> groups_;
   long long modal_count_ = 0;
   int max_lds_ = 0;
   int lds_budget_ = 80 * 1024;  // two workgroups per CU (160 KiB LDS)
   int tile_rows_ = 0;
-  int segment_rows_ = 0;  // marching kernel off: the tile kernel measured faster
+  bool specialized_ = true;
 
   hipStream_t stream_ = nullptr;
   LineDesc* d_lines_ = nullptr;

@@ -57,7 +57,7 @@ __global__ void teuk_fill_zero(cdouble* p, long long count) {
 
 // ------------------------------------------------------------ launchers
 #define TEUK_DECLARE_INSTANCE(S, U)                                                        \
-  hipError_t launch_stage_instance_##S##_##U(int stage, bool march, const StageArgs& a,    \
+  hipError_t launch_stage_instance_##S##_##U(int stage, int variant, const StageArgs& a,   \
                                              int n_tiles, int lds, hipStream_t s);         \
   hipError_t set_lds_limit_instance_##S##_##U(int bytes);
 TEUK_DECLARE_INSTANCE(0, 1)
@@ -72,7 +72,7 @@ TEUK_DECLARE_INSTANCE(2, 4)
 #undef TEUK_DECLARE_INSTANCE
 
 namespace {
-using LaunchFn = hipError_t (*)(int, bool, const StageArgs&, int, int, hipStream_t);
+using LaunchFn = hipError_t (*)(int, int, const StageArgs&, int, int, hipStream_t);
 using LimitFn = hipError_t (*)(int);
 // [scheme][ub class 0, 1, 2 for ub 1, 2, 4]
 constexpr LaunchFn kLaunch[3][3] = {
@@ -85,13 +85,13 @@ constexpr LimitFn kLimit[3][3] = {
     {set_lds_limit_instance_2_1, set_lds_limit_instance_2_2, set_lds_limit_instance_2_4}};
 }  // namespace
 
-// ub = ell tiles of the tile kernel (1, 2, 4); 0 = the marching kernel.
-hipError_t launch_stage(int scheme, int stage, int ub, const StageArgs& a, int n_tiles, int lds,
-                        hipStream_t s) {
-  const int uc = ub <= 1 ? 0 : (ub == 2 ? 1 : (ub == 4 ? 2 : -1));
-  if (scheme < 0 || scheme > 2 || ub < 0 || uc < 0 || stage < 0 || stage > 4)
-    return hipErrorInvalidValue;
-  return kLaunch[scheme][uc](stage, ub == 0, a, n_tiles, lds, s);
+// ub = ell tiles of the tile kernel (1, 2, 4); variant: kStageGeneral,
+// kStageKerr or kStageKerrGram.
+hipError_t launch_stage(int scheme, int stage, int ub, int variant, const StageArgs& a,
+                        int n_tiles, int lds, hipStream_t s) {
+  const int uc = ub == 1 ? 0 : (ub == 2 ? 1 : (ub == 4 ? 2 : -1));
+  if (scheme < 0 || scheme > 2 || uc < 0 || stage < 0 || stage > 4) return hipErrorInvalidValue;
+  return kLaunch[scheme][uc](stage, variant, a, n_tiles, lds, s);
 }
 
 hipError_t set_stage_lds_limit(int bytes) {

@@ -172,6 +172,19 @@ __device__ __forceinline__ void kerr_psi_group(int nl, int nt, double spin, cons
     }
 }
 
+// Stage outputs written with the nontemporal hint (one 128-bit store each):
+// measured faster for the Gram bands' group-of-16-ell store pattern and
+// slower for the narrow bands' row-major one, so rk_store takes it as a
+// parameter.  Loads stay plain (the hint slowed every class).
+typedef double double2v __attribute__((ext_vector_type(2)));
+template <bool NT>
+__device__ __forceinline__ void store_out(cdouble* p, cdouble v) {
+  if constexpr (NT)
+    __builtin_nontemporal_store(double2v{v.re, v.im}, reinterpret_cast<double2v*>(p));
+  else
+    *p = v;
+}
+
 // Classical RK4 stage combination of one value (STAGE 0: k -> kout).  The
 // accumulator carries y, so a step moves 16 state vectors through HBM:
 //   S1: acc = y + dt/6 k, Ya = y + dt/2 k      S2/S3: acc += dt/3 k,
@@ -186,7 +199,7 @@ template <int STAGE>
 __device__ __forceinline__ RkIn rk_load(const StageArgs& a, long long gi, bool need_y1 = false) {
   RkIn v{};
   if constexpr (STAGE == 1) {
-    if (need_y1) v.y0 = a.yin[gi];
+    if (need_y1) v.y0 = a.yin[gi];  // staged this launch: an L2 hit
   } else if constexpr (STAGE == 2 || STAGE == 3) {
     v.y0 = a.y[gi];
     v.acc0 = a.acc[gi];
@@ -197,19 +210,19 @@ __device__ __forceinline__ RkIn rk_load(const StageArgs& a, long long gi, bool n
 }
 
 // y1: the stage-1 input (= y).
-template <int STAGE>
+template <int STAGE, bool NT = false>
 __device__ __forceinline__ void rk_store(const StageArgs& a, long long gi, const RkIn& in,
                                          cdouble y1, cdouble k, double dt) {
   if constexpr (STAGE == 0) {
-    a.kout[gi] = k;
+    store_out<NT>(a.kout + gi, k);
   } else if constexpr (STAGE == 1) {
-    a.acc[gi] = cadd(y1, cscale(dt / 6.0, k));
-    a.yout[gi] = cadd(y1, cscale(0.5 * dt, k));
+    store_out<NT>(a.acc + gi, cadd(y1, cscale(dt / 6.0, k)));
+    store_out<NT>(a.yout + gi, cadd(y1, cscale(0.5 * dt, k)));
   } else if constexpr (STAGE == 2 || STAGE == 3) {
-    a.acc[gi] = cadd(in.acc0, cscale(dt / 3.0, k));
-    a.yout[gi] = cadd(in.y0, cscale(STAGE == 2 ? 0.5 * dt : dt, k));
+    store_out<NT>(a.acc + gi, cadd(in.acc0, cscale(dt / 3.0, k)));
+    store_out<NT>(a.yout + gi, cadd(in.y0, cscale(STAGE == 2 ? 0.5 * dt : dt, k)));
   } else {
-    a.yout[gi] = cadd(in.acc0, cscale(dt / 6.0, k));
+    store_out<NT>(a.yout + gi, cadd(in.acc0, cscale(dt / 6.0, k)));
   }
 }
 
@@ -269,7 +282,10 @@ __device__ __forceinline__ cdouble pre_projection(const TileCtx& c, int r, int l
 }
 
 // Output rows of the tile, one (row, ell) point per thread, for bands whose
-// A S is the identity to roundoff.
+// A S is the identity to roundoff.  A thread walks rows row_l, row_l + step,
+// ...; the RK inputs of its next kPlainDepth - 1 rows are in flight while a
+// row is computed.
+inline constexpr int kPlainDepth = 2;
 template <int SCHEME, int STAGE>
 __device__ __forceinline__ void plain_output(const StageArgs& a, const TileCtx& c, int tid) {
   const LineDesc& L = *c.L;
@@ -277,28 +293,36 @@ __device__ __forceinline__ void plain_output(const StageArgs& a, const TileCtx& 
   const int nl = L.n_ell, us = 3 * nl, n = L.nr;
   const int lane_l = tid % nl, row_l = tid / nl, step_l = kStageThreads / nl;
   if (row_l >= step_l) return;
-  // RK inputs requested one row ahead of their use
-  RkIn in[3], nxt[3];
+  RkIn in[kPlainDepth][3];
   auto fetch = [&](RkIn (&v)[3], int r) {
     const long long g = L.base + static_cast<long long>(r) * us + lane_l;
 #pragma unroll
     for (int f = 0; f < 3; ++f) v[f] = rk_load<STAGE>(a, g + f * nl, f == 0 && c.inplace);
   };
-  if (tile.t0 + row_l < tile.t1) fetch(in, tile.t0 + row_l);
-  for (int r = tile.t0 + row_l; r < tile.t1; r += step_l) {
-    const long long g = L.base + static_cast<long long>(r) * us + lane_l;
-    if (r + step_l < tile.t1) fetch(nxt, r + step_l);
-    const cdouble* u = c.U + (r - tile.lo) * us;
-    const cdouble kp = pre_projection<SCHEME, 0>(c, r, lane_l, u[2 * nl + lane_l]);
-    const cdouble xq = pre_projection<SCHEME, 1>(c, r, lane_l, u[2 * nl + lane_l]);
-    const cdouble xs = pre_projection<SCHEME, 2>(c, r, lane_l, u[2 * nl + lane_l]);
-    const cdouble dpt = apply_d<SCHEME>(*c.T, c.PT + lane_l, c.pts, tile.mlo, r, n, L.inv_h);
-    const cdouble pt = c.PT[(r - tile.mlo) * c.pts + lane_l];
-    rk_store<STAGE>(a, g, in[0], c.inplace ? in[0].y0 : u[lane_l], kp, c.dt);
-    rk_store<STAGE>(a, g + nl, in[1], u[nl + lane_l], cadd(dpt, xq), c.dt);
-    rk_store<STAGE>(a, g + 2 * nl, in[2], u[2 * nl + lane_l], cadd(pt, xs), c.dt);
 #pragma unroll
-    for (int f = 0; f < 3; ++f) in[f] = nxt[f];
+  for (int j = 0; j + 1 < kPlainDepth; ++j) {
+    const int r = tile.t0 + row_l + j * step_l;
+    if (r < tile.t1) fetch(in[j], r);
+  }
+  for (int r0 = tile.t0 + row_l; r0 < tile.t1; r0 += kPlainDepth * step_l) {
+#pragma unroll
+    for (int j = 0; j < kPlainDepth; ++j) {
+      const int r = r0 + j * step_l;
+      if (r >= tile.t1) break;
+      const int rf = r + (kPlainDepth - 1) * step_l;
+      if (rf < tile.t1) fetch(in[(j + kPlainDepth - 1) % kPlainDepth], rf);
+      const long long g = L.base + static_cast<long long>(r) * us + lane_l;
+      const cdouble* u = c.U + (r - tile.lo) * us;
+      const cdouble kp = pre_projection<SCHEME, 0>(c, r, lane_l, u[2 * nl + lane_l]);
+      const cdouble xq = pre_projection<SCHEME, 1>(c, r, lane_l, u[2 * nl + lane_l]);
+      const cdouble xs = pre_projection<SCHEME, 2>(c, r, lane_l, u[2 * nl + lane_l]);
+      const cdouble dpt = apply_d<SCHEME>(*c.T, c.PT + lane_l, c.pts, tile.mlo, r, n, L.inv_h);
+      const cdouble pt = c.PT[(r - tile.mlo) * c.pts + lane_l];
+      const RkIn (&v)[3] = in[j];
+      rk_store<STAGE>(a, g, v[0], c.inplace ? v[0].y0 : u[lane_l], kp, c.dt);
+      rk_store<STAGE>(a, g + nl, v[1], u[nl + lane_l], cadd(dpt, xq), c.dt);
+      rk_store<STAGE>(a, g + 2 * nl, v[2], u[2 * nl + lane_l], cadd(pt, xs), c.dt);
+    }
   }
 }
 
@@ -312,7 +336,7 @@ __device__ __forceinline__ void plain_output(const StageArgs& a, const TileCtx& 
 //   accumulator      element w: row g0 + lg + 4 w, ell 16 uo + lr,
 // so each RK load or store instruction covers 16 consecutive ell of four
 // rows.  UB = ell tiles of 16 accumulated at once.
-template <int SCHEME, int STAGE, int UB>
+template <int SCHEME, int STAGE, int UB, bool NT>
 __device__ __forceinline__ void gram_output_group(const StageArgs& a, const TileCtx& c,
                                                   const double* Gf, int g0, int lane) {
   const LineDesc& L = *c.L;
@@ -350,7 +374,9 @@ __device__ __forceinline__ void gram_output_group(const StageArgs& a, const Tile
     for (int uo = 0; uo < UB; ++uo) ar[uo] = ai[uo] = mfma_acc{0.0, 0.0, 0.0, 0.0};
     // without dissipation the Psi pre-projection terms vanish
     if (F != 2 || L.dissipate) {
-      // G (and M) fragments of block kb + 1 are in flight during block kb
+      // The G (and M) fragments of block kb + 1 are in flight during block
+      // kb.  (Unrolling this loop raised the Gram variant to 256 VGPRs and
+      // made it slower.)
       double g_cur[UB], m_cur[UB], g_nxt[UB], m_nxt[UB];
       auto load_block = [&](int kb, double (&g)[UB], double (&m)[UB]) {
 #pragma unroll
@@ -360,6 +386,7 @@ __device__ __forceinline__ void gram_output_group(const StageArgs& a, const Tile
         }
       };
       load_block(0, g_cur, m_cur);
+#pragma unroll
       for (int kb = 0; kb < K; ++kb) {
         if (kb + 1 < K) load_block(kb + 1, g_nxt, m_nxt);
         const int l = 4 * kb + lg;
@@ -402,7 +429,7 @@ __device__ __forceinline__ void gram_output_group(const StageArgs& a, const Tile
           k = cadd(k, apply_d<SCHEME>(*c.T, c.PT + l, c.pts, tile.mlo, r, n, L.inv_h));
         if constexpr (F == 2) k = cadd(k, c.PT[(r - tile.mlo) * c.pts + l]);
         const cdouble y1 = need_y1 ? in[slot][w].y0 : c.U[(r - tile.lo) * us + F * nl + l];
-        rk_store<STAGE>(a, out_index(F, uo, w), in[slot][w], y1, k, c.dt);
+        rk_store<STAGE, NT>(a, out_index(F, uo, w), in[slot][w], y1, k, c.dt);
       }
     }
   };
@@ -488,7 +515,7 @@ __device__ __forceinline__ void psi_t_phase(const StageArgs& a, const LineDesc& 
 }
 
 // Point RHS and RK combination on the output rows [t0, t1).
-template <int SCHEME, int STAGE, int UB>
+template <int SCHEME, int STAGE, int UB, int VARIANT>
 __device__ __forceinline__ void output_phase(const StageArgs& a, const LineDesc& L,
                                              const TileDesc& tile, const StageLds& s, int tid) {
   const int wave = tid >> 6, lane = tid & 63;
@@ -501,16 +528,26 @@ __device__ __forceinline__ void output_phase(const StageArgs& a, const LineDesc&
   }
   const double* Gf = a.mats + L.gfrag_off;
   for (int g0 = 16 * wave; g0 < orows; g0 += 16 * (kStageThreads / 64))
-    gram_output_group<SCHEME, STAGE, UB>(a, c, Gf, g0, lane);
+    gram_output_group<SCHEME, STAGE, UB, VARIANT == kStageKerrGram>(a, c, Gf, g0, lane);
 }
 
 // One workgroup = one radial tile of one line; UB = ell tiles of 16 the MFMA
-// routines hold at once (1, 2 or 4).
-template <int SCHEME, int STAGE, int UB>
+// routines hold at once (1, 2 or 4).  VARIANT fixes the line flags at
+// compile time for the common case (kStageGeneral keeps them at run time):
+// kStageKerr and kStageKerrGram serve Kerr lines in free-damped mode without
+// dissipation, with G = I and with the Gram projection respectively, and
+// compile every other path out.
+template <int SCHEME, int STAGE, int UB, int VARIANT>
 __global__ __launch_bounds__(kStageThreads, UB == 1 ? 3 : 2) void teuk_stage_kernel(StageArgs a) {
   extern __shared__ __align__(16) double smem[];
   const TileDesc tile = a.tiles[blockIdx.x];
-  const LineDesc L = a.lines[tile.line];
+  LineDesc L = a.lines[tile.line];
+  if constexpr (VARIANT != kStageGeneral) {
+    L.kerr = 1;
+    L.dissipate = 0;
+    L.constrained = 0;
+    L.gram = VARIANT == kStageKerrGram ? 1 : 0;
+  }
   const int tid = threadIdx.x;
   const int nl = L.n_ell;
   const int nt = L.ntheta;
@@ -547,99 +584,7 @@ __global__ __launch_bounds__(kStageThreads, UB == 1 ? 3 : 2) void teuk_stage_ker
 
   // 2. psi_t on rows [mlo, mhi); 3-4. the point RHS and the RK combination.
   psi_t_phase<SCHEME, UB>(a, L, tile, s, tid);
-  output_phase<SCHEME, STAGE, UB>(a, L, tile, s, tid);
-}
-
-// Marching form for narrow bands (n_ell <= 16) without dissipation in
-// free-damped mode: one workgroup streams the output rows [s0, s1) of one
-// line in steps of kMarchRows rows.  The LDS buffer holds kMarchKeep +
-// kMarchRows rows; after each step its last kMarchKeep rows (with their
-// psi_t, already computed) move to the front and the next rows, requested
-// from HBM into registers one step ahead, fill the rest.  Every row is read
-// once and its psi_t computed once per segment, and the loads overlap the
-// previous step's work.
-template <int SCHEME, int STAGE>
-__global__ __launch_bounds__(kStageThreads, 2) void teuk_march_kernel(StageArgs a) {
-  extern __shared__ __align__(16) double smem[];
-  // The segment's output rows are [t0, t1) of its descriptor.
-  const TileDesc seg = a.tiles[blockIdx.x];
-  const LineDesc L = a.lines[seg.line];
-  const int tid = threadIdx.x;
-  const int nl = L.n_ell, nt = L.ntheta, n = L.nr, us = 3 * nl;
-  constexpr int B = SchemeShape<SCHEME>::B;
-  constexpr int W = SchemeShape<SCHEME>::W;
-  constexpr int kRows = kMarchKeep + kMarchRows;
-
-  StageLds s{};
-  s.inplace = true;
-  s.T = reinterpret_cast<const SchemeTable*>(smem);
-  s.RC = reinterpret_cast<RowCoeffs*>(smem + kSchemeTableDoubles);
-  s.U = reinterpret_cast<cdouble*>(s.RC + kRows);
-  s.PT = s.U + kMarchKeep * us;
-  s.pts = us;
-  s.DP = s.U;  // unused: no stage constraint
-  s.NODES = reinterpret_cast<double*>(s.U + kRows * us);
-
-  // Rows [load, load_end) are brought in, kMarchRows per step.
-  const int load_end = min(n, seg.t1 + kMarchKeep);
-  int load = max(0, seg.t0 - kMarchKeep);
-  int next_out = seg.t0;
-  const cdouble* src = a.yin + L.base;
-  cdouble pre[kMarchPrefetch];
-  auto prefetch = [&](int r0) {
-    const int count = (min(r0 + kMarchRows, load_end) - r0) * us;
-    const cdouble* p = src + static_cast<long long>(r0) * us;
-#pragma unroll
-    for (int k = 0; k < kMarchPrefetch; ++k) pre[k] = p[min(tid + k * kStageThreads, count - 1)];
-  };
-  prefetch(load);
-  for (int j = tid; j < static_cast<int>(sizeof(SchemeTable) / 8); j += kStageThreads)
-    smem[j] = reinterpret_cast<const double*>(a.table)[j];
-  if (L.kerr)
-    for (int j = tid; j < 2 * nt; j += kStageThreads) s.NODES[j] = a.nodes[L.node_off + j];
-  for (;;) {
-    const int cnt = min(kMarchRows, load_end - load);
-    // buffer row b holds global row load - kMarchKeep + b
-    TileDesc t{};
-    t.line = seg.line;
-    t.lo = load - kMarchKeep;
-    t.hi = load + cnt;
-    t.mlo = load;
-    t.mhi = load + cnt;
-    {
-      const int count = cnt * us;
-      cdouble* dst = s.U + kMarchKeep * us;
-#pragma unroll
-      for (int k = 0; k < kMarchPrefetch; ++k) {
-        const int e = tid + k * kStageThreads;
-        if (e < count) dst[e] = pre[k];
-      }
-    }
-    s.rc_lo = t.lo;
-    row_coefficients(L, s.RC, t.lo, load, load + cnt, tid);
-    if (load + cnt < load_end) prefetch(load + cnt);
-    __syncthreads();
-    psi_t_phase<SCHEME, 1, false>(a, L, t, s, tid);
-    // output rows whose stencil columns are all staged
-    const int end = load + cnt;
-    int last = end == n ? seg.t1 : min(min(seg.t1, end - B), n - W);
-    last = max(last, next_out);
-    t.t0 = next_out;
-    t.t1 = last;
-    if (t.t1 > t.t0) output_phase<SCHEME, STAGE, 1>(a, L, t, s, tid);
-    next_out = last;
-    if (next_out >= seg.t1 || end >= load_end) break;
-    __syncthreads();
-    // keep the last kMarchKeep rows (their psi_t included) at the front
-    {
-      const int count = kMarchKeep * us;
-      const cdouble* from = s.U + kMarchRows * us;
-      for (int e = tid; e < count; e += kStageThreads) s.U[e] = from[e];
-      for (int j = tid; j < kMarchKeep; j += kStageThreads) s.RC[j] = s.RC[kMarchRows + j];
-    }
-    __syncthreads();
-    load = end;
-  }
+  output_phase<SCHEME, STAGE, UB, VARIANT>(a, L, tile, s, tid);
 }
 
 }  // namespace teuk
