@@ -1,0 +1,152 @@
+"""Standard linear workloads run to their stated lengths on the GPU.
+
+Two kinds of reference:
+  * the CPU oracle run on the same host, for the first hundreds to
+    thousands of steps (the FP64 bar, <= 1e-12 relative on the state);
+  * the oracle's full-length trajectories committed as fixtures
+    (tests/golden/long_runs.npz, written in the build container by
+    tests/golden/gen_long_runs.py): the Schwarzschild nr 257 D8-4 run for
+    2000 steps at CFL 0.02, the Kerr a = 0.7 ringdown (nr 1025 D10-5,
+    ell <= 16) for 20000 steps with its scri waveform every 10 steps, and the
+    near-extremal a = 0.999 run (nr 4097, ell <= 32) at 100, 1000 and 10000
+    steps.
+Two effects bound the full-length state comparison, and each bar below
+states which one applies:
+  * the Kerr ringdown carries a slowly growing discrete mode near
+    R = 0.87 R_H, so rounding differences grow by about 10x per 4000 steps
+    there while the scri waveform, which that region never reaches, stays
+    at 1e-14.  The growth belongs to the scheme: the oracle started from
+    this path's state after 4000 steps departs from the oracle's own
+    trajectory by the same 1.5e-13 at 8000 steps while staying within 4e-15
+    of this path;
+  * at ell 32 the reference's Wigner-d factorial sums (angular.hpp:170-217)
+    lose digits (|A S - I| = 1.8e-9) and their last bits depend on the host's
+    long-double arithmetic, so the fixtures made on the build host differ from
+    this host's band matrices by about 4e-10 relative from the first step;
+    the same-host comparison below is the parity check."""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+import helpers as H
+import teuk_kokkos_amd as T
+from teuk_kokkos_amd import workloads as W
+from test_oracle_known_answers import _fit_complex
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIX = np.load(os.path.join(HERE, "golden", "long_runs.npz"))
+
+
+def _scri_m2(plan):
+    recs = plan.sample_waveform(0)
+    return np.array([complex(r.rescaled_re, r.rescaled_im) for r in recs
+                     if r.order == 1 and r.boundary == 0 and r.m == 2])
+
+
+def _run(name, checkpoints, full=()):
+    """GPU run of a standard workload: the scri waveform at the fixture's
+    sampling, its radially subsampled state at `checkpoints` and the full
+    state at `full`."""
+    params = getattr(W, name)()
+    dt, every, stride, total = FIX[f"{name}_meta"]
+    every, stride, total = int(every), int(stride), int(total)
+    total = max([total] + list(checkpoints) + list(full))
+    print(f"[gpu] {name}: {total} steps, scri every {every}", flush=True)
+    scri, states, full_states = [], {}, {}
+    with T.Plan(W.config_text(**params)) as plan:
+        plan.set_graph(True)
+        plan.initialize()
+        mi = sorted(set(params["first_order_modes"]) | set(params["second_order_modes"])).index(2)
+        scri.append(_scri_m2(plan))
+        done = 0
+        while done < total:
+            plan.advance(every)
+            done += every
+            scri.append(_scri_m2(plan))
+            if done in checkpoints or done in full:
+                s = plan.export_state()
+                states[done] = s[mi, :3, ::stride, :]
+                if done in full:
+                    full_states[done] = s
+    return np.array(scri), states, full_states
+
+
+def _rel(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def _same_host_oracle(name, checkpoints):
+    """The oracle's states at the given step counts, run on this host."""
+    params = getattr(W, name)()
+    dt = params["final_time"] / params["steps"]
+    ref = H.oracle_initialized(params)
+    out, done = {}, 0
+    for n in sorted(checkpoints):
+        ref.advance(n - done, dt, time0=done * dt)
+        done = n
+        out[n] = ref.get_state()
+        print(f"[gpu] {name}: oracle at {n} steps", flush=True)
+    return out
+
+
+def test_schwarzschild_2000_steps_match_oracle_and_stay_bounded():
+    scri, states, _ = _run("case1", [2000])
+    err_w = _rel(scri, FIX["case1_scri"])
+    err_s = _rel(states[2000], FIX["case1_state_2000"])
+    print(f"[gpu] case1 2000 steps: waveform rel err {err_w:.3e}, state rel err {err_s:.3e}, "
+          f"max |state| {np.abs(states[2000]).max():.3e}", flush=True)
+    assert err_w <= 1e-10 and err_s <= 1e-12
+    assert np.isfinite(states[2000]).all() and np.abs(states[2000]).max() < 0.1
+
+
+def test_kerr_ringdown_20000_steps_waveform_and_qnm_match_oracle():
+    scri, states, full = _run("case2", [20000], full=[2000])
+    ref = _same_host_oracle("case2", [2000])
+    err_2000 = H.rel_err(full[2000], ref[2000])
+    want = FIX["case2_scri"]
+    err_w = _rel(scri, want)
+    l2 = [float(np.linalg.norm(scri[:, k] - want[:, k]) / np.linalg.norm(want[:, k]))
+          for k in (0, 1)]
+    err_s = _rel(states[20000], FIX["case2_state_20000"])
+    dt, every = FIX["case2_meta"][0], FIX["case2_meta"][1]
+    t = np.arange(scri.shape[0]) * dt * every
+    window = (30.0, 44.0)
+    f_gpu = _fit_complex(t, scri[:, 0], dt * every, *window)
+    f_ref = _fit_complex(t, want[:, 0], dt * every, *window)
+    print(f"[gpu] case2: state after 2000 steps vs same-host oracle {err_2000:.3e}; 20000 steps: "
+          f"waveform rel err {err_w:.3e}, L2 ell2/ell3 {l2[0]:.3e} {l2[1]:.3e}, state rel err "
+          f"{err_s:.3e}; QNM fit over {window}: gpu {f_gpu} oracle {f_ref}", flush=True)
+    assert err_2000 <= 1e-12
+    assert err_w <= 1e-12 and max(l2) <= 1e-12
+    # the growing interior mode (module docstring): 1e-7 at 20000 steps
+    assert err_s <= 1e-6
+    # the two-pole fit is ill-conditioned at this window (a 1e-14 change of
+    # the samples moves it by 1e-6); the late signal is dominated by the
+    # l = m = 2 mode, whose frequency lies within 2 % of Leaver's a = 0.7
+    # value 0.53260 (the 44 M run ends before the damping settles)
+    assert abs(f_gpu[0] - f_ref[0]) <= 1e-5 and abs(f_gpu[1] - f_ref[1]) <= 1e-5
+    assert abs(f_gpu[0] - 0.5326002435510186) < 0.02 * 0.5326
+
+
+def test_near_extremal_checkpoints_match_oracle():
+    scri, states, full = _run("case4", [100, 1000, 10000], full=[100, 1000])
+    errs = {}
+    ref = _same_host_oracle("case4", [100, 1000])
+    for n in (100, 1000):
+        errs[("same host", n)] = H.rel_err(full[n], ref[n])
+    for n in (100, 1000, 10000):
+        errs[("fixture", n)] = _rel(states[n], FIX[f"case4_state_{n}"])
+    err_w = _rel(scri, FIX["case4_scri"])
+    for k, v in errs.items():
+        print(f"[gpu] case4 {k[0]} after {k[1]} steps: state rel err {v:.3e}", flush=True)
+    print(f"[gpu] case4 scri waveform (every 100 steps) rel err vs fixture {err_w:.3e}", flush=True)
+    assert errs[("same host", 100)] <= 1e-12 and errs[("same host", 1000)] <= 1e-12
+    # host-dependent ell-32 band matrices (module docstring): 3.5e-9, 7.3e-9
+    # and 2.9e-10 measured at 100, 1000 and 10000 steps, 9.5e-12 at scri
+    assert max(errs[("fixture", n)] for n in (100, 1000, 10000)) <= 5e-8
+    assert err_w <= 1e-10

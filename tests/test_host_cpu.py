@@ -10,6 +10,7 @@ import re
 import numpy as np
 import pytest
 
+import helpers as H
 import oracle as O
 import teuk_kokkos_amd as T
 from teuk_kokkos_amd import workloads as W
@@ -196,3 +197,33 @@ def test_plan_creation_without_gpu_fails_loudly():
         pass
     with pytest.raises((T.TeukError, T.TeukInvalidArgument)):
         T.Plan("config_version = 1\n")
+
+
+def _so_config(**over):
+    base = dict(spin=0.6, compactification_length=1.5, nr=41, radial_discretization="d10-5",
+                ellmax_first=8, ellmax_second=12, first_order_modes=[-2, 2],
+                second_order_modes=[-4, 0, 4], initial_data__add_sharp_partner=True,
+                seeds=[(2, 2, 1e-3)], second_order__enabled=True,
+                second_order__source_mode="unrestricted", final_time=1e-9, steps=1)
+    base.update(over)
+    return W.config_text(**H.small_case(**base))
+
+
+def test_second_order_plane_past_the_32_bit_background_offsets_is_refused():
+    """The source kernel addresses a plane's background records (96 bytes
+    each) with 32-bit offsets: a plane one node past 2^32 / 96 points is
+    refused at plan creation, before any device work."""
+    limit = (1 << 32) // 96
+    nr = limit // 15 + 1  # ntheta 15
+    with pytest.raises(T.TeukInvalidArgument, match="nr x ntheta <"):
+        T.Plan(_so_config(nr=nr, ntheta=15))
+
+
+def test_second_order_source_projection_past_the_lds_limit_is_refused():
+    """Nine targets at ntheta 32 need 9 x 4 x 9 x 32 x 16 bytes of LDS in the
+    source projection, more than a workgroup's 160 KiB: refused at plan
+    creation."""
+    with pytest.raises(T.TeukInvalidArgument, match="source projection"):
+        T.Plan(_so_config(first_order_modes=[-2, -1, 0, 1, 2],
+                          second_order_modes=[-4, -3, -2, -1, 0, 1, 2, 3, 4],
+                          ellmax_second=24, ntheta=32, seeds=[(2, 2, 1e-3)]))

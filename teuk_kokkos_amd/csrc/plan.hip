@@ -95,6 +95,9 @@ Plan::Plan(const std::vector<RunParameters>& params) {
     if (p.second_order_enabled != so_mode_)
       throw std::invalid_argument(
           "a plan cannot mix first-order-only and second_order.enabled instances");
+  if (so_mode_)
+    for (const auto& in : inst_)
+      so_check_limits(in.params, static_cast<int>(in.registry.targets().size()));
   check(hipStreamCreate(&stream_), "hipStreamCreate");
   if (so_mode_) {
     so_build();
@@ -628,26 +631,18 @@ double Plan::import_check(int i, const cplx* host, double tol) {
                        hipMemcpyDeviceToHost, stream_),
         "read band check");
   check(hipStreamSynchronize(stream_), "band check");
-  // The largest relative residual over the rows.  A row whose norm is below
-  // the rounding resolution (2^-52) of the largest row of its (mode, field)
-  // carries no meaningful content and counts as zero: such rows are pure
-  // rounding remainders of cancellations (the reconstruction fields at
-  // scri), whose direction, in band or not, is set by the last bits.
+  // The largest relative residual over the rows, each row against its own
+  // norm (pipeline_bands.hpp:127-153): a row that is a rounding remainder
+  // must still be a band synthesis, which every state this library writes
+  // is (its projections are analysis then synthesis).
   double combined = 0.0;
-  for (int mf = 0; mf < M * 13 && std::isfinite(combined); ++mf) {
-    const double* rn = norms.data() + 2 * static_cast<size_t>(mf) * N;
-    double top = 0.0;
-    for (int r = 0; r < N; ++r) top = std::max(top, rn[2 * r]);
-    const double floor = top * std::numeric_limits<double>::epsilon();
-    for (int r = 0; r < N; ++r) {
-      const double line = rn[2 * r], res = rn[2 * r + 1];
-      if (!std::isfinite(line) || !std::isfinite(res)) {
-        combined = std::numeric_limits<double>::quiet_NaN();
-        break;
-      }
-      if (line <= floor) continue;
-      combined = std::max(combined, res / std::max(line, std::numeric_limits<double>::min()));
+  for (size_t row = 0; row < norms.size() / 2 && std::isfinite(combined); ++row) {
+    const double line = norms[2 * row], res = norms[2 * row + 1];
+    if (!std::isfinite(line) || !std::isfinite(res)) {
+      combined = std::numeric_limits<double>::quiet_NaN();
+      break;
     }
+    combined = std::max(combined, res / std::max(line, std::numeric_limits<double>::min()));
   }
   if (!std::isfinite(combined) || combined > tol) {
     import_release();

@@ -127,6 +127,7 @@ class Plan {
   void run_initial_data(InitArgs a, const std::vector<InitLine>& il,
                         const std::vector<cdouble>& seeds, int max_nr, int lds);
   // second-order graph (so_plan.hip)
+  static void so_check_limits(const RunParameters& p, int n_targets);
   void so_build();
   void so_free();
   void so_initialize();

@@ -182,6 +182,83 @@ __device__ __forceinline__ mfma4 row_block_op(const double* op, int T, int tile,
   return acc;
 }
 
+// Band projection of one 16-row block as analysis then synthesis
+// (angular_coordinator.hpp:125-146), both on FP64 MFMA:
+//   C^T[ell][m] = sum_u A[ell][u] X[m][u]     (ell tiles of 16, up to four)
+//   Out[m][t]   = sum_ell C[m][ell] S[t][ell]
+// Accumulator element w of the first product holds C^T[16 lt + lg + 4 w][lr],
+// which is the A operand of ell block (lt, w) of the second, so the
+// coefficients never leave registers.  `an` and `sn` are the kOpAnalyze and
+// kOpSynthesize operators (column-major like every operator).  Rows m and the
+// output layout are those of row_block_op.
+template <class AF>
+__device__ __forceinline__ mfma4 row_block_project(const double* an, const double* sn, int T,
+                                                   int n_ell, int tile, int lane, AF a_of) {
+  const int lr = lane & 15, lg = lane >> 4;
+  const int LT = (n_ell + 15) / 16;
+  mfma4 c[4];
+#pragma unroll
+  for (int lt = 0; lt < 4; ++lt) c[lt] = mfma4{0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < T; k0 += 16) {
+    double bv[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int u = k0 + 4 * q + lg;
+      bv[q] = u < T ? a_of(lr, u) : 0.0;
+    }
+#pragma unroll
+    for (int lt = 0; lt < 4; ++lt) {
+      if (lt >= LT) break;
+      const int ell = 16 * lt + lr;
+      double av[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int u = k0 + 4 * q + lg;
+        av[q] = (u < T && ell < n_ell) ? an[u * T + ell] : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (k0 + 4 * q < T) c[lt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q], bv[q], c[lt], 0, 0, 0);
+    }
+  }
+  const int tcol = 16 * tile + lr;
+  mfma4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int lt = 0; lt < 4; ++lt) {
+    if (lt >= LT) break;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      if (16 * lt + 4 * w >= n_ell) break;
+      const int ell = 16 * lt + 4 * w + lg;
+      const double b = (ell < n_ell && tcol < T) ? sn[ell * T + tcol] : 0.0;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(c[lt][w], b, acc, 0, 0, 0);
+    }
+  }
+  return acc;
+}
+
+// The same projection of one theta row at node t by one thread (rows wider
+// than an MFMA tile): the n_ell coefficients, then their synthesis at t.
+__device__ __forceinline__ cdouble row_project(const double* ops, int an_off, int sn_off,
+                                               int n_ell, const cdouble* in, int t, int T) {
+  if (an_off < 0) return cmake(0.0, 0.0);
+  const double* an = ops + an_off;
+  const double* sn = ops + sn_off;
+  double re = 0.0, im = 0.0;
+  for (int ell = 0; ell < n_ell; ++ell) {
+    double cr = 0.0, ci = 0.0;
+    for (int u = 0; u < T; ++u) {
+      const double w = an[u * T + ell];
+      cr += w * in[u].re;
+      ci += w * in[u].im;
+    }
+    const double s = sn[ell * T + t];
+    re += s * cr;
+    im += s * ci;
+  }
+  return cmake(re, im);
+}
+
 // ------------------------------------------------------------ linear triple
 // KIND 0: stage fields 0..2 -> out 0..2, D Psi -> dpsi      (first order)
 // KIND 1: out fields 0..2 -> tan 0..2, D Psi -> dpsi_t      (its tangent)
@@ -310,7 +387,8 @@ __global__ __launch_bounds__(kSoThreads) void so_linear_kernel(SoArgs a) {
   // band projection of the outputs on FP64 MFMA: the rows (output row,
   // field) are contiguous theta rows of OUT; a block takes eight of them,
   // real and imaginary parts, and the blocks go round the waves
-  const int proj_off = I.op[coord][mode][kOpProject];
+  const int an_off = I.op[coord][mode][kOpAnalyze], sn_off = I.op[coord][mode][kOpSynthesize];
+  const int n_band = I.band_n[coord][mode];
   const int lane = tid & 63, lg = lane >> 4, lr = lane & 15;
   const int crows = orows * 3, ttiles = (T + 15) / 16;
   for (int blk = tid >> 6; blk < ((crows + 7) / 8) * ttiles; blk += kSoThreads / 64) {
@@ -321,8 +399,9 @@ __global__ __launch_bounds__(kSoThreads) void so_linear_kernel(SoArgs a) {
       const cdouble v = OUT[cr * T + u];
       return (mm & 1) ? v.im : v.re;
     };
-    const mfma4 acc = proj_off < 0 ? mfma4{0.0, 0.0, 0.0, 0.0}
-                                   : row_block_op(a.ops + proj_off, T, tile_t, lane, a_of);
+    const mfma4 acc = an_off < 0 ? mfma4{0.0, 0.0, 0.0, 0.0}
+                                 : row_block_project(a.ops + an_off, a.ops + sn_off, T, n_band,
+                                                     tile_t, lane, a_of);
     const int t = 16 * tile_t + lr;
 #pragma unroll
     for (int w = 0; w < 4; ++w) {
@@ -467,7 +546,8 @@ __global__ __launch_bounds__(NT) void so_recon_kernel(SoArgs a) {
     return eth(coord, true, cconj(xr[se]), cconj(dr_row[se]), ang);
   };
   auto project = [&](int coord, const cdouble* in) {
-    return row_op<CH>(a.ops, op(coord, kOpProject), in + pi * T, t, T);
+    return row_project(a.ops, op(coord, kOpAnalyze), op(coord, kOpSynthesize),
+                       I.band_n[coord][mode], in + pi * T, t, T);
   };
   // Angular operators applied to whole LDS rows on FP64 MFMA when a theta
   // row fits one 16-wide tile: a block is (operator j, parent mode, eight
@@ -488,7 +568,9 @@ __global__ __launch_bounds__(NT) void so_recon_kernel(SoArgs a) {
     for (int blk = tid >> 6; blk < nops * np * nsb; blk += waves) {
       const int j = blk / (np * nsb), rem = blk - j * (np * nsb), p = rem / nsb;
       const int sb = rem - p * nsb;
-      const int off = I.op[coords[j]][I.parent_mode[p]][kinds[j]];
+      const int pm = I.parent_mode[p];
+      const bool two_step = kinds[j] == kOpProject;
+      const int off = I.op[coords[j]][pm][two_step ? kOpAnalyze : kinds[j]];
       const int ps = sharp[j] ? I.parent_sharp[p] : p;
       const cdouble* in = reinterpret_cast<const cdouble*>(smem) + srow[j] * MT + (ps < 0 ? 0 : ps) * T;
       cdouble* out = reinterpret_cast<cdouble*>(smem) + drow[j] * MT + p * T;
@@ -498,8 +580,11 @@ __global__ __launch_bounds__(NT) void so_recon_kernel(SoArgs a) {
         const cdouble v = in[s * kReconLdsRows * MT + u];
         return (m & 1) ? v.im : v.re;
       };
-      const mfma4 acc = off < 0 || ps < 0 ? mfma4{0.0, 0.0, 0.0, 0.0}
-                                          : row_block_op(a.ops + off, T, 0, lane, a_of);
+      mfma4 acc = {0.0, 0.0, 0.0, 0.0};
+      if (off >= 0 && ps >= 0)
+        acc = two_step ? row_block_project(a.ops + off, a.ops + I.op[coords[j]][pm][kOpSynthesize],
+                                           T, I.band_n[coords[j]][pm], 0, lane, a_of)
+                       : row_block_op(a.ops + off, T, 0, lane, a_of);
 #pragma unroll
       for (int w = 0; w < 4; ++w) {
         const int m = lg + 4 * w, s = 8 * sb + (m >> 1);
@@ -779,8 +864,9 @@ __global__ __launch_bounds__(kSoThreads) void so_source_kernel(SoArgs a, int max
   const int r = pt / T, t = pt - r * T;
   const double R = I.h * r;
   // Every access is a plane start plus the one in-plane byte offset, a
-  // 32-bit value (planes hold fewer than 2^27 points), so the per-thread
-  // part of every address is one register shared by all the loads.
+  // 32-bit value (a plane and its background records stay below 4 GiB,
+  // Plan::so_check_limits), so the per-thread part of every address is one
+  // register shared by all the loads.
   const unsigned off = static_cast<unsigned>(pt) * static_cast<unsigned>(sizeof(cdouble));
   auto at = [&](const cdouble* p) {
     return *reinterpret_cast<const cdouble*>(reinterpret_cast<const char*>(p) + off);
@@ -910,7 +996,8 @@ __global__ __launch_bounds__(kSoThreads) void so_source_projection_kernel(SoArgs
   // projections: block (ts, pair, tile), pair 0 = (D, D_t), 1 = (T, T_t)
   for (int blk = wave; blk < NT * 2 * ttiles; blk += kWaves) {
     const int ts = blk / (2 * ttiles), pr = (blk / ttiles) % 2, tile = blk % ttiles;
-    const int off = I.op[pr == 0 ? kCoSecond : kCoT][I.target_mode[ts]][kOpProject];
+    const int pc = pr == 0 ? kCoSecond : kCoT, pm = I.target_mode[ts];
+    const int off = I.op[pc][pm][kOpAnalyze];
     auto a_of = [&](int m, int u) {
       const int rr = m >> 2, c = 2 * pr + ((m >> 1) & 1);
       if (rr >= rows) return 0.0;
@@ -918,7 +1005,8 @@ __global__ __launch_bounds__(kSoThreads) void so_source_projection_kernel(SoArgs
       return (m & 1) ? v.im : v.re;
     };
     const mfma4 acc = off < 0 ? mfma4{0.0, 0.0, 0.0, 0.0}
-                              : row_block_op(a.ops + off, T, tile, lane, a_of);
+                              : row_block_project(a.ops + off, a.ops + I.op[pc][pm][kOpSynthesize],
+                                                  T, I.band_n[pc][pm], tile, lane, a_of);
     const int t = 16 * tile + lr;
 #pragma unroll
     for (int w = 0; w < 4; ++w) {

@@ -14,6 +14,8 @@
 namespace teuk {
 
 inline constexpr int kSoMaxModes = 16;
+// Dynamic LDS a workgroup can take on gfx950 (160 KiB per CU).
+inline constexpr int kMaxLdsBytes = 160 * 1024;
 inline constexpr int kSoMaxTargets = 16;
 inline constexpr int kSoMaxPairs = 64;
 inline constexpr int kSoThreads = 256;
@@ -56,8 +58,15 @@ enum SoCoord {
 // Per (coordinator, mode) T x T nodal operators (row-major, out = Op * in):
 // band projection S A, Laplacian S Lambda A, and the angular parts of eth and
 // eth': S' diag(raising or lowering factor) A with the ell alignment of the
-// spin-shifted band (ghp_device.hpp:174-196).
-enum SoOp { kOpProject = 0, kOpLaplacian, kOpRaise, kOpLower, kSoOps };
+// spin-shifted band (ghp_device.hpp:174-196).  The band projections are
+// applied as the reference applies them, analysis then synthesis
+// (angular_coordinator.hpp:125-146): kOpAnalyze holds A (n_ell x T, its rows
+// from n_ell on zero) and kOpSynthesize S (T x n_ell, its columns from n_ell
+// on zero), so a projected row is a synthesis of coefficients and stays in
+// the band to rounding relative to itself, even where it is a small
+// remainder of a much larger input.  kOpProject, the same projection as one
+// matrix S A, only names the projection in operator lists.
+enum SoOp { kOpProject = 0, kOpLaplacian, kOpRaise, kOpLower, kOpAnalyze, kOpSynthesize, kSoOps };
 
 struct SoInstance {
   long long base;   // offset of the (M, 13, N, T) block in the state buffers
@@ -84,6 +93,7 @@ struct SoInstance {
   int pair_m1[kSoMaxPairs], pair_m2[kSoMaxPairs];
   int coord_spin[kSoCoords], coord_boost[kSoCoords];
   int op[kSoCoords][kSoMaxModes][kSoOps];  // double offset, -1 where inactive
+  int band_n[kSoCoords][kSoMaxModes];      // n_ell of the coordinator's band on the mode
 };
 
 // One radial tile of one (instance, mode) for the linear triple kernel:

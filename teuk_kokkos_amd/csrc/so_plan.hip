@@ -46,6 +46,11 @@ std::array<std::vector<double>, kSoOps> nodal_operators(int spin, int m, int E, 
       out[kOpProject][t * T + u] = static_cast<double>(p);
       out[kOpLaplacian][t * T + u] = static_cast<double>(l);
     }
+  for (int k = 0; k < n; ++k)
+    for (int u = 0; u < T; ++u) {
+      out[kOpAnalyze][static_cast<size_t>(k) * T + u] = own.A[static_cast<size_t>(k) * T + u];
+      out[kOpSynthesize][static_cast<size_t>(u) * T + k] = own.S[static_cast<size_t>(u) * n + k];
+    }
   for (int dir = 0; dir < 2; ++dir) {
     const int s2 = dir == 0 ? spin + 1 : spin - 1;
     const int lmin2 = std::max(std::abs(s2), std::abs(m));
@@ -81,6 +86,22 @@ int linear_lds(int rows, int mrows, int T) {
 constexpr int kSoLdsBudget = 48 * 1024;
 
 }  // namespace
+
+// Size limits of the second-order kernels, checked before any device work:
+// the source kernel addresses a radial x theta plane, and the plane's Kerr
+// background records, with 32-bit byte offsets; the source projection keeps
+// 9 rows of kProjRows x targets x ntheta complex values in LDS.
+void Plan::so_check_limits(const RunParameters& p, int n_targets) {
+  const long long plane = static_cast<long long>(p.nr) * p.ntheta;
+  if (plane * static_cast<long long>(sizeof(SoBackground)) >= (1LL << 32))
+    throw std::invalid_argument("second-order plans need nr x ntheta < " +
+                                std::to_string((1LL << 32) / sizeof(SoBackground)));
+  const long long proj = 16LL * 9 * kProjRows * n_targets * p.ntheta;
+  if (proj > kMaxLdsBytes)
+    throw std::invalid_argument(
+        "second-order source projection needs second-order modes x ntheta <= " +
+        std::to_string(kMaxLdsBytes / (16 * 9 * kProjRows)));
+}
 
 void Plan::so_build() {
   so_ = std::make_unique<SoHost>();
@@ -151,8 +172,10 @@ void Plan::so_build() {
       I.coord_boost[c] = kCoordBoost[c];
       const bool second = c == kCoSecond || c == kCoT;
       const int E = second ? p.ell_max_second : p.ell_max_first;
-      for (int mi = 0; mi < kSoMaxModes; ++mi)
+      for (int mi = 0; mi < kSoMaxModes; ++mi) {
         for (int o = 0; o < kSoOps; ++o) I.op[c][mi][o] = -1;
+        I.band_n[c][mi] = 0;
+      }
       for (int mi = 0; mi < M; ++mi) {
         const int m = modes[mi];
         const bool active = second ? in.registry.is_target(m) : in.registry.is_parent(m);
@@ -173,6 +196,7 @@ void Plan::so_build() {
           off = it->second;
         }
         for (int o = 0; o < kSoOps; ++o) I.op[c][mi][o] = off + o * T * T;
+        I.band_n[c][mi] = E - std::max(std::abs(kCoordSpin[c]), std::abs(m)) + 1;
         // a shifted band that does not exist (ell_max below its minimum ell)
         const int s_up = kCoordSpin[c] + 1, s_dn = kCoordSpin[c] - 1;
         if (E < std::max(std::abs(s_up), std::abs(m))) I.op[c][mi][kOpRaise] = -1;
@@ -251,10 +275,8 @@ void Plan::so_build() {
     // raw sums and their projections (4 x T per target and row), and the
     // angular part of eth'_3 T (T per target and row)
     H.proj_lds = std::max(H.proj_lds, static_cast<int>(16 * 9LL * kProjRows * I.n_targets * T));
-    // the source kernel addresses a radial x theta plane with 32-bit byte
-    // offsets
-    if (static_cast<long long>(N) * T >= (1LL << 27))
-      throw std::invalid_argument("second-order plans need nr x ntheta < 2^27");
+    // sized limits of the kernels (so_check_limits), checked again here
+    so_check_limits(p, static_cast<int>(in.registry.targets().size()));
     H.max_source = std::max(H.max_source, static_cast<long long>(N) * T);
     H.max_targets = std::max(H.max_targets, I.n_targets);
     H.max_n = std::max(H.max_n, N);
