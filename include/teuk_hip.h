@@ -21,10 +21,17 @@
  *    complex128 (re, im) in (mode, field = 13, radial, theta) order with
  *    theta fastest (include/teuk/pipeline_storage.hpp:18-33,
  *    include/teuk/io.hpp:96-99).
- *  - A plan owns all device memory and one HIP stream; it allocates only in
- *    teuk_plan_create* and teuk_plan_set_* tuning calls.  Calls that take
- *    device pointers enqueue on the plan's stream and return without
- *    synchronising; teuk_sync waits.  A plan is not thread-safe.
+ *  - A plan owns all its device memory, streams and events.  It allocates
+ *    device memory and creates streams and events only in teuk_plan_create*,
+ *    the teuk_plan_set_* tuning calls and teuk_initialize (its one-time
+ *    initial-data tables); every other call allocates nothing (kernel timing
+ *    aside, whose events are made while it is on).  teuk_allocation_count
+ *    shows it, as the reference's tests check its own allocation counts
+ *    (tests/test_spatial_pipeline.cpp:102-125).
+ *  - Work is enqueued on the plan's stream, or on the caller's stream after
+ *    teuk_plan_set_stream (the reference's execution-space argument).  Calls
+ *    that take device pointers return without synchronising; teuk_sync
+ *    waits.  A plan is not thread-safe.
  *  - A batch plan holds several independent instances (each its own
  *    configuration) that advance together, one launch per RK stage.
  */
@@ -211,7 +218,19 @@ int teuk_source_pairs(const teuk_plan* plan, int instance, double* out, int capa
  * Execution tuning and measurement (no reference counterpart: they replace
  * the reference's SpatialPipelineTiming, spatial_pipeline.hpp:27-38)
  * ------------------------------------------------------------------- */
+/* Graph mode (default off): one RK4 step of every execution group is
+ * captured as a hipGraph now, and teuk_advance replays it (second-order
+ * plans keep direct launches). */
 int teuk_plan_set_graph(teuk_plan* plan, int enable);
+/* The execution-space argument of SpatialPipeline::evaluate_rhs_at_time and
+ * step (spatial_pipeline.hpp:458-462, 590): every later call enqueues on
+ * `stream` (a hipStream_t the caller owns and keeps alive while the plan
+ * uses it); NULL returns to the plan's own stream.  Waits for the work
+ * already enqueued on the previous stream. */
+int teuk_plan_set_stream(teuk_plan* plan, void* stream);
+/* Device allocations, streams and events made by the library so far (all
+ * plans and teuk_device_alloc). */
+int teuk_allocation_count(unsigned long long* count);
 /* LDS budget per workgroup (bytes) and fixed tile rows (0 = automatic);
  * both re-tile the plan. */
 int teuk_plan_set_lds_budget(teuk_plan* plan, int bytes);

@@ -89,6 +89,8 @@ SIGNATURES = {
     "teuk_plan_group_info": (_I, [_P, _I, _IP, _IP, _IP, _IP, _DP]),
     "teuk_plan_group_variant": (_I, [_P, _I, _IP]),
     "teuk_plan_set_specialized": (_I, [_P, _I]),
+    "teuk_plan_set_stream": (_I, [_P, _P]),
+    "teuk_allocation_count": (_I, [ctypes.POINTER(ctypes.c_ulonglong)]),
     "teuk_device_alloc": (_I, [ctypes.c_size_t, ctypes.POINTER(_P)]),
     "teuk_device_free": (_I, [_P]),
     "teuk_memcpy": (_I, [_P, _P, ctypes.c_size_t, _I]),
@@ -172,6 +174,15 @@ def band_matrices(spin: int, m: int, ell_max: int, ntheta: int):
     if got != nl:
         raise TeukInvalidArgument(lib().teuk_config_last_error().decode())
     return S, A
+
+
+def allocation_count() -> int:
+    """Device allocations, streams and events the library has made so far."""
+    n = ctypes.c_ulonglong()
+    rc = lib().teuk_allocation_count(ctypes.byref(n))
+    if rc != TEUK_OK:
+        _raise(rc, lib().teuk_config_last_error().decode())
+    return n.value
 
 
 class DeviceArray:
@@ -398,6 +409,12 @@ class Plan:
 
     def set_tile_rows(self, rows: int):
         self._check(lib().teuk_plan_set_tile_rows(self._h, rows))
+
+    def set_stream(self, stream: int | None):
+        """Enqueue every later call on the caller's hipStream_t (an integer
+        handle, e.g. torch.cuda.Stream().cuda_stream); None returns to the
+        plan's own stream."""
+        self._check(lib().teuk_plan_set_stream(self._h, ctypes.c_void_p(stream or 0)))
 
     def set_specialized(self, enable: bool):
         """Stage kernels specialised per line variant (the default), or every

@@ -13,6 +13,7 @@
 #include "diag_layout.hpp"
 #include "host_math.hpp"
 #include "plan.hpp"
+#include "plan_detail.hpp"
 
 struct teuk_plan {
   std::unique_ptr<teuk::Plan> plan;
@@ -345,6 +346,19 @@ int teuk_plan_set_graph(teuk_plan* plan, int enable) {
   return guarded(plan ? plan->error : dummy, [&] { plan_of(plan).set_graph(enable != 0); });
 }
 
+int teuk_plan_set_stream(teuk_plan* plan, void* stream) {
+  std::string dummy;
+  return guarded(plan ? plan->error : dummy,
+                 [&] { plan_of(plan).set_stream(static_cast<hipStream_t>(stream)); });
+}
+
+int teuk_allocation_count(unsigned long long* count) {
+  return guarded(g_config_error, [&] {
+    if (!count) throw std::invalid_argument("null output pointer");
+    *count = teuk::detail::g_allocations.load();
+  });
+}
+
 int teuk_plan_set_lds_budget(teuk_plan* plan, int bytes) {
   std::string dummy;
   return guarded(plan ? plan->error : dummy, [&] { plan_of(plan).set_lds_budget(bytes); });
@@ -440,6 +454,7 @@ int teuk_device_alloc(size_t bytes, void** ptr) {
     *ptr = nullptr;
     const hipError_t e = hipMalloc(ptr, bytes ? bytes : 1);
     if (e != hipSuccess) throw std::runtime_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+    ++teuk::detail::g_allocations;
   });
 }
 

@@ -60,6 +60,34 @@ std::array<double, kHorizonStencil> horizon_weights(int d, double h) {
 
 }  // namespace
 
+// The diagnostics tables and reduction buffers of the largest instance, and
+// for second-order plans the per-pair source buffer, made at plan creation.
+void Plan::allocate_diagnostics() {
+  long long waves = 1;
+  int m_cap = 1, t_cap = 1;
+  long long pair_values = 1;
+  for (const Instance& in : inst_) {
+    waves = std::max<long long>(waves, diag_waves(in.M, in.N, in.T));
+    m_cap = std::max(m_cap, in.M);
+    t_cap = std::max(t_cap, in.T);
+    pair_values = std::max<long long>(
+        pair_values, static_cast<long long>(in.registry.ordered_pairs().size()) * 2 * in.N * in.T);
+  }
+  d_diag_sharp_ = device_alloc<int>(m_cap);
+  d_diag_nodes_ = device_alloc<double>(2 * t_cap);
+  d_diag_partial_ = device_alloc<double>(static_cast<size_t>(waves) * 2 * kDqCount);
+  d_diag_pflags_ = device_alloc<int>(waves);
+  d_diag_out_ = device_alloc<double>(2 * kDqCount + 1);
+  diag_waves_cap_ = waves;
+  diag_m_cap_ = m_cap;
+  diag_t_cap_ = t_cap;
+  if (so_mode_) {
+    SoHost& H = *so_;
+    H.pairs = device_alloc<cdouble>(pair_values);
+    H.pairs_cap = pair_values;
+  }
+}
+
 void Plan::sample_diagnostics(int i, double time, double* out) {
   if (!std::isfinite(time) || time < 0.0)
     throw std::invalid_argument("diagnostics need a finite, nonnegative sample time");
@@ -85,22 +113,18 @@ void Plan::sample_diagnostics(int i, double time, double* out) {
     }
   }
   const int waves = diag_waves(M, N, T);
-  // small per-call tables and reduction buffers
-  int* d_sharp = device_alloc<int>(M);
-  double* d_nodes = device_alloc<double>(2 * T);
-  double* d_partial = device_alloc<double>(static_cast<size_t>(waves) * 2 * kDqCount);
-  int* d_pflags = device_alloc<int>(waves);
-  double* d_out = device_alloc<double>(2 * kDqCount + 1);
+  // tables and reduction buffers made at plan creation (allocate_diagnostics)
+  if (waves > diag_waves_cap_ || M > diag_m_cap_ || T > diag_t_cap_)
+    throw std::logic_error("diagnostics buffers smaller than the instance");
+  int* d_sharp = d_diag_sharp_;
+  double* d_nodes = d_diag_nodes_;
+  double* d_partial = d_diag_partial_;
+  int* d_pflags = d_diag_pflags_;
+  double* d_out = d_diag_out_;
   std::vector<double> red(2 * kDqCount + 1);
   std::vector<cplx> tail(static_cast<size_t>(M) * 2 * kHorizonStencil * T);
   std::vector<cplx> pairs_host;
-  auto release = [&] {
-    for (void* p : {static_cast<void*>(d_sharp), static_cast<void*>(d_nodes),
-                    static_cast<void*>(d_partial), static_cast<void*>(d_pflags),
-                    static_cast<void*>(d_out)})
-      if (p) (void)hipFree(p);
-  };
-  try {
+  {
     check(hipMemcpy(d_sharp, sharp.data(), M * sizeof(int), hipMemcpyHostToDevice), "sharp");
     check(hipMemcpy(d_nodes, nodes.data(), 2 * T * sizeof(double), hipMemcpyHostToDevice), "nodes");
     const cdouble* state = nullptr;
@@ -111,13 +135,8 @@ void Plan::sample_diagnostics(int i, double time, double* out) {
       so_upload_active();
       const long long pair_values =
           static_cast<long long>(in.registry.ordered_pairs().size()) * 2 * N * T;
-      if (pair_values > H.pairs_cap) {
-        if (H.pairs) (void)hipFree(H.pairs);
-        H.pairs = nullptr;
-        H.pairs_cap = 0;
-        H.pairs = device_alloc<cdouble>(pair_values);
-        H.pairs_cap = pair_values;
-      }
+      if (pair_values > H.pairs_cap)
+        throw std::logic_error("pair report buffer smaller than the instance");
       H.capture_pairs = true;
       try {
         so_rhs(i, 1, H.y, H.k, true);
@@ -142,8 +161,7 @@ void Plan::sample_diagnostics(int i, double time, double* out) {
     } else {
       // the nodal state (export) and its reference-layout RHS, staged in the
       // import buffers
-      import_release();
-      ensure_staging(count, 0);
+      if (count > imp_cap_) throw std::logic_error("diagnostics staging smaller than the instance");
       std::vector<cplx> host(static_cast<size_t>(count));
       export_state(i, host.data());
       check(hipMemcpy(d_imp_in_, host.data(), count * sizeof(cdouble), hipMemcpyHostToDevice),
@@ -187,11 +205,7 @@ void Plan::sample_diagnostics(int i, double time, double* out) {
               "read horizon rows");
       }
     check(hipStreamSynchronize(stream_), "diagnostics");
-  } catch (...) {
-    release();
-    throw;
   }
-  release();
 
   const int flags = static_cast<int>(red[2 * kDqCount]);
   const bool pipeline_bad = flags & (kDfState | kDfRhs | kDfFirstConstraint | kDfSecondConstraint |

@@ -98,15 +98,22 @@ Plan::Plan(const std::vector<RunParameters>& params) {
   if (so_mode_)
     for (const auto& in : inst_)
       so_check_limits(in.params, static_cast<int>(in.registry.targets().size()));
-  check(hipStreamCreate(&stream_), "hipStreamCreate");
+  check(hipStreamCreate(&own_stream_), "hipStreamCreate");
+  ++g_allocations;
+  stream_ = own_stream_;
+  fork_ = create_event(false);
   if (so_mode_) {
     so_build();
+    allocate_staging();
+    allocate_diagnostics();
     check(hipStreamSynchronize(stream_), "plan setup");
     return;
   }
   build_lines();
   build_tiles();
   upload_tables();
+  allocate_staging();
+  allocate_diagnostics();
   d_y_ = device_alloc<cdouble>(modal_count_);
   d_acc_ = device_alloc<cdouble>(modal_count_);
   d_ya_ = device_alloc<cdouble>(modal_count_);
@@ -119,6 +126,7 @@ Plan::Plan(const std::vector<RunParameters>& params) {
 
 Plan::~Plan() {
   if (stream_) hipStreamSynchronize(stream_);
+  if (own_stream_ && own_stream_ != stream_) hipStreamSynchronize(own_stream_);
   if (graph_exec_) hipGraphExecDestroy(graph_exec_);
   for (hipEvent_t e : events_) hipEventDestroy(e);
   for (hipEvent_t e : joins_) hipEventDestroy(e);
@@ -134,11 +142,13 @@ Plan::~Plan() {
                   static_cast<void*>(d_yb_), static_cast<void*>(d_k_),
                   static_cast<void*>(d_imp_in_), static_cast<void*>(d_imp_res_),
                   static_cast<void*>(d_imp_flags_), static_cast<void*>(d_imp_norms_),
-                  d_imp_rows_,
-                  static_cast<void*>(d_imp_mats_)})
+                  d_imp_rows_, static_cast<void*>(d_imp_mats_),
+                  static_cast<void*>(d_diag_sharp_), static_cast<void*>(d_diag_nodes_),
+                  static_cast<void*>(d_diag_partial_), static_cast<void*>(d_diag_pflags_),
+                  static_cast<void*>(d_diag_out_)})
     if (p) hipFree(p);
   so_free();
-  if (stream_) hipStreamDestroy(stream_);
+  if (own_stream_) hipStreamDestroy(own_stream_);
 }
 
 // S and A of one band in the operand order of the Kerr MFMA kernels (the
@@ -382,6 +392,12 @@ void Plan::build_tiles() {
     g.count = static_cast<int>(tiles_.size()) - g.first;
     groups_.push_back(g);
   }
+  // a stream and a join event per execution group beyond the first, made
+  // here (plan creation and the re-tiling set_* calls) so steps create none
+  while (side_.size() + 1 < groups_.size()) {
+    side_.push_back(create_stream());
+    joins_.push_back(create_event(false));
+  }
 }
 
 void Plan::upload_tables() {
@@ -601,10 +617,10 @@ double Plan::import_check(int i, const cplx* host, double tol) {
     }
   }
   const long long count = nodal_value_count(i);
-  ensure_staging(count, static_cast<long long>(M) * 13 * N);
-  if (!d_imp_flags_) d_imp_flags_ = device_alloc<unsigned long long>(2);
-  d_imp_rows_ = device_alloc<ImportRow>(rows.size());
-  d_imp_mats_ = device_alloc<double>(std::max<size_t>(mats.size(), 1));
+  if (count > imp_cap_ || static_cast<long long>(M) * 13 * N > imp_rows_cap_ ||
+      static_cast<long long>(rows.size()) > imp_row_tables_cap_ ||
+      static_cast<long long>(mats.size()) > imp_mats_cap_)
+    throw std::logic_error("import staging smaller than the instance (allocate_staging)");
   check(hipMemcpy(d_imp_rows_, rows.data(), rows.size() * sizeof(ImportRow),
                   hipMemcpyHostToDevice),
         "upload import bands");
@@ -651,31 +667,35 @@ double Plan::import_check(int i, const cplx* host, double tol) {
   return combined;
 }
 
-void Plan::ensure_staging(long long values, long long rows) {
-  if (values > imp_cap_) {
-    for (void* p : {static_cast<void*>(d_imp_in_), static_cast<void*>(d_imp_res_)})
-      if (p) (void)hipFree(p);
-    d_imp_in_ = d_imp_res_ = nullptr;
-    imp_cap_ = 0;
-    d_imp_in_ = device_alloc<cdouble>(values);
-    d_imp_res_ = device_alloc<cdouble>(values);
-    imp_cap_ = values;
+// Import and diagnostics staging for the largest instance, allocated once at
+// plan creation so that import, checkpoint load and the sampled diagnostics
+// allocate nothing: the nodal rows and their residuals (or RHS), per-row
+// norms, the flags, and the band tables of one instance (at most four bands
+// per mode, spins -2, -1, 0 and two band limits, each S, A and G^-1 A).
+void Plan::allocate_staging() {
+  long long values = 1, rows = 1, row_tables = 1, mats = 1;
+  for (int i = 0; i < instance_count(); ++i) {
+    const Instance& in = inst_[i];
+    values = std::max(values, nodal_value_count(i));
+    rows = std::max(rows, static_cast<long long>(in.M) * 13 * in.N);
+    row_tables = std::max(row_tables, static_cast<long long>(in.M) * 13);
+    mats = std::max(mats, static_cast<long long>(in.M) * 6 * 3 * in.T * in.T);
   }
-  if (rows > imp_rows_cap_) {
-    if (d_imp_norms_) (void)hipFree(d_imp_norms_);
-    d_imp_norms_ = nullptr;
-    imp_rows_cap_ = 0;
-    d_imp_norms_ = device_alloc<double>(2 * rows);
-    imp_rows_cap_ = rows;
-  }
+  d_imp_in_ = device_alloc<cdouble>(values);
+  d_imp_res_ = device_alloc<cdouble>(values);
+  imp_cap_ = values;
+  d_imp_norms_ = device_alloc<double>(2 * rows);
+  imp_rows_cap_ = rows;
+  d_imp_flags_ = device_alloc<unsigned long long>(2);
+  d_imp_rows_ = device_alloc<ImportRow>(row_tables);
+  imp_row_tables_cap_ = row_tables;
+  d_imp_mats_ = device_alloc<double>(mats);
+  imp_mats_cap_ = mats;
 }
 
-void Plan::import_release() {
-  for (void* p : {d_imp_rows_, static_cast<void*>(d_imp_mats_)})
-    if (p) hipFree(p);
-  d_imp_rows_ = nullptr;
-  d_imp_mats_ = nullptr;
-}
+// The staged rows of an import stay in the staging buffers until the next
+// import; nothing is freed.
+void Plan::import_release() {}
 
 double Plan::import_state(int i, const cplx* host, double tol) {
   if (so_mode_) return so_import(i, host, tol);
@@ -770,23 +790,12 @@ hipEvent_t Plan::timing_event() {
 // each runs its four stage launches on a stream of its own (forked from and
 // joined back into the plan's stream): their tails and their different
 // bottlenecks (MFMA-heavy wide bands, memory-bound narrow ones) overlap.
-void Plan::launch_step(double dt_override) {
-  const bool timed = timing_ && !use_graph_;
-  hipEvent_t step_begin = nullptr;
-  if (timed) {
-    step_begin = timing_event();
-    check(hipEventRecord(step_begin, stream_), "event");
-  }
+// With `capturing` the launches go into a hipGraph and are not timed;
+// otherwise, with kernel timing on, HIP events bracket every launch.
+void Plan::launch_step(double dt_override, bool capturing) {
+  const bool timed = timing_ && !capturing;
   const int ng = static_cast<int>(groups_.size());
-  while (static_cast<int>(side_.size()) < ng - 1) {
-    hipStream_t st;
-    check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
-    side_.push_back(st);
-    hipEvent_t e;
-    check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
-    joins_.push_back(e);
-  }
-  if (!fork_) check(hipEventCreateWithFlags(&fork_, hipEventDisableTiming), "hipEventCreate");
+  if (side_.size() + 1 < groups_.size()) throw std::logic_error("group streams missing");
   if (ng > 1) check(hipEventRecord(fork_, stream_), "fork");
   for (int gi = 0; gi < ng; ++gi) {
     const Group& g = groups_[gi];
@@ -824,11 +833,40 @@ void Plan::launch_step(double dt_override) {
     if (gi > 0) check(hipEventRecord(joins_[gi - 1], st), "join");
   }
   for (int gi = 1; gi < ng; ++gi) check(hipStreamWaitEvent(stream_, joins_[gi - 1], 0), "join wait");
-  if (timed) {
-    hipEvent_t step_end = timing_event();
-    check(hipEventRecord(step_end, stream_), "event");
-    step_events_.push_back({step_begin, step_end});
+}
+
+// HIP events around one step on the plan's stream when timing is on.
+hipEvent_t Plan::step_begin_event() {
+  if (!timing_) return nullptr;
+  hipEvent_t e = timing_event();
+  check(hipEventRecord(e, stream_), "event");
+  return e;
+}
+
+void Plan::step_end_event(hipEvent_t begin) {
+  if (!begin) return;
+  hipEvent_t e = timing_event();
+  check(hipEventRecord(e, stream_), "event");
+  step_events_.push_back({begin, e});
+}
+
+void Plan::capture_graph() {
+  if (graph_exec_) {
+    hipGraphExecDestroy(graph_exec_);
+    graph_exec_ = nullptr;
   }
+  hipGraph_t graph;
+  check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture");
+  try {
+    launch_step(0.0, true);
+  } catch (...) {
+    (void)hipStreamEndCapture(stream_, &graph);
+    throw;
+  }
+  check(hipStreamEndCapture(stream_, &graph), "end capture");
+  const hipError_t e = hipGraphInstantiate(&graph_exec_, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  check(e, "instantiate graph");
 }
 
 void Plan::record_time_step(double dt_override) {
@@ -858,7 +896,9 @@ void Plan::step(double t, double dt) {
       step_events_.push_back({e0, e1});
     }
   } else {
-    launch_step(dt);
+    hipEvent_t e0 = step_begin_event();
+    launch_step(dt, false);
+    step_end_event(e0);
   }
   for (auto& in : inst_) {
     in.time = t + dt;
@@ -891,34 +931,21 @@ void Plan::advance(int steps) {
     return;
   }
   if (use_graph_) {
-    if (!graph_exec_) {
-      hipGraph_t graph;
-      check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture");
-      launch_step(0.0);
-      check(hipStreamEndCapture(stream_, &graph), "end capture");
-      check(hipGraphInstantiate(&graph_exec_, graph, nullptr, nullptr, 0), "instantiate graph");
-      hipGraphDestroy(graph);
-    }
+    if (!graph_exec_) capture_graph();  // made by set_graph; rebuilt after a failed capture
     for (int s = 0; s < steps; ++s) {
       // with timing on, HIP events around each replay (recorded outside the
       // graph) give the GPU time of the step
-      hipEvent_t e0 = nullptr;
-      if (timing_) {
-        e0 = timing_event();
-        check(hipEventRecord(e0, stream_), "event");
-      }
+      hipEvent_t e0 = step_begin_event();
       check(hipGraphLaunch(graph_exec_, stream_), "graph launch");
-      if (timing_) {
-        hipEvent_t e1 = timing_event();
-        check(hipEventRecord(e1, stream_), "event");
-        step_events_.push_back({e0, e1});
-      }
+      step_end_event(e0);
       record_time_step(0.0);
     }
     return;
   }
   for (int s = 0; s < steps; ++s) {
-    launch_step(0.0);
+    hipEvent_t e0 = step_begin_event();
+    launch_step(0.0, false);
+    step_end_event(e0);
     record_time_step(0.0);
   }
 }
@@ -1024,12 +1051,24 @@ std::vector<WaveformRecordC> Plan::sample_waveform(int i) {
 
 void Plan::sync() { check(hipStreamSynchronize(stream_), "sync"); }
 
+// Graph mode captures one RK4 step here (and again when a set_* call
+// re-tiles), so teuk_advance only replays it.  Second-order plans take host
+// decisions between passes and keep launching directly.
 void Plan::set_graph(bool enable) {
-  use_graph_ = enable;
-  if (!enable && graph_exec_) {
+  use_graph_ = enable && !so_mode_;
+  if (graph_exec_) {
     hipGraphExecDestroy(graph_exec_);
     graph_exec_ = nullptr;
   }
+  if (use_graph_) capture_graph();
+}
+
+// The caller's stream for every later enqueue (the reference's execution
+// space instance, spatial_pipeline.hpp:458-462); null returns to the plan's
+// own stream.  Work already enqueued on the previous stream is waited for.
+void Plan::set_stream(hipStream_t s) {
+  check(hipStreamSynchronize(stream_), "stream switch");
+  stream_ = s ? s : own_stream_;
 }
 
 void Plan::set_kernel_timing(bool enable) {
@@ -1085,10 +1124,6 @@ void Plan::set_lds_budget(int bytes) {
   build_tiles();
   upload_tables();
   set_graph(use_graph_);
-  if (graph_exec_) {
-    hipGraphExecDestroy(graph_exec_);
-    graph_exec_ = nullptr;
-  }
 }
 
 
@@ -1098,10 +1133,7 @@ void Plan::set_tile_rows(int rows) {
   tile_rows_ = rows;
   build_tiles();
   upload_tables();
-  if (graph_exec_) {
-    hipGraphExecDestroy(graph_exec_);
-    graph_exec_ = nullptr;
-  }
+  set_graph(use_graph_);
 }
 
 void Plan::set_specialized(bool enable) {
@@ -1109,10 +1141,7 @@ void Plan::set_specialized(bool enable) {
   specialized_ = enable;
   build_tiles();
   upload_tables();
-  if (graph_exec_) {
-    hipGraphExecDestroy(graph_exec_);
-    graph_exec_ = nullptr;
-  }
+  set_graph(use_graph_);
 }
 
 int Plan::group_variant(int g) const {

@@ -89,6 +89,7 @@ class Plan {
   std::vector<WaveformRecordC> sample_waveform(int i);
   void sync();
   void set_graph(bool enable);
+  void set_stream(hipStream_t s);
   void set_kernel_timing(bool enable);
   void kernel_timing(double* total_ms, long long* launches);
   // GPU time of whole steps (fork of the group streams to their join).
@@ -150,7 +151,12 @@ class Plan {
   void build_lines();
   void build_tiles();
   void upload_tables();
-  void launch_step(double dt_override);
+  void launch_step(double dt_override, bool capturing);
+  void capture_graph();
+  hipEvent_t step_begin_event();
+  void step_end_event(hipEvent_t begin);
+  void allocate_staging();
+  void allocate_diagnostics();
   void record_time_step(double dt_override);
 
   std::vector<Instance> inst_;
@@ -170,7 +176,8 @@ class Plan {
   int tile_rows_ = 0;
   bool specialized_ = true;
 
-  hipStream_t stream_ = nullptr;
+  hipStream_t stream_ = nullptr;      // where work is enqueued (the caller's, or own_stream_)
+  hipStream_t own_stream_ = nullptr;
   LineDesc* d_lines_ = nullptr;
   TileDesc* d_tiles_ = nullptr;
   double* d_mats_ = nullptr;
@@ -179,14 +186,21 @@ class Plan {
   int* d_line_mode_ = nullptr;
   cdouble *d_y_ = nullptr, *d_acc_ = nullptr, *d_ya_ = nullptr, *d_yb_ = nullptr,
           *d_k_ = nullptr;
-  // import staging, grown on demand: the instance's rows, their residuals,
-  // per-row norms and the any-residual flag; the band operators of the
-  // import in progress
+  // import staging for the largest instance (allocate_staging): its rows,
+  // their residuals, per-row norms and the any-residual flag; the band
+  // operators of the import in progress
   cdouble *d_imp_in_ = nullptr, *d_imp_res_ = nullptr;
   double* d_imp_norms_ = nullptr;
   unsigned long long* d_imp_flags_ = nullptr;
-  long long imp_cap_ = 0, imp_rows_cap_ = 0;
-  void ensure_staging(long long values, long long rows);
+  long long imp_cap_ = 0, imp_rows_cap_ = 0, imp_row_tables_cap_ = 0, imp_mats_cap_ = 0;
+  // sampled-diagnostics tables and reduction buffers (allocate_diagnostics)
+  int* d_diag_sharp_ = nullptr;
+  double* d_diag_nodes_ = nullptr;
+  double* d_diag_partial_ = nullptr;
+  int* d_diag_pflags_ = nullptr;
+  double* d_diag_out_ = nullptr;
+  long long diag_waves_cap_ = 0;
+  int diag_m_cap_ = 0, diag_t_cap_ = 0;
   void* d_imp_rows_ = nullptr;
   double* d_imp_mats_ = nullptr;
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -20,12 +21,33 @@ inline void check(hipError_t e, const char* what) {
     throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// Device allocations, streams and events the library has created (plans
+// and teuk_device_alloc): teuk_allocation_count reads it, so a caller or a
+// test can see that a call allocates nothing.
+inline std::atomic<unsigned long long> g_allocations{0};
+
 template <class T>
 inline T* device_alloc(size_t count) {
   if (count == 0) count = 1;
   void* p = nullptr;
   check(hipMalloc(&p, count * sizeof(T)), "hipMalloc");
+  ++g_allocations;
   return static_cast<T*>(p);
+}
+
+inline hipStream_t create_stream() {
+  hipStream_t st = nullptr;
+  check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+  ++g_allocations;
+  return st;
+}
+
+inline hipEvent_t create_event(bool timing) {
+  hipEvent_t e = nullptr;
+  check(timing ? hipEventCreate(&e) : hipEventCreateWithFlags(&e, hipEventDisableTiming),
+        "hipEventCreate");
+  ++g_allocations;
+  return e;
 }
 
 inline SchemeTable make_table(RadialScheme s) {
