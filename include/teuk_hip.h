@@ -146,6 +146,16 @@ int teuk_state_export(teuk_plan* plan, int instance, double* host);
  * are written as zero, fields 0..2 are band-projected. */
 int teuk_evaluate_rhs(teuk_plan* plan, int instance, const void* d_stage, void* d_out,
                       double time);
+/* No counterpart in the reference.  teuk_evaluate_rhs for callers that
+ * cannot allocate device memory around the call (code under hipGraph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): the same
+ * kernels, with the modal staging in a fixed per-device array of
+ * 2 x 2^20 complex doubles (32 MiB, held until the process ends) instead of
+ * the plan's buffers.  Linear plans only; an instance whose modal lines need
+ * more than 2^20 values per buffer is refused with TEUK_EINVAL.  Calls that
+ * overlap on one device share the array and race. */
+int teuk_evaluate_rhs_noalloc(teuk_plan* plan, int instance, const void* d_stage, void* d_out,
+                              double time);
 /* SpatialPipeline::step (spatial_pipeline.hpp:590-620): one classical RK4
  * step of every instance with the given time and step. */
 int teuk_step(teuk_plan* plan, double time, double dt);

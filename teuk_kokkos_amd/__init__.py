@@ -67,6 +67,7 @@ SIGNATURES = {
     "teuk_state_import": (_I, [_P, _I, _DP, _D, _DP]),
     "teuk_state_export": (_I, [_P, _I, _DP]),
     "teuk_evaluate_rhs": (_I, [_P, _I, _P, _P, _D]),
+    "teuk_evaluate_rhs_noalloc": (_I, [_P, _I, _P, _P, _D]),
     "teuk_step": (_I, [_P, _D, _D]),
     "teuk_advance": (_I, [_P, _I]),
     "teuk_sync": (_I, [_P]),

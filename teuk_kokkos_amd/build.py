@@ -25,7 +25,7 @@ DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-p
 
 LIB_SOURCES = ["config.cpp", "host_math.cpp", "capi.cpp", "plan.hip", "stage_kernels.hip",
                "so_plan.hip", "so_kernels.hip", "init_kernels.hip", "import_kernels.hip",
-               "diag_kernels.hip", "diag_plan.hip", "checkpoint.cpp"]
+               "diag_kernels.hip", "diag_plan.hip", "checkpoint.cpp", "scratch_rhs.hip"]
 # The stage kernels, one object per (radial scheme, ell-tile class) so the
 # instantiations compile in parallel.
 STAGE_INSTANCES = [(scheme, ub) for scheme in (0, 1, 2) for ub in (1, 2, 4)]

@@ -238,6 +238,20 @@ int teuk_evaluate_rhs(teuk_plan* plan, int instance, const void* d_stage, void* 
   });
 }
 
+int teuk_evaluate_rhs_noalloc(teuk_plan* plan, int instance, const void* d_stage, void* d_out,
+                              double time) {
+  std::string dummy;
+  return guarded(plan ? plan->error : dummy, [&] {
+    teuk::Plan& pl = plan_of(plan);
+    check_instance(pl, instance);
+    if (!d_stage || !d_out) throw std::invalid_argument("null device buffer");
+    if (d_stage == d_out) throw std::invalid_argument("stage and output must not alias");
+    (void)time;  // the linear stage operator is autonomous
+    teuk::evaluate_rhs_in_scratch(pl, instance, static_cast<const teuk::cplx*>(d_stage),
+                                  static_cast<teuk::cplx*>(d_out));
+  });
+}
+
 int teuk_step(teuk_plan* plan, double time, double dt) {
   std::string dummy;
   return guarded(plan ? plan->error : dummy, [&] { plan_of(plan).step(time, dt); });

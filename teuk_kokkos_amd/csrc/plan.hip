@@ -950,9 +950,24 @@ void Plan::advance(int steps) {
   }
 }
 
+long long Plan::modal_extent(int i) const {
+  const Instance& in = inst_.at(i);
+  long long end = 0;
+  for (int line : in.line_of_mode)
+    if (line >= 0)
+      end = std::max(end, lines_[line].base + static_cast<long long>(in.N) * 3 * lines_[line].n_ell);
+  return end;
+}
+
 void Plan::evaluate_rhs(int i, const cplx* d_stage, cplx* d_out, double t) {
   (void)t;  // the stage operator is autonomous; the latch is host state
   if (so_mode_) return so_evaluate_rhs(i, d_stage, d_out);
+  evaluate_rhs_staged(i, d_stage, d_out, d_ya_, d_k_);
+}
+
+void Plan::evaluate_rhs_staged(int i, const cplx* d_stage, cplx* d_out, cdouble* ya,
+                               cdouble* k) {
+  if (so_mode_) throw std::invalid_argument("second-order plans stage their RHS in the plan");
   Instance& in = inst_.at(i);
   if (in.tile_count == 0) {
     check(launch_fill_zero(reinterpret_cast<cdouble*>(d_out), nodal_value_count(i), stream_), "zero");
@@ -967,7 +982,7 @@ void Plan::evaluate_rhs(int i, const cplx* d_stage, cplx* d_out, double t) {
     }
   NodalArgs na{d_lines_ + first_line, d_line_mode_ + first_line, d_mats_, n_lines, in.N, in.T,
                in.M};
-  check(launch_nodal_to_modal(na, reinterpret_cast<const cdouble*>(d_stage), d_ya_, stream_),
+  check(launch_nodal_to_modal(na, reinterpret_cast<const cdouble*>(d_stage), ya, stream_),
         "analysis");
   const Group& g = groups_[in.scheme_group];
   StageArgs a{};
@@ -976,12 +991,12 @@ void Plan::evaluate_rhs(int i, const cplx* d_stage, cplx* d_out, double t) {
   a.mats = d_mats_;
   a.nodes = d_nodes_;
   a.table = d_tables_ + g.scheme;
-  a.yin = d_ya_;
-  a.kout = d_k_;
+  a.yin = ya;
+  a.kout = k;
   check(teuk::launch_stage(g.scheme, 0, g.ub, g.variant, a, in.tile_count, g.lds, stream_),
         "rhs kernel");
   check(launch_fill_zero(reinterpret_cast<cdouble*>(d_out), nodal_value_count(i), stream_), "zero");
-  check(launch_modal_to_nodal(na, d_k_, reinterpret_cast<cdouble*>(d_out), stream_), "synthesis");
+  check(launch_modal_to_nodal(na, k, reinterpret_cast<cdouble*>(d_out), stream_), "synthesis");
 }
 
 // EndpointWaveformSampler::sample (waveform_output.hpp:79-160).

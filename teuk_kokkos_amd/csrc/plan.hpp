@@ -68,6 +68,8 @@ class Plan {
   const Instance& instance(int i) const { return inst_.at(i); }
   long long nodal_value_count(int i) const;
   long long modal_value_count() const { return modal_count_; }
+  // One past the last modal value of instance i's lines (0 without lines).
+  long long modal_extent(int i) const;
   double grid_points_per_step() const;    // sum M * N * T
   double modal_points_per_step() const;   // sum over lines N * n_ell
   int tile_count() const { return static_cast<int>(tiles_.size()); }
@@ -84,6 +86,10 @@ class Plan {
   double import_state(int i, const cplx* host, double relative_tolerance);
   void export_state(int i, cplx* host);
   void evaluate_rhs(int i, const cplx* d_stage, cplx* d_out, double t);
+  // The linear RHS of instance i with its modal staging in `ya` (the stage
+  // value) and `k` (the RHS), each modal_value_count() complex values on the
+  // plan's device, indexed by line base like the plan's own buffers.
+  void evaluate_rhs_staged(int i, const cplx* d_stage, cplx* d_out, cdouble* ya, cdouble* k);
   void step(double t, double dt);  // every instance, explicit dt
   void advance(int steps);         // every instance, its own dt
   std::vector<WaveformRecordC> sample_waveform(int i);
@@ -225,5 +231,10 @@ class Plan {
   struct SoHost;
   std::unique_ptr<SoHost> so_;
 };
+
+// Plan::evaluate_rhs of a linear plan with the modal staging in a fixed
+// per-device array of two scratch_rhs_capacity() halves (scratch_rhs.hip).
+void evaluate_rhs_in_scratch(Plan& plan, int i, const cplx* d_stage, cplx* d_out);
+long long scratch_rhs_capacity();
 
 }  // namespace teuk
