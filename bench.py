@@ -53,29 +53,62 @@ def workload(name: str):
     raise SystemExit(f"unknown workload {name}")
 
 
+def _host_cpu():
+    """Host CPU model (as lscpu prints it) and the logical CPUs this process
+    may run on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    return model, usable
+
+
 def cpu_baseline(cases, target_seconds: float):
-    """Oracle (nodal FP64 C++/OpenMP restatement) on a bounded sample."""
+    """Oracle (nodal FP64 C++/OpenMP restatement) on a bounded sample: timed
+    with every host core this job may use (OMP_NUM_THREADS when the host sets
+    it, else the CPUs in the affinity mask) and with one thread."""
     sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests"))
     import helpers as H
-    threads = min(16, os.cpu_count() or 1)
+    model, usable = _host_cpu()
+    env_threads = os.environ.get("OMP_NUM_THREADS", "")
+    threads = int(env_threads) if env_threads.isdigit() and int(env_threads) > 0 else usable
     sample = cases[:8]
-    pipes = [H.oracle_initialized(c, threads=threads) for c in sample]
     dts = [c["final_time"] / c["steps"] for c in sample]
     pts = sum(len(set(c["first_order_modes"]) | set(c["second_order_modes"])) * c["nr"] *
               c["ntheta"] for c in sample)
-    t0 = time.perf_counter()
-    for p, dt in zip(pipes, dts):
-        p.advance(1, dt)
-    one = time.perf_counter() - t0
-    steps = max(1, int(target_seconds / max(one, 1e-6)))
-    t0 = time.perf_counter()
-    for p, dt in zip(pipes, dts):
-        p.advance(steps, dt)
-    sec = time.perf_counter() - t0
-    return {"value": pts * steps / sec, "unit": "grid-point-updates/s", "cores": threads,
-            "kind": "port",
+
+    def rate(n_threads, seconds):
+        pipes = [H.oracle_initialized(c, threads=n_threads) for c in sample]
+        t0 = time.perf_counter()
+        for p, dt in zip(pipes, dts):
+            p.advance(1, dt)
+        one = time.perf_counter() - t0
+        steps = max(1, int(seconds / max(one, 1e-6)))
+        t0 = time.perf_counter()
+        for p, dt in zip(pipes, dts):
+            p.advance(steps, dt)
+        sec = time.perf_counter() - t0
+        return pts * steps / sec, steps, sec
+
+    value, steps, sec = rate(threads, target_seconds)
+    single, steps1, sec1 = rate(1, target_seconds / 2)
+    return {"value": value, "unit": "grid-point-updates/s", "cores": threads, "kind": "port",
             "sample": f"first {len(sample)} instances of the workload, {steps} RK4 steps each, "
-                      f"{sec:.1f} s"}
+                      f"{sec:.1f} s on {threads} threads",
+            "single_thread": {"value": single, "cores": 1,
+                              "sample": f"same instances, {steps1} RK4 steps each, "
+                                        f"{sec1:.1f} s"},
+            "host": {"cpu_model": model, "nproc": os.cpu_count(), "usable_cpus": usable,
+                     "omp_num_threads": env_threads or None}}
 
 
 def _under_profiler() -> bool:
@@ -135,6 +168,80 @@ def measure_traffic(workload_name: str, second_order: bool, algorithmic_stage_by
                             "write": total["WRITE_SIZE_scale"]}}
 
 
+def summarize_trace(trace_dir: str, bench_line: str) -> str:
+    """Markdown summary of a `rocprofv3 --kernel-trace --stats --output-format
+    csv` run of this script: per-kernel statistics, and over the timed steps
+    (the last steps x 4 x groups stage-kernel dispatches) the per-step kernel
+    time (sum of durations), busy time (union of the dispatch intervals: the
+    groups overlap) and span, set against the bench line of the same run."""
+    line = json.loads(open(bench_line).read().strip().splitlines()[-1])
+    traces = glob.glob(os.path.join(trace_dir, "**", "*kernel_trace.csv"), recursive=True)
+    if not traces:
+        raise SystemExit(f"no kernel_trace.csv under {trace_dir}")
+    rows = []
+    for r in csv.DictReader(open(traces[0])):
+        rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"]))
+    rows.sort()
+    stats = {}
+    for t0, t1, name in rows:
+        st = stats.setdefault(name, [0, 0.0, float("inf"), 0.0])
+        d = (t1 - t0) / 1e3
+        st[0] += 1
+        st[1] += d
+        st[2] = min(st[2], d)
+        st[3] = max(st[3], d)
+    total = sum(v[1] for v in stats.values())
+    out = ["| kernel | calls | avg us | min us | max us | % |", "|---|---|---|---|---|---|"]
+    for name, (n, tot, lo, hi) in sorted(stats.items(), key=lambda kv: -kv[1][1]):
+        out.append(f"| `{name}` | {n} | {tot / n:.1f} | {lo:.1f} | {hi:.1f} | "
+                   f"{100 * tot / total:.1f} |")
+    steps = int(line["steps"])
+    rl = line.get("roofline") or {}
+    groups = [g for g in rl.get("groups", []) if g.get("tiles", 0) > 0]
+    n_groups = max(1, len(groups))
+    stage = [r for r in rows if _timed_kernel(r[2], line["config"].get("modal_points_per_step",
+                                                                        0) == 0)]
+    timed = stage[-steps * 4 * n_groups:]
+    busy, cur0, cur1 = 0, None, None
+    for t0, t1, _ in timed:
+        if cur1 is None or t0 > cur1:
+            if cur1 is not None:
+                busy += cur1 - cur0
+            cur0, cur1 = t0, t1
+        else:
+            cur1 = max(cur1, t1)
+    if cur1 is not None:
+        busy += cur1 - cur0
+    sum_ms = sum(t1 - t0 for t0, t1, _ in timed) / 1e6 / steps
+    busy_ms = busy / 1e6 / steps
+    span_ms = (timed[-1][1] - timed[0][0]) / 1e6 / steps if timed else float("nan")
+    modal = line["config"].get("modal_points_per_step", 0)
+    summary = [
+        f"Timed region: the last {len(timed)} stage-kernel dispatches ({steps} steps x 4 RK "
+        f"stages x {n_groups} execution groups).",
+        "",
+        "| per RK4 step | ms |",
+        "|---|---|",
+        f"| bench line `ms_per_step` (host clock, this profiled run) | "
+        f"{line['ms_per_step']:.4f} |",
+        f"| bench line step time from HIP events (4 x `avg_stage_ms`) | "
+        f"{4 * rl.get('avg_stage_ms', float('nan')):.4f} |",
+        f"| trace: span from the first dispatch's start to the last's end | {span_ms:.4f} |",
+        f"| trace: busy time (union of dispatch intervals) | {busy_ms:.4f} |",
+        f"| trace: sum of dispatch durations (groups overlap) | {sum_ms:.4f} |",
+    ]
+    if modal:
+        ach = BYTES_PER_MODAL_STEP * modal / (busy_ms * 1e-3) / 1e9
+        summary += ["",
+                    f"Algorithmic bytes per step: {BYTES_PER_MODAL_STEP} B x {modal:,.0f} modal "
+                    f"coefficients = {BYTES_PER_MODAL_STEP * modal / 1e9:.3f} GB.  From the "
+                    f"trace's busy time: {ach:.0f} GB/s = {ach / HBM_PEAK_GBS:.3f} of "
+                    f"{HBM_PEAK_GBS:.0f} GB/s; the bench line reports "
+                    f"{rl.get('achieved', float('nan')):.0f} GB/s = "
+                    f"{rl.get('frac', float('nan')):.3f}."]
+    return "\n".join(summary + ["", "## Kernel statistics (kernel_trace.csv)", ""] + out) + "\n"
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -152,11 +259,23 @@ def main() -> int:
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 --pmc child runs that measure HBM traffic")
     ap.add_argument("--autotune", action="store_true",
-                    help="sweep LDS budget x tile rows before timing (cached per workload)")
+                    help="tune every execution group's tile shape before timing and store the "
+                         "pick in the tuning cache (by default a cached pick is applied)")
+    ap.add_argument("--no-tuning", action="store_true",
+                    help="run the default tile shapes, ignoring the tuning cache")
+    ap.add_argument("--tuning-cache", default=None,
+                    help="tuning cache file (default: the package's tuning_cache.json)")
+    ap.add_argument("--summarize-trace", default=None, metavar="DIR",
+                    help="print a markdown summary of a rocprofv3 kernel trace of this script "
+                         "(with --bench-line, the JSON line of the same run) and exit")
+    ap.add_argument("--bench-line", default=None)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this size (a counter "
                          "calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
+    if args.summarize_trace:
+        print(summarize_trace(args.summarize_trace, args.bench_line), end="")
+        return 0
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (got --gpus "
               f"{args.gpus})", file=sys.stderr)
@@ -183,12 +302,18 @@ def main() -> int:
         plan.set_lds_budget(args.lds_budget)
     if args.tile_rows:
         plan.set_tile_rows(args.tile_rows)
-    if args.autotune:
+    tuning_source = "default"
+    if plan.modal_points > 0 and not args.no_tuning and not (args.lds_budget or args.tile_rows):
         from teuk_kokkos_amd import autotune
-        plan.initialize()
-        best = autotune.tune(plan, W.texts(cases), log=lambda m: print(m, file=sys.stderr,
-                                                                      flush=True))
-        print(f"[bench] autotuned: {best}", file=sys.stderr, flush=True)
+        cache_path = args.tuning_cache or autotune.DEFAULT_CACHE
+        if args.autotune:
+            plan.initialize()
+            best = autotune.tune(plan, W.texts(cases), cache_path=cache_path, use_cache=False,
+                                 log=lambda m: print(m, file=sys.stderr, flush=True))
+            tuning_source = "autotune"
+            print(f"[bench] autotuned: {best}", file=sys.stderr, flush=True)
+        elif autotune.apply_cached(plan, W.texts(cases), cache_path) is not None:
+            tuning_source = "cache"
     t0 = time.perf_counter()
     plan.initialize()
     print(f"[bench] initial data {time.perf_counter() - t0:.1f} s; tiles {plan.tiles()}",
@@ -253,7 +378,12 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": desc, "instances": len(cases), "grid_points_per_step": grid,
                    "modal_points_per_step": modal, "precision": "fp64",
-                   "graph": bool(use_graph)},
+                   "graph": bool(use_graph),
+                   "tuning": {"source": tuning_source,
+                              "shapes": {f"{g['scheme']}/{g['ell_tiles']}/{g['variant']}":
+                                         {k: g[k] for k in ("lds_budget", "tile_rows",
+                                                            "tiles", "lds_bytes")}
+                                         for g in groups}}},
         "modal_updates_per_s": modal * args.steps / seconds,
         "roofline": roofline,
         "cpu_baseline": None,

@@ -269,6 +269,13 @@ int teuk_plan_group_info(const teuk_plan* plan, int group, int* scheme, int* ell
 /* The group's line variant: 0 general, 1 Kerr with G = I, 2 Kerr with the
  * Gram projection (both without dissipation, free-damped). */
 int teuk_plan_group_variant(const teuk_plan* plan, int group, int* variant);
+/* Tile shape of the execution group keyed (scheme, ell_tiles, variant), now
+ * and after later re-tiling: LDS budget per workgroup and tile rows (0 = the
+ * plan's setting).  Re-tiles the plan; results do not depend on the shape.
+ * teuk_plan_group_shape reports the shape set for group `group`. */
+int teuk_plan_set_group_shape(teuk_plan* plan, int scheme, int ell_tiles, int variant,
+                              int lds_budget, int tile_rows);
+int teuk_plan_group_shape(const teuk_plan* plan, int group, int* lds_budget, int* tile_rows);
 
 /* ---------------------------------------------------------------------
  * Device buffers for callers without a HIP runtime of their own (no

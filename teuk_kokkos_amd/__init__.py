@@ -90,6 +90,8 @@ SIGNATURES = {
     "teuk_plan_group_info": (_I, [_P, _I, _IP, _IP, _IP, _IP, _DP]),
     "teuk_plan_group_variant": (_I, [_P, _I, _IP]),
     "teuk_plan_set_specialized": (_I, [_P, _I]),
+    "teuk_plan_set_group_shape": (_I, [_P, _I, _I, _I, _I, _I]),
+    "teuk_plan_group_shape": (_I, [_P, _I, _IP, _IP]),
     "teuk_plan_set_stream": (_I, [_P, _P]),
     "teuk_allocation_count": (_I, [ctypes.POINTER(ctypes.c_ulonglong)]),
     "teuk_device_alloc": (_I, [ctypes.c_size_t, ctypes.POINTER(_P)]),
@@ -460,6 +462,17 @@ class Plan:
                                                    ctypes.byref(ms)))
             var = ctypes.c_int()
             self._check(lib().teuk_plan_group_variant(self._h, g, ctypes.byref(var)))
+            sh = [ctypes.c_int() for _ in range(2)]
+            self._check(lib().teuk_plan_group_shape(self._h, g, *[ctypes.byref(x) for x in sh]))
             out.append({"scheme": v[0].value, "ell_tiles": v[1].value, "variant": var.value,
-                        "tiles": v[2].value, "lds_bytes": v[3].value, "kernel_ms": ms.value})
+                        "tiles": v[2].value, "lds_bytes": v[3].value, "kernel_ms": ms.value,
+                        "lds_budget": sh[0].value, "tile_rows": sh[1].value})
         return out
+
+    def set_group_shape(self, scheme: int, ell_tiles: int, variant: int, lds_budget: int = 0,
+                        tile_rows: int = 0):
+        """Tile shape of the execution group keyed (scheme, ell_tiles,
+        variant): LDS budget and tile rows (0 = the plan's setting).
+        Re-tiles; results do not depend on the shape."""
+        self._check(lib().teuk_plan_set_group_shape(self._h, scheme, ell_tiles, variant,
+                                                    lds_budget, tile_rows))

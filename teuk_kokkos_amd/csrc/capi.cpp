@@ -462,6 +462,27 @@ int teuk_plan_group_variant(const teuk_plan* plan, int group, int* variant) {
   });
 }
 
+int teuk_plan_set_group_shape(teuk_plan* plan, int scheme, int ell_tiles, int variant,
+                              int lds_budget, int tile_rows) {
+  std::string dummy;
+  return guarded(plan ? plan->error : dummy, [&] {
+    teuk::Plan::GroupShape shape;
+    shape.lds_budget = lds_budget;
+    shape.tile_rows = tile_rows;
+    plan_of(plan).set_group_shape(scheme, ell_tiles, variant, shape);
+  });
+}
+
+int teuk_plan_group_shape(const teuk_plan* plan, int group, int* lds_budget, int* tile_rows) {
+  auto* p = const_cast<teuk_plan*>(plan);
+  std::string dummy;
+  return guarded(p ? p->error : dummy, [&] {
+    const teuk::Plan::GroupShape s = plan_of(p).group_shape(group);
+    if (lds_budget) *lds_budget = s.lds_budget;
+    if (tile_rows) *tile_rows = s.tile_rows;
+  });
+}
+
 int teuk_device_alloc(size_t bytes, void** ptr) {
   return guarded(g_config_error, [&] {
     if (!ptr) throw std::invalid_argument("null output pointer");

@@ -278,19 +278,40 @@ void Plan::build_tiles() {
   max_lds_ = 0;
   long long total_rows = 0;
   for (const auto& L : lines_) total_rows += L.nr;
-  // Enough workgroups to fill 256 CUs several times over.
-  const int parallel_rows = static_cast<int>(std::max<long long>(16, total_rows / 1024));
+  // Enough workgroups to fill 256 CUs several times over, but no tile under
+  // 64 rows: below that the stencil halo's psi_t rows and re-read rows cost
+  // more than the extra parallelism gains (a single nr = 4097 ell-32 run
+  // steps 30 % faster with 64-row tiles than with 16).
+  const int parallel_rows = static_cast<int>(std::max<long long>(64, total_rows / 1024));
   std::vector<std::vector<TileDesc>> per(instance_count());
-  std::vector<int> inst_lds(instance_count(), 0), inst_ub(instance_count(), 1);
+  std::vector<int> inst_lds(instance_count(), 0);
+  // Execution group of each instance: (scheme, ell-tile class of its widest
+  // line, line variant); an instance takes a compile-time variant when all
+  // its lines share it (stage_variant).
+  std::vector<GroupKey> inst_key(instance_count());
+  for (int i = 0; i < instance_count(); ++i) {
+    const Instance& in = inst_[i];
+    int ub = 1, variant = -1;
+    for (int mi = 0; mi < in.M; ++mi) {
+      const int li = in.line_of_mode[mi];
+      if (li < 0) continue;
+      ub = std::max(ub, band_tiles_class(lines_[li].n_ell));
+      const int v = specialized_ ? stage_variant(lines_[li]) : kStageGeneral;
+      variant = variant < 0 || variant == v ? v : kStageGeneral;
+    }
+    inst_key[i] = {static_cast<int>(in.params.scheme), ub, std::max(variant, 0)};
+  }
   for (int i = 0; i < instance_count(); ++i) {
     Instance& in = inst_[i];
     const RadialScheme rs = in.params.scheme;
+    const GroupShape shape = shape_of(inst_key[i]);
+    const int budget = shape.lds_budget > 0 ? shape.lds_budget : lds_budget_;
+    const int rows_set = shape.tile_rows > 0 ? shape.tile_rows : tile_rows_;
     for (int mi = 0; mi < in.M; ++mi) {
       const int li = in.line_of_mode[mi];
       if (li < 0) continue;
       const LineDesc& L = lines_[li];
       const int n = L.nr;
-      inst_ub[i] = std::max(inst_ub[i], band_tiles_class(L.n_ell));
       auto ranges = [&](int t0, int t1, TileDesc& t) {
         t.t0 = t0;
         t.t1 = t1;
@@ -312,7 +333,7 @@ void Plan::build_tiles() {
       };
       // Largest row count whose tiles (boundary-closure tiles included) all
       // fit the LDS budget, capped for parallelism.
-      int rows = tile_rows_ > 0 ? tile_rows_ : std::min(256, parallel_rows);
+      int rows = rows_set > 0 ? rows_set : std::min(256, parallel_rows);
       rows = std::max(1, std::min(rows, n));
       std::vector<TileDesc> line_tiles;
       for (;;) {
@@ -329,13 +350,13 @@ void Plan::build_tiles() {
           line_tiles.push_back(t);
           t0 += len;
         }
-        if (worst <= lds_budget_ || rows == 1) break;
+        if (worst <= budget || rows == 1) break;
         rows = rows > 16 ? rows - 4 : rows - 1;
       }
       // Kerr lines: the psi_t rows of a tile (output rows plus the stencil
       // halo) go to the four waves in MFMA groups of 16, so prefer tiles
       // whose interior psi_t rows fill whole rounds of 64.
-      if (L.kerr && tile_rows_ <= 0 && n > 2 * rows) {
+      if (L.kerr && rows_set <= 0 && n > 2 * rows) {
         TileDesc probe{};
         ranges(n / 2 - rows / 2, n / 2 - rows / 2 + rows, probe);
         const int halo = (probe.mhi - probe.mlo) - rows;
@@ -362,21 +383,10 @@ void Plan::build_tiles() {
       }
     }
   }
-  // One launch per (scheme, ell-tile class, line variant); a launch reserves
-  // its largest tile's LDS for every workgroup.  An instance takes a
-  // compile-time variant when all its lines share it (stage_variant).
-  std::map<std::tuple<int, int, int>, std::vector<int>> buckets;
-  for (int i = 0; i < instance_count(); ++i) {
-    const Instance& in = inst_[i];
-    int variant = -1;
-    for (int mi = 0; mi < in.M; ++mi) {
-      const int li = in.line_of_mode[mi];
-      if (li < 0) continue;
-      const int v = specialized_ ? stage_variant(lines_[li]) : kStageGeneral;
-      variant = variant < 0 || variant == v ? v : kStageGeneral;
-    }
-    buckets[{static_cast<int>(in.params.scheme), inst_ub[i], std::max(variant, 0)}].push_back(i);
-  }
+  // One launch per execution group; a launch reserves its largest tile's LDS
+  // for every workgroup.
+  std::map<GroupKey, std::vector<int>> buckets;
+  for (int i = 0; i < instance_count(); ++i) buckets[inst_key[i]].push_back(i);
   for (const auto& [key, members] : buckets) {
     Group g{std::get<0>(key), std::get<1>(key), std::get<2>(key),
             static_cast<int>(tiles_.size()), 0, 0};
@@ -1157,6 +1167,31 @@ void Plan::set_specialized(bool enable) {
   build_tiles();
   upload_tables();
   set_graph(use_graph_);
+}
+
+Plan::GroupShape Plan::shape_of(const GroupKey& k) const {
+  const auto it = shapes_.find(k);
+  return it == shapes_.end() ? GroupShape{} : it->second;
+}
+
+void Plan::set_group_shape(int scheme, int ub, int variant, const GroupShape& shape) {
+  if (scheme < 0 || scheme > 2 || (ub != 1 && ub != 2 && ub != 4) || variant < kStageGeneral ||
+      variant > kStageKerrGram)
+    throw std::invalid_argument("execution group key out of range");
+  if (shape.lds_budget != 0 && (shape.lds_budget < 16 * 1024 || shape.lds_budget > 160 * 1024))
+    throw std::invalid_argument("LDS budget must be 0 or in [16 KiB, 160 KiB]");
+  if (shape.tile_rows < 0) throw std::invalid_argument("tile rows must be nonnegative");
+  if (so_mode_) return;
+  shapes_[{scheme, ub, variant}] = shape;
+  build_tiles();
+  upload_tables();
+  set_graph(use_graph_);
+}
+
+Plan::GroupShape Plan::group_shape(int g) const {
+  if (g < 0 || g >= group_count()) throw std::invalid_argument("group index out of range");
+  const Group& gr = groups_[g];
+  return shape_of({gr.scheme, gr.ub, gr.variant});
 }
 
 int Plan::group_variant(int g) const {

@@ -6,8 +6,10 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <memory>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "config.hpp"
@@ -110,6 +112,15 @@ class Plan {
   // every line through the general kernel.  Re-tiles.
   void set_specialized(bool enable);
   int group_variant(int g) const;
+  // Tile shape of one execution group: LDS budget and tile rows (0 = the
+  // plan's setting).
+  struct GroupShape {
+    int lds_budget = 0, tile_rows = 0;
+  };
+  // Keyed by (scheme, ell-tile class, variant); applies to the group of that
+  // key now or after later re-tiling.  Re-tiles.
+  void set_group_shape(int scheme, int ub, int variant, const GroupShape& shape);
+  GroupShape group_shape(int g) const;
   SourceActivation activation(int i) const { return inst_.at(i).activation; }
   void set_activation(int i, const SourceActivation& a);
   bool second_order() const { return so_mode_; }
@@ -176,6 +187,9 @@ class Plan {
     int scheme, ub, variant, first, count, lds;
   };
   std::vector<Group> groups_;
+  using GroupKey = std::tuple<int, int, int>;  // (scheme, ub, variant)
+  std::map<GroupKey, GroupShape> shapes_;
+  GroupShape shape_of(const GroupKey& k) const;
   long long modal_count_ = 0;
   int max_lds_ = 0;
   int lds_budget_ = 80 * 1024;  // two workgroups per CU (160 KiB LDS)

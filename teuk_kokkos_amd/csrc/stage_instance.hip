@@ -15,9 +15,9 @@ namespace teuk {
 
 namespace {
 template <int V>
-hipError_t launch_variant(int stage, const StageArgs& a, int n_tiles, int lds, hipStream_t s) {
+hipError_t launch_variant(int stage, const StageArgs& a, int workgroups, int lds, hipStream_t s) {
   constexpr int S = TEUK_SCHEME, U = TEUK_UB;
-  const dim3 grid(n_tiles), block(kStageThreads);
+  const dim3 grid(workgroups), block(kStageThreads);
   switch (stage) {
     case 0: hipLaunchKernelGGL((teuk_stage_kernel<S, 0, U, V>), grid, block, lds, s, a); break;
     case 1: hipLaunchKernelGGL((teuk_stage_kernel<S, 1, U, V>), grid, block, lds, s, a); break;
@@ -47,12 +47,12 @@ hipError_t limit_variant(int bytes) {
 
 hipError_t TEUK_NAME(launch_stage_instance_, TEUK_SCHEME, TEUK_UB)(int stage, int variant,
                                                                   const StageArgs& a,
-                                                                  int n_tiles, int lds,
+                                                                  int workgroups, int lds,
                                                                   hipStream_t s) {
   switch (variant) {
-    case kStageGeneral: return launch_variant<kStageGeneral>(stage, a, n_tiles, lds, s);
-    case kStageKerr: return launch_variant<kStageKerr>(stage, a, n_tiles, lds, s);
-    case kStageKerrGram: return launch_variant<kStageKerrGram>(stage, a, n_tiles, lds, s);
+    case kStageGeneral: return launch_variant<kStageGeneral>(stage, a, workgroups, lds, s);
+    case kStageKerr: return launch_variant<kStageKerr>(stage, a, workgroups, lds, s);
+    case kStageKerrGram: return launch_variant<kStageKerrGram>(stage, a, workgroups, lds, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -58,7 +58,7 @@ __global__ void teuk_fill_zero(cdouble* p, long long count) {
 // ------------------------------------------------------------ launchers
 #define TEUK_DECLARE_INSTANCE(S, U)                                                        \
   hipError_t launch_stage_instance_##S##_##U(int stage, int variant, const StageArgs& a,   \
-                                             int n_tiles, int lds, hipStream_t s);         \
+                                             int workgroups, int lds, hipStream_t s);      \
   hipError_t set_lds_limit_instance_##S##_##U(int bytes);
 TEUK_DECLARE_INSTANCE(0, 1)
 TEUK_DECLARE_INSTANCE(0, 2)
@@ -86,11 +86,12 @@ constexpr LimitFn kLimit[3][3] = {
 }  // namespace
 
 // ub = ell tiles of the tile kernel (1, 2, 4); variant: kStageGeneral,
-// kStageKerr or kStageKerrGram.
+// kStageKerr or kStageKerrGram; one workgroup per tile.
 hipError_t launch_stage(int scheme, int stage, int ub, int variant, const StageArgs& a,
                         int n_tiles, int lds, hipStream_t s) {
   const int uc = ub == 1 ? 0 : (ub == 2 ? 1 : (ub == 4 ? 2 : -1));
-  if (scheme < 0 || scheme > 2 || uc < 0 || stage < 0 || stage > 4) return hipErrorInvalidValue;
+  if (scheme < 0 || scheme > 2 || uc < 0 || stage < 0 || stage > 4 || n_tiles <= 0)
+    return hipErrorInvalidValue;
   return kLaunch[scheme][uc](stage, variant, a, n_tiles, lds, s);
 }
 

@@ -531,16 +531,15 @@ __device__ __forceinline__ void output_phase(const StageArgs& a, const LineDesc&
     gram_output_group<SCHEME, STAGE, UB, VARIANT == kStageKerrGram>(a, c, Gf, g0, lane);
 }
 
-// One workgroup = one radial tile of one line; UB = ell tiles of 16 the MFMA
-// routines hold at once (1, 2 or 4).  VARIANT fixes the line flags at
-// compile time for the common case (kStageGeneral keeps them at run time):
-// kStageKerr and kStageKerrGram serve Kerr lines in free-damped mode without
-// dissipation, with G = I and with the Gram projection respectively, and
-// compile every other path out.
+// One radial tile of one line; UB = ell tiles of 16 the MFMA routines hold
+// at once (1, 2 or 4).  VARIANT fixes the line flags at compile time for the
+// common case (kStageGeneral keeps them at run time): kStageKerr and
+// kStageKerrGram serve Kerr lines in free-damped mode without dissipation,
+// with G = I and with the Gram projection respectively, and compile every
+// other path out.
 template <int SCHEME, int STAGE, int UB, int VARIANT>
-__global__ __launch_bounds__(kStageThreads, UB == 1 ? 3 : 2) void teuk_stage_kernel(StageArgs a) {
-  extern __shared__ __align__(16) double smem[];
-  const TileDesc tile = a.tiles[blockIdx.x];
+__device__ __forceinline__ void stage_tile(const StageArgs& a, const TileDesc tile,
+                                           double* smem) {
   LineDesc L = a.lines[tile.line];
   if constexpr (VARIANT != kStageGeneral) {
     L.kerr = 1;
@@ -585,6 +584,14 @@ __global__ __launch_bounds__(kStageThreads, UB == 1 ? 3 : 2) void teuk_stage_ker
   // 2. psi_t on rows [mlo, mhi); 3-4. the point RHS and the RK combination.
   psi_t_phase<SCHEME, UB>(a, L, tile, s, tid);
   output_phase<SCHEME, STAGE, UB, VARIANT>(a, L, tile, s, tid);
+}
+
+// One workgroup per tile.  (A loop over several tiles per workgroup keeps
+// more values live across tiles and measured slower at every count.)
+template <int SCHEME, int STAGE, int UB, int VARIANT>
+__global__ __launch_bounds__(kStageThreads, UB == 1 ? 3 : 2) void teuk_stage_kernel(StageArgs a) {
+  extern __shared__ __align__(16) double smem[];
+  stage_tile<SCHEME, STAGE, UB, VARIANT>(a, a.tiles[blockIdx.x], smem);
 }
 
 }  // namespace teuk
