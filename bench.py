@@ -318,10 +318,10 @@ def main() -> int:
     plan.initialize()
     print(f"[bench] initial data {time.perf_counter() - t0:.1f} s; tiles {plan.tiles()}",
           file=sys.stderr, flush=True)
-    # first-order plans replay each RK4 step as one captured hipGraph; the
-    # second-order graph launches its kernels directly (its latch decisions
-    # are taken on the host between steps)
-    use_graph = not args.eager and plan.modal_points > 0
+    # each RK4 step replayed as one captured hipGraph (second-order plans
+    # launch directly only the steps that split at the source start or need
+    # a constraint evaluation for the latch)
+    use_graph = not args.eager
     plan.set_graph(use_graph)
     plan.advance(args.warmup)
     plan.sync()
@@ -352,7 +352,8 @@ def main() -> int:
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                     "kernel": "second-order stage graph (10 launches per RK stage)",
-                    "avg_stage_ms": stage_s * 1e3}
+                    "avg_stage_ms": stage_s * 1e3,
+                    "graph_replays": plan.graph_replays()}
     elif timed_steps:
         # One RK stage = one launch per execution group, the groups running
         # concurrently on streams of their own: a stage's algorithmic bytes

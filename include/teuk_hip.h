@@ -228,10 +228,15 @@ int teuk_source_pairs(const teuk_plan* plan, int instance, double* out, int capa
  * Execution tuning and measurement (no reference counterpart: they replace
  * the reference's SpatialPipelineTiming, spatial_pipeline.hpp:27-38)
  * ------------------------------------------------------------------- */
-/* Graph mode (default off): one RK4 step of every execution group is
- * captured as a hipGraph now, and teuk_advance replays it (second-order
- * plans keep direct launches). */
+/* Graph mode (default off): one RK4 step of every instance is captured as a
+ * hipGraph now, and teuk_advance replays it.  Second-order plans replay it
+ * for every step in which no instance splits at its source start and no
+ * latch decision needs a constraint evaluation (the Unrestricted and
+ * Disabled policies decide on the host); other steps launch directly, with
+ * the same results bit for bit.  teuk_plan_graph_replays counts the steps
+ * taken as replays. */
 int teuk_plan_set_graph(teuk_plan* plan, int enable);
+int teuk_plan_graph_replays(const teuk_plan* plan, long long* replays);
 /* The execution-space argument of SpatialPipeline::evaluate_rhs_at_time and
  * step (spatial_pipeline.hpp:458-462, 590): every later call enqueues on
  * `stream` (a hipStream_t the caller owns and keeps alive while the plan

@@ -93,6 +93,7 @@ SIGNATURES = {
     "teuk_plan_set_group_shape": (_I, [_P, _I, _I, _I, _I, _I]),
     "teuk_plan_group_shape": (_I, [_P, _I, _IP, _IP]),
     "teuk_plan_set_stream": (_I, [_P, _P]),
+    "teuk_plan_graph_replays": (_I, [_P, ctypes.POINTER(ctypes.c_longlong)]),
     "teuk_allocation_count": (_I, [ctypes.POINTER(ctypes.c_ulonglong)]),
     "teuk_device_alloc": (_I, [ctypes.c_size_t, ctypes.POINTER(_P)]),
     "teuk_device_free": (_I, [_P]),
@@ -406,6 +407,12 @@ class Plan:
     # ---------------------------------------------------------------- tuning
     def set_graph(self, enable: bool):
         self._check(lib().teuk_plan_set_graph(self._h, int(enable)))
+
+    def graph_replays(self) -> int:
+        """Steps taken as replays of the captured step graph."""
+        n = ctypes.c_longlong()
+        self._check(lib().teuk_plan_graph_replays(self._h, ctypes.byref(n)))
+        return n.value
 
     def set_lds_budget(self, nbytes: int):
         self._check(lib().teuk_plan_set_lds_budget(self._h, nbytes))

@@ -360,6 +360,15 @@ int teuk_plan_set_graph(teuk_plan* plan, int enable) {
   return guarded(plan ? plan->error : dummy, [&] { plan_of(plan).set_graph(enable != 0); });
 }
 
+int teuk_plan_graph_replays(const teuk_plan* plan, long long* replays) {
+  auto* p = const_cast<teuk_plan*>(plan);
+  std::string dummy;
+  return guarded(p ? p->error : dummy, [&] {
+    if (!replays) throw std::invalid_argument("null output pointer");
+    *replays = plan_of(p).graph_replays();
+  });
+}
+
 int teuk_plan_set_stream(teuk_plan* plan, void* stream) {
   std::string dummy;
   return guarded(plan ? plan->error : dummy,

@@ -868,7 +868,10 @@ void Plan::capture_graph() {
   hipGraph_t graph;
   check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture");
   try {
-    launch_step(0.0, true);
+    if (so_mode_)
+      so_rk_launches(so_->d_dt_step);
+    else
+      launch_step(0.0, true);
   } catch (...) {
     (void)hipStreamEndCapture(stream_, &graph);
     throw;
@@ -919,11 +922,16 @@ void Plan::step(double t, double dt) {
 void Plan::advance(int steps) {
   if (steps < 0) throw std::invalid_argument("step count must be nonnegative");
   if (so_mode_) {
+    if (use_graph_ && !graph_exec_) capture_graph();
     for (int s = 0; s < steps; ++s) {
       std::vector<double> t, dt;
       for (const auto& in : inst_) {
         t.push_back(in.time);
         dt.push_back(in.dt);
+      }
+      if (use_graph_ && so_graph_step(t, dt)) {
+        record_time_step(0.0);
+        continue;
       }
       hipEvent_t e0 = nullptr;
       if (timing_) {
@@ -948,6 +956,7 @@ void Plan::advance(int steps) {
       hipEvent_t e0 = step_begin_event();
       check(hipGraphLaunch(graph_exec_, stream_), "graph launch");
       step_end_event(e0);
+      ++graph_replays_;
       record_time_step(0.0);
     }
     return;
@@ -1080,7 +1089,7 @@ void Plan::sync() { check(hipStreamSynchronize(stream_), "sync"); }
 // re-tiles), so teuk_advance only replays it.  Second-order plans take host
 // decisions between passes and keep launching directly.
 void Plan::set_graph(bool enable) {
-  use_graph_ = enable && !so_mode_;
+  use_graph_ = enable;
   if (graph_exec_) {
     hipGraphExecDestroy(graph_exec_);
     graph_exec_ = nullptr;

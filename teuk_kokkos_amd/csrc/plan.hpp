@@ -97,6 +97,8 @@ class Plan {
   std::vector<WaveformRecordC> sample_waveform(int i);
   void sync();
   void set_graph(bool enable);
+  // Steps taken as replays of the captured step graph so far.
+  long long graph_replays() const { return graph_replays_; }
   void set_stream(hipStream_t s);
   void set_kernel_timing(bool enable);
   void kernel_timing(double* total_ms, long long* launches);
@@ -153,6 +155,8 @@ class Plan {
   void so_export(int i, cplx* host);
   void so_rhs(int first, int n, const cdouble* in, cdouble* out, bool constraints);
   void so_rk_pass(const std::vector<double>& dts);
+  void so_rk_launches(const double* d_dt);
+  bool so_graph_step(const std::vector<double>& t, const std::vector<double>& dt);
   void so_upload_active();
   bool so_consider(int i, double time);
   void so_step(const std::vector<double>& t, const std::vector<double>& dt);
@@ -226,6 +230,7 @@ class Plan {
 
   bool use_graph_ = false;
   hipGraphExec_t graph_exec_ = nullptr;
+  long long graph_replays_ = 0;
   bool timing_ = false;
   hipEvent_t timing_event();
   std::vector<hipEvent_t> events_;  // pool, reused after set_kernel_timing

@@ -31,6 +31,7 @@ struct Plan::SoHost {
   SoTile* d_tiles[3] = {nullptr, nullptr, nullptr};
   int* d_active = nullptr;
   double* d_dt = nullptr;
+  double* d_dt_step = nullptr;  // each instance's configured step (the graph's)
   double* d_red = nullptr;
   cdouble *y = nullptr, *acc = nullptr, *ya = nullptr, *yb = nullptr, *k = nullptr;
   cdouble *tan = nullptr, *dpsi = nullptr, *dpsi_t = nullptr, *rdr = nullptr, *rdr_t = nullptr;

@@ -334,6 +334,14 @@ void Plan::so_build() {
   H.d_active = device_alloc<int>(instance_count());
   H.d_dt = device_alloc<double>(instance_count());
   H.d_red = device_alloc<double>(static_cast<size_t>(instance_count()) * kSoReduceDoubles);
+  {
+    std::vector<double> steps(instance_count());
+    for (int i = 0; i < instance_count(); ++i) steps[i] = inst_[i].dt;
+    H.d_dt_step = device_alloc<double>(steps.size());
+    check(hipMemcpy(H.d_dt_step, steps.data(), steps.size() * sizeof(double),
+                    hipMemcpyHostToDevice),
+          "upload configured steps");
+  }
   for (cdouble** b : {&H.y, &H.acc, &H.ya, &H.yb, &H.k}) {
     *b = device_alloc<cdouble>(H.values);
     check(launch_fill_zero(*b, H.values, stream_), "zero state");
@@ -368,7 +376,8 @@ void Plan::so_free() {
   if (!so_) return;
   SoHost& H = *so_;
   void* ptrs[] = {H.d_bg, H.d_inst, H.d_ops,  H.d_nodes, H.d_tables, H.d_tiles[0], H.d_tiles[1],
-                  H.d_tiles[2], H.d_active, H.d_dt, H.d_red, H.y, H.acc, H.ya, H.yb, H.k,
+                  H.d_tiles[2], H.d_active, H.d_dt, H.d_dt_step, H.d_red, H.y, H.acc, H.ya,
+                  H.yb, H.k,
                   H.tan, H.dpsi, H.dpsi_t, H.rdr, H.rdr_t, H.rang, H.src, H.cons, H.terms,
                   H.rang_t, H.ex, H.raw, H.rangx, H.pairs};
   for (void* p : ptrs)
@@ -451,13 +460,22 @@ void Plan::so_rk_pass(const std::vector<double>& dts) {
   check(hipMemcpyAsync(H.d_dt, dts.data(), dts.size() * sizeof(double), hipMemcpyHostToDevice,
                        stream_),
         "upload steps");
+  so_rk_launches(H.d_dt);
+  check(hipStreamSynchronize(stream_), "RK4 pass");
+}
+
+// The launches of one RK4 pass with the per-instance steps at d_dt: four RHS
+// evaluations, each followed by its RK combination.  Kernel launches only,
+// so a stream capture of it is the step graph.
+void Plan::so_rk_launches(const double* d_dt) {
+  SoHost& H = *so_;
   const cdouble* in_of[5] = {nullptr, H.y, H.ya, H.yb, H.ya};
   cdouble* out_of[5] = {nullptr, H.ya, H.yb, H.ya, H.y};
   for (int s = 1; s <= 4; ++s) {
     so_rhs(0, instance_count(), in_of[s], H.k, false);
     SoRkArgs r{};
     r.inst = H.d_inst;
-    r.dt = H.d_dt;
+    r.dt = d_dt;
     r.k = H.k;
     r.y = H.y;
     r.acc = H.acc;
@@ -467,7 +485,6 @@ void Plan::so_rk_pass(const std::vector<double>& dts) {
     r.n_inst = instance_count();
     check(so_launch_rk(r, stream_), "RK4 combination");
   }
-  check(hipStreamSynchronize(stream_), "RK4 pass");
 }
 
 // consider_source_activation (spatial_pipeline.hpp:1242-1289) for one
@@ -566,6 +583,44 @@ void Plan::so_step(const std::vector<double>& t, const std::vector<double>& dt) 
   for (int i = 0; i < n; ++i)
     if (!inst_[i].activation.active && end[i] >= start[i]) so_consider(i, end[i]);
   so_upload_active();
+}
+
+// so_step as a replay of the captured step graph, for a step (each instance
+// its configured dt) in which no instance splits at its source start and no
+// latch decision needs a constraint evaluation: the Unrestricted and
+// Disabled policies decide on the host (source_activation.hpp:92-233), and
+// the latch reaches the kernels through the device flags, uploaded only when
+// one changes.  Returns false, doing nothing, for any other step.
+bool Plan::so_graph_step(const std::vector<double>& t, const std::vector<double>& dt) {
+  const int n = instance_count();
+  for (int i = 0; i < n; ++i) {
+    const Instance& in = inst_[i];
+    if (in.activation.active) continue;
+    const double start = in.params.source_start_time, end = t[i] + dt[i];
+    if (!std::isfinite(end)) throw std::invalid_argument("pipeline step end time is not finite");
+    if (t[i] < start && start < end) return false;
+    if ((t[i] >= start || end >= start) && in.params.source_mode == SourceMode::ConstraintAware)
+      return false;
+  }
+  bool changed = false;
+  for (int i = 0; i < n; ++i) {
+    const Instance& in = inst_[i];
+    if (!in.activation.active && t[i] >= in.params.source_start_time)
+      changed = so_consider(i, t[i]) || changed;
+  }
+  if (changed) so_upload_active();
+  hipEvent_t e0 = step_begin_event();
+  check(hipGraphLaunch(graph_exec_, stream_), "graph launch");
+  step_end_event(e0);
+  ++graph_replays_;
+  changed = false;
+  for (int i = 0; i < n; ++i) {
+    const Instance& in = inst_[i];
+    if (!in.activation.active && t[i] + dt[i] >= in.params.source_start_time)
+      changed = so_consider(i, t[i] + dt[i]) || changed;
+  }
+  if (changed) so_upload_active();
+  return true;
 }
 
 // Gaussian data with the Galerkin P (pipeline_initial_data.hpp:142-340) on
