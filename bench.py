@@ -197,11 +197,14 @@ def summarize_trace(trace_dir: str, bench_line: str) -> str:
                    f"{100 * tot / total:.1f} |")
     steps = int(line["steps"])
     rl = line.get("roofline") or {}
+    second_order = line["config"].get("modal_points_per_step", 0) == 0
     groups = [g for g in rl.get("groups", []) if g.get("tiles", 0) > 0]
-    n_groups = max(1, len(groups))
-    stage = [r for r in rows if _timed_kernel(r[2], line["config"].get("modal_points_per_step",
-                                                                        0) == 0)]
-    timed = stage[-steps * 4 * n_groups:]
+    # launches per RK stage: one per execution group, or the second-order
+    # graph's ten (three linear triples, two radial D, two reconstructions,
+    # source, source projection, RK combination)
+    per_stage = 10 if second_order else max(1, len(groups))
+    stage = [r for r in rows if _timed_kernel(r[2], second_order)]
+    timed = stage[-steps * 4 * per_stage:]
     busy, cur0, cur1 = 0, None, None
     for t0, t1, _ in timed:
         if cur1 is None or t0 > cur1:
@@ -216,9 +219,10 @@ def summarize_trace(trace_dir: str, bench_line: str) -> str:
     busy_ms = busy / 1e6 / steps
     span_ms = (timed[-1][1] - timed[0][0]) / 1e6 / steps if timed else float("nan")
     modal = line["config"].get("modal_points_per_step", 0)
+    grid = line["config"].get("grid_points_per_step", 0)
     summary = [
         f"Timed region: the last {len(timed)} stage-kernel dispatches ({steps} steps x 4 RK "
-        f"stages x {n_groups} execution groups).",
+        f"stages x {per_stage} " + ("launches" if second_order else "execution groups") + ").",
         "",
         "| per RK4 step | ms |",
         "|---|---|",
@@ -230,11 +234,14 @@ def summarize_trace(trace_dir: str, bench_line: str) -> str:
         f"| trace: busy time (union of dispatch intervals) | {busy_ms:.4f} |",
         f"| trace: sum of dispatch durations (groups overlap) | {sum_ms:.4f} |",
     ]
-    if modal:
-        ach = BYTES_PER_MODAL_STEP * modal / (busy_ms * 1e-3) / 1e9
+    per_unit, units, what = ((BYTES_PER_GRID_STEP_13, grid, "grid points (13 fields)")
+                             if second_order else
+                             (BYTES_PER_MODAL_STEP, modal, "modal coefficients"))
+    if units:
+        ach = per_unit * units / (busy_ms * 1e-3) / 1e9
         summary += ["",
-                    f"Algorithmic bytes per step: {BYTES_PER_MODAL_STEP} B x {modal:,.0f} modal "
-                    f"coefficients = {BYTES_PER_MODAL_STEP * modal / 1e9:.3f} GB.  From the "
+                    f"Algorithmic bytes per step: {per_unit} B x {units:,.0f} {what} = "
+                    f"{per_unit * units / 1e9:.3f} GB.  From the "
                     f"trace's busy time: {ach:.0f} GB/s = {ach / HBM_PEAK_GBS:.3f} of "
                     f"{HBM_PEAK_GBS:.0f} GB/s; the bench line reports "
                     f"{rl.get('achieved', float('nan')):.0f} GB/s = "
