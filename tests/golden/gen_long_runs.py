@@ -15,8 +15,13 @@ zero in the linear evolution.
   case4  a = 0.999 nr 4097 D10-5, ell <= 32, CFL 0.5: the state at 100,
          1000 and 10000 steps at every 128th radial row, scri modal
          coefficients every 100 steps.
+  case4x the same run to its stated 100000 steps: the state at 100000
+         steps at every 128th radial row, scri modal coefficients every
+         1000 steps (written to long_runs_1e5.npz with --out; about four
+         hours of six host threads).
 
 Usage: python tests/golden/gen_long_runs.py [--threads N] [--cases case1,case2,case4]
+           [--out FILE]
 """
 from __future__ import annotations
 
@@ -41,12 +46,13 @@ PLAN = {
     "case1": dict(every=10, stride=8, checkpoints=[2000]),
     "case2": dict(every=10, stride=32, checkpoints=[20000]),
     "case4": dict(every=100, stride=128, checkpoints=[100, 1000, 10000]),
+    "case4x": dict(every=1000, stride=128, checkpoints=[100000], workload="case4"),
 }
 
 
 def run(name: str, threads: int, out: dict) -> None:
     cfg = PLAN[name]
-    params = getattr(W, name)()
+    params = getattr(W, cfg.get("workload", name))()
     dt = params["final_time"] / params["steps"]
     pipe = H.oracle_initialized(params, threads=threads)
     modes = pipe.setup.modes
@@ -60,6 +66,8 @@ def run(name: str, threads: int, out: dict) -> None:
         pipe.advance(every, dt, time0=done * dt)
         done += every
         scri.append(pipe.endpoint_modal(mi)[0])
+        if done % (10 * every) == 0:
+            print(f"{name}: {done} of {total} steps, {time.time() - t0:.0f} s", flush=True)
         if done in cfg["checkpoints"]:
             state = pipe.get_state()
             out[f"{name}_state_{done}"] = state[mi, :3, ::stride, :].copy()
@@ -72,12 +80,13 @@ def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--cases", default="case1,case2,case4")
+    ap.add_argument("--out", default=OUT)
     args = ap.parse_args()
-    out = dict(np.load(OUT)) if os.path.exists(OUT) else {}
+    out = dict(np.load(args.out)) if os.path.exists(args.out) else {}
     for name in args.cases.split(","):
         run(name, args.threads, out)
-        np.savez_compressed(OUT, **out)
-    print(f"wrote {OUT}")
+        np.savez_compressed(args.out, **out)
+    print(f"wrote {args.out}")
 
 
 if __name__ == "__main__":

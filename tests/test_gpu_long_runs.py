@@ -23,7 +23,10 @@ states which one applies:
     lose digits (|A S - I| = 1.8e-9) and their last bits depend on the host's
     long-double arithmetic, so the fixtures made on the build host differ from
     this host's band matrices by about 4e-10 relative from the first step;
-    the same-host comparison below is the parity check."""
+    the same-host comparison below is the parity check.
+The 64-instance second-order batch runs its stated 200 steps (through the
+source switch-on at 10 dt) as graph replays, and two of its instances are
+compared with same-host oracle runs per field."""
 from __future__ import annotations
 
 import os
@@ -150,3 +153,43 @@ def test_near_extremal_checkpoints_match_oracle():
     # and 2.9e-10 measured at 100, 1000 and 10000 steps, 9.5e-12 at scri
     assert max(errs[("fixture", n)] for n in (100, 1000, 10000)) <= 5e-8
     assert err_w <= 1e-10
+
+
+def _field_errors(got, ref):
+    """Per-field relative errors, each field against its own scale."""
+    errs = []
+    for f in range(13):
+        scale = np.abs(ref[:, f]).max()
+        diff = np.abs(got[:, f] - ref[:, f]).max()
+        errs.append(diff / scale if scale > 0 else (0.0 if diff == 0 else np.inf))
+    return np.array(errs)
+
+
+def test_second_order_64_instance_batch_to_its_stated_length():
+    cases = W.second_order_batch(64)
+    steps = cases[0]["steps"]
+    picks = (0, 63)
+    print(f"[gpu] second-order batch: 64 instances, {steps} steps", flush=True)
+    with T.Plan(W.texts(cases)) as plan:
+        plan.set_graph(True)
+        plan.initialize()
+        plan.advance(steps)
+        got = {k: plan.export_state(k) for k in picks}
+        latch = {k: plan.source_activation(k) for k in picks}
+        replays = plan.graph_replays()
+    print(f"[gpu]   {replays} of {steps} steps replayed", flush=True)
+    assert replays >= steps - 2
+    for k in picks:
+        params = cases[k]
+        ref = H.oracle_initialized(params)
+        ref.advance(steps, params["final_time"] / params["steps"])
+        assert latch[k].active and ref.activation()["active"]
+        assert latch[k].activation_time == ref.activation()["activation_time"]
+        errs = _field_errors(got[k], ref.get_state())
+        print(f"[gpu]   instance {k} (a = {params['spin']:.3f}) after {steps} steps: per-field "
+              f"rel err max {errs.max():.3e} (first-order {errs[:3].max():.3e}, "
+              f"reconstruction {errs[3:10].max():.3e}, second-order {errs[10:].max():.3e})",
+              flush=True)
+        assert np.isfinite(got[k]).all() and np.abs(got[k][:, 10:]).max() > 0
+        # measured 2.0e-13 and 1.0e-13 (instances 0 and 63)
+        assert errs.max() <= 1e-12, errs
