@@ -142,8 +142,12 @@ __device__ __forceinline__ void kerr_psi_group(int nl, int nt, double spin, cons
       const double pr = spr[v], pi = spi[v];
       const double num_re = sxr[v] - (gre * pr - gi * pi);
       const double num_im = sxi[v] - (gre * pi + gi * pr);
-      nre[v] = tok ? num_re / ct : 0.0;
-      nim[v] = tok ? num_im / ct : 0.0;
+      // one correctly rounded reciprocal and two products instead of two
+      // divisions (1.3 % of the sweep's stage time; the result differs from
+      // the quotient by about an ulp)
+      const double rct = 1.0 / ct;
+      nre[v] = tok ? num_re * rct : 0.0;
+      nim[v] = tok ? num_im * rct : 0.0;
     }
 #pragma unroll
     for (int v = 0; v < 4; ++v)
