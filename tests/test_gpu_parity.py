@@ -284,6 +284,23 @@ def test_step_and_group_timing_account_for_every_launch():
     assert sum(g["tiles"] for g in groups) == sum(g["tiles"] for g in per_group)
 
 
+def test_explicit_steps_are_timed_in_graph_mode():
+    """teuk_step launches directly even in graph mode; with timing on its
+    steps count in step_timing next to the replays of teuk_advance."""
+    cases = _two_group_cases()
+    with T.Plan([W.config_text(**dict(c)) for c in cases]) as plan:
+        plan.initialize()
+        plan.set_graph(True)
+        plan.set_kernel_timing(True)
+        plan.advance(2)
+        t, _ = plan.progress()
+        plan.step(t, plan.time_step())
+        ms, steps = plan.step_timing()
+        plan.set_kernel_timing(False)
+        replays = plan.graph_replays()
+    assert steps == 3 and ms > 0.0 and replays == 2
+
+
 def test_graph_replays_are_timed_and_match_eager_steps():
     """With timing on, each graph replay is bracketed by HIP events on the
     plan's stream (no per-launch events inside the graph), and the replayed
