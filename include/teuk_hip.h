@@ -151,9 +151,11 @@ int teuk_evaluate_rhs(teuk_plan* plan, int instance, const void* d_stage, void* 
  * capture, tight serving loops, harnesses that forbid hipMalloc): the same
  * kernels, with the modal staging in a fixed per-device array of
  * 2 x 2^20 complex doubles (32 MiB, held until the process ends) instead of
- * the plan's buffers.  Linear plans only; an instance whose modal lines need
- * more than 2^20 values per buffer is refused with TEUK_EINVAL.  Calls that
- * overlap on one device share the array and race. */
+ * the plan's buffers.  Linear plans only.  The staging is indexed like the
+ * plan's own buffers, so an instance is refused with TEUK_EINVAL when its
+ * modal lines end past 2^20 values in the plan's modal order (in a batch
+ * plan the lines of the instances before it count).  Calls that overlap on
+ * one device share the array and race. */
 int teuk_evaluate_rhs_noalloc(teuk_plan* plan, int instance, const void* d_stage, void* d_out,
                               double time);
 /* SpatialPipeline::step (spatial_pipeline.hpp:590-620): one classical RK4
