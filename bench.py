@@ -27,9 +27,10 @@ METRIC = ("grid-point-updates/s, mixed vs FP64 on GPU, at ≤1e-9 waveform err; 
           "% HBM peak")
 HBM_PEAK_GBS = 8000.0
 # Algorithmic HBM bytes per retained modal coefficient per RK4 step: three
-# complex128 fields, 16 state vectors (stage reads/writes of y, stage value
-# and accumulator); one step is four launches.
-BYTES_PER_MODAL_STEP = 3 * 16 * 16
+# complex128 fields, 12 state vectors (stage 1 reads y and writes Y2; stage 2
+# reads Y2 and y, writes Y3 and the partial sum C; stage 3 reads Y3 and y,
+# writes Y4; stage 4 reads Y4 and C, writes y); one step is four launches.
+BYTES_PER_MODAL_STEP = 3 * 16 * 12
 # Second-order workloads evolve all 13 nodal fields: 13 complex128 values
 # per grid point and 16 state vectors per RK4 step.
 BYTES_PER_GRID_STEP_13 = 13 * 16 * 16

@@ -1,6 +1,6 @@
 // Device-side descriptors shared by the plan (host) and the stage kernels.
 //
-// Evolved state layout in HBM (per buffer y, acc, Ya, Yb):
+// Evolved state layout in HBM (per buffer y, C, Ya, Yb):
 //   for every radial line (instance, active first-order mode m):
 //     [radial r = 0..nr-1][field f in {P, Q, Psi}][ell = ell_min..ell_max]
 //   complex128 values, one contiguous block per line at LineDesc::base.
@@ -138,7 +138,7 @@ struct StageArgs {
   const SchemeTable* table;
   const cdouble* yin;   // RHS input (stage value)
   const cdouble* y;     // state at the start of the step
-  cdouble* acc;         // y + dt/6 k1 + dt/3 k2 + dt/3 k3
+  cdouble* acc;         // C = y + dt/6 (k1 + 2 k2) - y/3 (written by stage 2)
   cdouble* yout;        // next stage value, or y after stage 4
   cdouble* kout;        // RHS output (stage 0 only)
   double dt_override;   // > 0: use instead of LineDesc::dt

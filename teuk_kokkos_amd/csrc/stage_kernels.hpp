@@ -189,44 +189,56 @@ __device__ __forceinline__ void store_out(cdouble* p, cdouble v) {
     *p = v;
 }
 
-// Classical RK4 stage combination of one value (STAGE 0: k -> kout).  The
-// accumulator carries y, so a step moves 16 state vectors through HBM:
-//   S1: acc = y + dt/6 k, Ya = y + dt/2 k      S2/S3: acc += dt/3 k,
-//   Y = y + c dt k                             S4: y = acc + dt/6 k.
+// Classical RK4 stage combination of one value (STAGE 0: k -> kout), in a
+// form that moves 12 state vectors per step through HBM.  With Y_s the
+// input of stage s (Y_1 = y) and the partial sum C = y + dt/6 (k1 + 2 k2) - y/3:
+//   S1: Y2 = y + dt/2 k1
+//   S2: Y3 = y + dt/2 k2,  C = (y + Y2)/3 + dt/3 k2   (dt/6 k1 = (Y2 - y)/3)
+//   S3: Y4 = y + dt k3
+//   S4: y  = C + Y4/3 + dt/6 k4                        (dt/3 k3 = (Y4 - y)/3)
+// which is y + dt/6 (k1 + 2 k2 + 2 k3 + k4).  Reads per stage: Y_s (staged)
+// plus y in S2 and S3, C in S4; writes: Y_{s+1}, plus C in S2.  Every term is
+// of the size of the state, so the rewriting costs no accuracy.
 struct RkIn {
-  cdouble y0, acc0;
+  cdouble v;   // y (stages 2, 3) or C (stage 4)
+  cdouble ys;  // Y_s at the point, when re-read from HBM
 };
 
-// need_y1: stage 1 also reads its input from HBM (the staged copy of P was
-// overwritten by psi_t).
+// need_ys: the stage input at the point is read from HBM (an L2 hit: the
+// tile staged it this launch) because psi_t overwrote the staged copy of P.
 template <int STAGE>
-__device__ __forceinline__ RkIn rk_load(const StageArgs& a, long long gi, bool need_y1 = false) {
+__device__ __forceinline__ RkIn rk_load(const StageArgs& a, long long gi, bool need_ys = false) {
   RkIn v{};
-  if constexpr (STAGE == 1) {
-    if (need_y1) v.y0 = a.yin[gi];  // staged this launch: an L2 hit
-  } else if constexpr (STAGE == 2 || STAGE == 3) {
-    v.y0 = a.y[gi];
-    v.acc0 = a.acc[gi];
-  } else if constexpr (STAGE == 4) {
-    v.acc0 = a.acc[gi];
-  }
+  if constexpr (STAGE == 2 || STAGE == 3) v.v = a.y[gi];
+  if constexpr (STAGE == 4) v.v = a.acc[gi];
+  if constexpr (STAGE == 1 || STAGE == 2 || STAGE == 4)
+    if (need_ys) v.ys = a.yin[gi];
   return v;
 }
 
-// y1: the stage-1 input (= y).
+// c k + b with explicit fused multiply-adds, so that every instantiation of
+// the stage kernels rounds the combination identically.
+__device__ __forceinline__ cdouble cfma(double c, cdouble k, cdouble b) {
+  return cmake(__builtin_fma(c, k.re, b.re), __builtin_fma(c, k.im, b.im));
+}
+
+// ys: the stage input Y_s at the point.
 template <int STAGE, bool NT = false>
 __device__ __forceinline__ void rk_store(const StageArgs& a, long long gi, const RkIn& in,
-                                         cdouble y1, cdouble k, double dt) {
+                                         cdouble ys, cdouble k, double dt) {
+  constexpr double third = 1.0 / 3.0;
   if constexpr (STAGE == 0) {
     store_out<NT>(a.kout + gi, k);
   } else if constexpr (STAGE == 1) {
-    store_out<NT>(a.acc + gi, cadd(y1, cscale(dt / 6.0, k)));
-    store_out<NT>(a.yout + gi, cadd(y1, cscale(0.5 * dt, k)));
-  } else if constexpr (STAGE == 2 || STAGE == 3) {
-    store_out<NT>(a.acc + gi, cadd(in.acc0, cscale(dt / 3.0, k)));
-    store_out<NT>(a.yout + gi, cadd(in.y0, cscale(STAGE == 2 ? 0.5 * dt : dt, k)));
+    store_out<NT>(a.yout + gi, cfma(0.5 * dt, k, ys));
+  } else if constexpr (STAGE == 2) {
+    const cdouble mean = cmake(third * (in.v.re + ys.re), third * (in.v.im + ys.im));
+    store_out<NT>(a.acc + gi, cfma(dt * third, k, mean));
+    store_out<NT>(a.yout + gi, cfma(0.5 * dt, k, in.v));
+  } else if constexpr (STAGE == 3) {
+    store_out<NT>(a.yout + gi, cfma(dt, k, in.v));
   } else {
-    store_out<NT>(a.yout + gi, cadd(in.acc0, cscale(dt / 6.0, k)));
+    store_out<NT>(a.yout + gi, cfma(dt / 6.0, k, cfma(third, ys, in.v)));
   }
 }
 
@@ -323,7 +335,7 @@ __device__ __forceinline__ void plain_output(const StageArgs& a, const TileCtx& 
       const cdouble dpt = apply_d<SCHEME>(*c.T, c.PT + lane_l, c.pts, tile.mlo, r, n, L.inv_h);
       const cdouble pt = c.PT[(r - tile.mlo) * c.pts + lane_l];
       const RkIn (&v)[3] = in[j];
-      rk_store<STAGE>(a, g, v[0], c.inplace ? v[0].y0 : u[lane_l], kp, c.dt);
+      rk_store<STAGE>(a, g, v[0], c.inplace ? v[0].ys : u[lane_l], kp, c.dt);
       rk_store<STAGE>(a, g + nl, v[1], u[nl + lane_l], cadd(dpt, xq), c.dt);
       rk_store<STAGE>(a, g + 2 * nl, v[2], u[2 * nl + lane_l], cadd(pt, xs), c.dt);
     }
@@ -363,7 +375,7 @@ __device__ __forceinline__ void gram_output_group(const StageArgs& a, const Tile
 
   auto field = [&](auto field_tag) {
     constexpr int F = decltype(field_tag)::value;
-    const bool need_y1 = F == 0 && c.inplace;
+    const bool need_ys = F == 0 && c.inplace;
     constexpr int NR = UB <= 2 ? UB : 1;  // ell tiles of RK inputs requested early
     RkIn in[NR][4];
     if constexpr (UB <= 2) {
@@ -371,7 +383,7 @@ __device__ __forceinline__ void gram_output_group(const StageArgs& a, const Tile
       for (int uo = 0; uo < UB; ++uo)
 #pragma unroll
         for (int w = 0; w < 4; ++w)
-          if (uo < LT) in[uo][w] = rk_load<STAGE>(a, out_index(F, uo, w), need_y1);
+          if (uo < LT) in[uo][w] = rk_load<STAGE>(a, out_index(F, uo, w), need_ys);
     }
     mfma_acc ar[UB], ai[UB];
 #pragma unroll
@@ -422,7 +434,7 @@ __device__ __forceinline__ void gram_output_group(const StageArgs& a, const Tile
       const int slot = UB <= 2 ? uo : 0;
       if constexpr (UB > 2) {
 #pragma unroll
-        for (int w = 0; w < 4; ++w) in[0][w] = rk_load<STAGE>(a, out_index(F, uo, w), need_y1);
+        for (int w = 0; w < 4; ++w) in[0][w] = rk_load<STAGE>(a, out_index(F, uo, w), need_ys);
       }
 #pragma unroll
       for (int w = 0; w < 4; ++w) {
@@ -432,8 +444,8 @@ __device__ __forceinline__ void gram_output_group(const StageArgs& a, const Tile
         if constexpr (F == 1)
           k = cadd(k, apply_d<SCHEME>(*c.T, c.PT + l, c.pts, tile.mlo, r, n, L.inv_h));
         if constexpr (F == 2) k = cadd(k, c.PT[(r - tile.mlo) * c.pts + l]);
-        const cdouble y1 = need_y1 ? in[slot][w].y0 : c.U[(r - tile.lo) * us + F * nl + l];
-        rk_store<STAGE, NT>(a, out_index(F, uo, w), in[slot][w], y1, k, c.dt);
+        const cdouble ys = need_ys ? in[slot][w].ys : c.U[(r - tile.lo) * us + F * nl + l];
+        rk_store<STAGE, NT>(a, out_index(F, uo, w), in[slot][w], ys, k, c.dt);
       }
     }
   };
