@@ -115,6 +115,22 @@ struct SchemeShape {
   static constexpr int C = SCHEME == 2 ? 16 : (SCHEME == 1 ? 12 : 6);
   static constexpr int B = SCHEME == 2 ? 5 : (SCHEME == 1 ? 4 : 2);
   static constexpr int O = SCHEME == 2 ? 6 : (SCHEME == 1 ? 5 : 3);
+  // Interior antisymmetric stencil d_k (k = 1..B) of h*D: the same rationals,
+  // rounded the same way, as the host table (host_math.cpp), so the kernels
+  // take them as immediates instead of LDS reads.
+  static constexpr double interior(int k) {
+    if constexpr (SCHEME == 2) {
+      constexpr double d[6] = {0.0, 5.0 / 6.0, -5.0 / 21.0, 5.0 / 84.0, -5.0 / 504.0,
+                               1.0 / 1260.0};
+      return d[k];
+    } else if constexpr (SCHEME == 1) {
+      constexpr double d[5] = {0.0, 4.0 / 5.0, -1.0 / 5.0, 4.0 / 105.0, -1.0 / 280.0};
+      return d[k];
+    } else {
+      constexpr double d[3] = {0.0, 2.0 / 3.0, -1.0 / 12.0};
+      return d[k];
+    }
+  }
 };
 
 // Stage-kernel variants: line flags fixed at compile time, or read from the

@@ -9,14 +9,17 @@
 namespace teuk {
 
 // h*D applied at global row `row` to values v(g) = base[(g - lo) * stride].
-template <int SCHEME>
+// INTERIOR: the caller guarantees W <= row < n - W, so only the interior
+// stencil is compiled (no branch: the loads of several stencils can then be
+// issued together).
+template <int SCHEME, bool INTERIOR = false>
 __device__ __forceinline__ cdouble apply_d(const SchemeTable& T, const cdouble* base, int stride,
                                            int lo, int row, int n, double inv_h) {
   constexpr int W = SchemeShape<SCHEME>::W;
   constexpr int C = SchemeShape<SCHEME>::C;
   constexpr int B = SchemeShape<SCHEME>::B;
   double re = 0.0, im = 0.0;
-  if (row < W) {
+  if (!INTERIOR && row < W) {
     const double* c = &T.left[row * kMaxClosureWidth];
     const cdouble* v = base + (0 - lo) * stride;
     // closure rows are rare: keep them compact to save registers
@@ -26,7 +29,7 @@ __device__ __forceinline__ cdouble apply_d(const SchemeTable& T, const cdouble* 
       re += c[k] * x.re;
       im += c[k] * x.im;
     }
-  } else if (row + W >= n) {
+  } else if (!INTERIOR && row + W >= n) {
     const double* c = &T.left[(n - 1 - row) * kMaxClosureWidth];
     const cdouble* v = base + (n - 1 - lo) * stride;
 #pragma unroll 2
@@ -41,8 +44,9 @@ __device__ __forceinline__ cdouble apply_d(const SchemeTable& T, const cdouble* 
     for (int k = B; k >= 1; --k) {
       const cdouble a = v[k * stride];
       const cdouble b = v[-k * stride];
-      re += T.interior[k] * (a.re - b.re);
-      im += T.interior[k] * (a.im - b.im);
+      const double d = SchemeShape<SCHEME>::interior(k);
+      re += d * (a.re - b.re);
+      im += d * (a.im - b.im);
     }
   }
   return cmake(inv_h * re, inv_h * im);

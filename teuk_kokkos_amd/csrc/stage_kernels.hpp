@@ -262,7 +262,7 @@ struct TileCtx {
 // i.e. they carry G = A S in modal coordinates (full_spatial.hpp:99-280).
 //   P: H_R D Q_eff + q Q_eff + psi Psi [+ Lambda_l lap_in] (+ dissipation)
 //   Q: -gamma (Q - D Psi) (+ dissipation)        Psi: (dissipation)
-template <int SCHEME, int FIELD, bool WITH_LAP = true>
+template <int SCHEME, int FIELD, bool WITH_LAP = true, bool IN = false>
 __device__ __forceinline__ cdouble pre_projection(const TileCtx& c, int r, int l, cdouble lap_in) {
   const LineDesc& L = *c.L;
   const TileDesc& tile = *c.tile;
@@ -273,10 +273,10 @@ __device__ __forceinline__ cdouble pre_projection(const TileCtx& c, int r, int l
     cdouble qeff, dq;
     if (L.constrained) {
       qeff = c.DP[(r - tile.mlo) * nl + l];
-      dq = apply_d<SCHEME>(*c.T, c.DP + l, nl, tile.mlo, r, n, L.inv_h);
+      dq = apply_d<SCHEME, IN>(*c.T, c.DP + l, nl, tile.mlo, r, n, L.inv_h);
     } else {
       qeff = u[nl + l];
-      dq = apply_d<SCHEME>(*c.T, c.U + nl + l, us, tile.lo, r, n, L.inv_h);
+      dq = apply_d<SCHEME, IN>(*c.T, c.U + nl + l, us, tile.lo, r, n, L.inv_h);
     }
     const RowCoeffs& rc = c.RC[r - c.rc_lo];
     const double lam = -static_cast<double>((L.ell_min + l + 2) * (L.ell_min + l - 1));
@@ -287,7 +287,7 @@ __device__ __forceinline__ cdouble pre_projection(const TileCtx& c, int r, int l
     else
       x = cadd(radial, coupling);
   } else if constexpr (FIELD == 1) {
-    const cdouble dpsi = apply_d<SCHEME>(*c.T, c.U + 2 * nl + l, us, tile.lo, r, n, L.inv_h);
+    const cdouble dpsi = apply_d<SCHEME, IN>(*c.T, c.U + 2 * nl + l, us, tile.lo, r, n, L.inv_h);
     x = cscale(-c.gamma, csub(u[nl + l], dpsi));
   } else {
     x = cmake(0.0, 0.0);
@@ -299,48 +299,70 @@ __device__ __forceinline__ cdouble pre_projection(const TileCtx& c, int r, int l
 
 // Output rows of the tile, one (row, ell) point per thread, for bands whose
 // A S is the identity to roundoff.  A thread walks rows row_l, row_l + step,
-// ...; the RK inputs of its next kPlainDepth - 1 rows are in flight while a
-// row is computed.
-inline constexpr int kPlainDepth = 2;
-template <int SCHEME, int STAGE>
-__device__ __forceinline__ void plain_output(const StageArgs& a, const TileCtx& c, int tid) {
+// ...; the RK inputs of its first kPlainDepth rows are requested when the
+// tile starts (plain_prefetch, before the staging copy, so they arrive while
+// the tile is staged and psi_t is formed) and each consumed slot is refilled
+// with the row kPlainDepth steps ahead.
+// Rows in flight per thread: four for narrow bands (measured best of 2, 4
+// and 6); wide bands (UB > 1) keep two, within their register budget.
+template <int UB>
+inline constexpr int kPlainDepth = UB == 1 ? 4 : 2;
+template <int STAGE>
+__device__ __forceinline__ void plain_fetch(const StageArgs& a, const LineDesc& L, bool inplace,
+                                            int r, int lane_l, RkIn (&v)[3]) {
+  const int nl = L.n_ell;
+  const long long g = L.base + static_cast<long long>(r) * 3 * nl + lane_l;
+#pragma unroll
+  for (int f = 0; f < 3; ++f) v[f] = rk_load<STAGE>(a, g + f * nl, f == 0 && inplace);
+}
+
+template <int STAGE, int D>
+__device__ __forceinline__ void plain_prefetch(const StageArgs& a, const LineDesc& L,
+                                               const TileDesc& tile, bool inplace, int tid,
+                                               RkIn (&in)[D][3]) {
+  const int nl = L.n_ell;
+  const int lane_l = tid % nl, row_l = tid / nl, step_l = kStageThreads / nl;
+  if (row_l >= step_l) return;
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    const int r = tile.t0 + row_l + j * step_l;
+    if (r < tile.t1) plain_fetch<STAGE>(a, L, inplace, r, lane_l, in[j]);
+  }
+}
+
+template <int SCHEME, int STAGE, int D, bool IN>
+__device__ __forceinline__ void plain_output(const StageArgs& a, const TileCtx& c, int tid,
+                                             RkIn (&in)[D][3]) {
   const LineDesc& L = *c.L;
   const TileDesc& tile = *c.tile;
   const int nl = L.n_ell, us = 3 * nl, n = L.nr;
   const int lane_l = tid % nl, row_l = tid / nl, step_l = kStageThreads / nl;
   if (row_l >= step_l) return;
-  RkIn in[kPlainDepth][3];
-  auto fetch = [&](RkIn (&v)[3], int r) {
-    const long long g = L.base + static_cast<long long>(r) * us + lane_l;
+  for (int r0 = tile.t0 + row_l; r0 < tile.t1; r0 += D * step_l) {
 #pragma unroll
-    for (int f = 0; f < 3; ++f) v[f] = rk_load<STAGE>(a, g + f * nl, f == 0 && c.inplace);
-  };
-#pragma unroll
-  for (int j = 0; j + 1 < kPlainDepth; ++j) {
-    const int r = tile.t0 + row_l + j * step_l;
-    if (r < tile.t1) fetch(in[j], r);
-  }
-  for (int r0 = tile.t0 + row_l; r0 < tile.t1; r0 += kPlainDepth * step_l) {
-#pragma unroll
-    for (int j = 0; j < kPlainDepth; ++j) {
+    for (int j = 0; j < D; ++j) {
       const int r = r0 + j * step_l;
       if (r >= tile.t1) break;
-      const int rf = r + (kPlainDepth - 1) * step_l;
-      if (rf < tile.t1) fetch(in[(j + kPlainDepth - 1) % kPlainDepth], rf);
       const long long g = L.base + static_cast<long long>(r) * us + lane_l;
       const cdouble* u = c.U + (r - tile.lo) * us;
-      const cdouble kp = pre_projection<SCHEME, 0>(c, r, lane_l, u[2 * nl + lane_l]);
-      const cdouble xq = pre_projection<SCHEME, 1>(c, r, lane_l, u[2 * nl + lane_l]);
-      const cdouble xs = pre_projection<SCHEME, 2>(c, r, lane_l, u[2 * nl + lane_l]);
-      const cdouble dpt = apply_d<SCHEME>(*c.T, c.PT + lane_l, c.pts, tile.mlo, r, n, L.inv_h);
+      const cdouble kp = pre_projection<SCHEME, 0, true, IN>(c, r, lane_l, u[2 * nl + lane_l]);
+      const cdouble xq = pre_projection<SCHEME, 1, true, IN>(c, r, lane_l, u[2 * nl + lane_l]);
+      const cdouble xs = pre_projection<SCHEME, 2, true, IN>(c, r, lane_l, u[2 * nl + lane_l]);
+      const cdouble dpt = apply_d<SCHEME, IN>(*c.T, c.PT + lane_l, c.pts, tile.mlo, r, n, L.inv_h);
       const cdouble pt = c.PT[(r - tile.mlo) * c.pts + lane_l];
       const RkIn (&v)[3] = in[j];
       rk_store<STAGE>(a, g, v[0], c.inplace ? v[0].ys : u[lane_l], kp, c.dt);
       rk_store<STAGE>(a, g + nl, v[1], u[nl + lane_l], cadd(dpt, xq), c.dt);
       rk_store<STAGE>(a, g + 2 * nl, v[2], u[2 * nl + lane_l], cadd(pt, xs), c.dt);
+      const int rf = r + D * step_l;
+      if (rf < tile.t1) plain_fetch<STAGE>(a, L, c.inplace, rf, lane_l, in[j]);
     }
   }
 }
+
+// G (and M) fragment blocks in flight ahead of the block in use (two or three
+// measured no faster).
+inline constexpr int kGramAhead = 1;
 
 // Output rows [g0, g0 + 16) of the tile (relative to t0) by one wave for
 // bands that need G = A S: k = G X (+ M Psi for P, M = G Lambda G: the
@@ -352,7 +374,7 @@ __device__ __forceinline__ void plain_output(const StageArgs& a, const TileCtx& 
 //   accumulator      element w: row g0 + lg + 4 w, ell 16 uo + lr,
 // so each RK load or store instruction covers 16 consecutive ell of four
 // rows.  UB = ell tiles of 16 accumulated at once.
-template <int SCHEME, int STAGE, int UB, bool NT>
+template <int SCHEME, int STAGE, int UB, bool NT, bool IN>
 __device__ __forceinline__ void gram_output_group(const StageArgs& a, const TileCtx& c,
                                                   const double* Gf, int g0, int lane) {
   const LineDesc& L = *c.L;
@@ -373,10 +395,13 @@ __device__ __forceinline__ void gram_output_group(const StageArgs& a, const Tile
     return L.base + static_cast<long long>(out_row(w)) * us + F * nl + min(16 * uo + lr, nl - 1);
   };
 
+  constexpr int NR = UB <= 2 ? UB : 1;  // ell tiles of RK inputs requested early
+
   auto field = [&](auto field_tag) {
     constexpr int F = decltype(field_tag)::value;
     const bool need_ys = F == 0 && c.inplace;
-    constexpr int NR = UB <= 2 ? UB : 1;  // ell tiles of RK inputs requested early
+    // RK inputs of this field's accumulator positions (requesting all three
+    // fields' inputs up front exceeds the register budget: spills, slower)
     RkIn in[NR][4];
     if constexpr (UB <= 2) {
 #pragma unroll
@@ -390,10 +415,10 @@ __device__ __forceinline__ void gram_output_group(const StageArgs& a, const Tile
     for (int uo = 0; uo < UB; ++uo) ar[uo] = ai[uo] = mfma_acc{0.0, 0.0, 0.0, 0.0};
     // without dissipation the Psi pre-projection terms vanish
     if (F != 2 || L.dissipate) {
-      // The G (and M) fragments of block kb + 1 are in flight during block
-      // kb.  (Unrolling this loop raised the Gram variant to 256 VGPRs and
-      // made it slower.)
-      double g_cur[UB], m_cur[UB], g_nxt[UB], m_nxt[UB];
+      // The G (and M) fragments of blocks kb + 1 .. kb + kGramAhead (from L2)
+      // are in flight during block kb.
+      constexpr int GD = kGramAhead;
+      double gq[GD + 1][UB], mq[GD + 1][UB];
       auto load_block = [&](int kb, double (&g)[UB], double (&m)[UB]) {
 #pragma unroll
         for (int uo = 0; uo < UB; ++uo) {
@@ -401,14 +426,18 @@ __device__ __forceinline__ void gram_output_group(const StageArgs& a, const Tile
           if constexpr (F == 0) m[uo] = uo < LT ? Mf[(uo * K + kb) * 64 + lane] : 0.0;
         }
       };
-      load_block(0, g_cur, m_cur);
+#pragma unroll
+      for (int d = 0; d < GD; ++d)
+        if (d < K) load_block(d, gq[d], mq[d]);
 #pragma unroll
       for (int kb = 0; kb < K; ++kb) {
-        if (kb + 1 < K) load_block(kb + 1, g_nxt, m_nxt);
+        if (kb + GD < K) load_block(kb + GD, gq[GD], mq[GD]);
+        const double (&g_cur)[UB] = gq[0];
+        const double (&m_cur)[UB] = mq[0];
         const int l = 4 * kb + lg;
         cdouble x = zero, ps = zero;
         if (l < nl) {
-          x = pre_projection<SCHEME, F, false>(c, r_in, l, zero);
+          x = pre_projection<SCHEME, F, false, IN>(c, r_in, l, zero);
           if constexpr (F == 0) ps = u_in[2 * nl + l];
         }
 #pragma unroll
@@ -422,10 +451,12 @@ __device__ __forceinline__ void gram_output_group(const StageArgs& a, const Tile
           }
         }
 #pragma unroll
-        for (int uo = 0; uo < UB; ++uo) {
-          g_cur[uo] = g_nxt[uo];
-          if constexpr (F == 0) m_cur[uo] = m_nxt[uo];
-        }
+        for (int d = 0; d < GD; ++d)
+#pragma unroll
+          for (int uo = 0; uo < UB; ++uo) {
+            gq[d][uo] = gq[d + 1][uo];
+            if constexpr (F == 0) mq[d][uo] = mq[d + 1][uo];
+          }
       }
     }
 #pragma unroll
@@ -442,7 +473,7 @@ __device__ __forceinline__ void gram_output_group(const StageArgs& a, const Tile
         const int r = out_row(w), l = 16 * uo + lr;
         cdouble k = cmake(ar[uo][w], ai[uo][w]);
         if constexpr (F == 1)
-          k = cadd(k, apply_d<SCHEME>(*c.T, c.PT + l, c.pts, tile.mlo, r, n, L.inv_h));
+          k = cadd(k, apply_d<SCHEME, IN>(*c.T, c.PT + l, c.pts, tile.mlo, r, n, L.inv_h));
         if constexpr (F == 2) k = cadd(k, c.PT[(r - tile.mlo) * c.pts + l]);
         const cdouble ys = need_ys ? in[slot][w].ys : c.U[(r - tile.lo) * us + F * nl + l];
         rk_store<STAGE, NT>(a, out_index(F, uo, w), in[slot][w], ys, k, c.dt);
@@ -530,21 +561,23 @@ __device__ __forceinline__ void psi_t_phase(const StageArgs& a, const LineDesc& 
   }
 }
 
-// Point RHS and RK combination on the output rows [t0, t1).
-template <int SCHEME, int STAGE, int UB, int VARIANT>
+// Point RHS and RK combination on the output rows [t0, t1).  IN: every
+// output row is an interior row of the radial operator (no closure rows).
+template <int SCHEME, int STAGE, int UB, int VARIANT, bool IN>
 __device__ __forceinline__ void output_phase(const StageArgs& a, const LineDesc& L,
-                                             const TileDesc& tile, const StageLds& s, int tid) {
+                                             const TileDesc& tile, const StageLds& s, int tid,
+                                             RkIn (&pre)[kPlainDepth<UB>][3]) {
   const int wave = tid >> 6, lane = tid & 63;
   const int orows = tile.t1 - tile.t0;
   TileCtx c{&L,     &tile, s.T,       s.RC,      s.rc_lo, s.U, s.PT, s.DP, s.pts, s.inplace,
             a.dt_override > 0.0 ? a.dt_override : L.dt, L.constrained ? 0.0 : L.gamma};
   if (!L.gram) {
-    plain_output<SCHEME, STAGE>(a, c, tid);
+    plain_output<SCHEME, STAGE, kPlainDepth<UB>, IN>(a, c, tid, pre);
     return;
   }
   const double* Gf = a.mats + L.gfrag_off;
   for (int g0 = 16 * wave; g0 < orows; g0 += 16 * (kStageThreads / 64))
-    gram_output_group<SCHEME, STAGE, UB, VARIANT == kStageKerrGram>(a, c, Gf, g0, lane);
+    gram_output_group<SCHEME, STAGE, UB, VARIANT == kStageKerrGram, IN>(a, c, Gf, g0, lane);
 }
 
 // One radial tile of one line; UB = ell tiles of 16 the MFMA routines hold
@@ -588,8 +621,13 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const TileDesc ti
   s.NODES = reinterpret_cast<double*>(L.constrained ? s.DP + mrows * nl : s.DP);
 
   // 1. stage the tile rows [lo, hi) of all three fields, the tables and the
-  //    per-row coefficients.
+  //    per-row coefficients; the RK inputs of the first output rows (G = I
+  //    bands) are requested once the staging loads are issued.
+  // (Requested before the staging loads they delay the staging, which the
+  // rest of the tile waits for: slower.)
+  RkIn pre[kPlainDepth<UB>][3];
   stage_copy(s.U, a.yin + L.base + static_cast<long long>(tile.lo) * us, rows * us, tid);
+  if (!L.gram) plain_prefetch<STAGE>(a, L, tile, s.inplace, tid, pre);
   for (int j = tid; j < static_cast<int>(sizeof(SchemeTable) / 8); j += kStageThreads)
     smem[j] = reinterpret_cast<const double*>(a.table)[j];
   if (L.kerr)
@@ -599,15 +637,37 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const TileDesc ti
 
   // 2. psi_t on rows [mlo, mhi); 3-4. the point RHS and the RK combination.
   psi_t_phase<SCHEME, UB>(a, L, tile, s, tid);
-  output_phase<SCHEME, STAGE, UB, VARIANT>(a, L, tile, s, tid);
+  constexpr int W = SchemeShape<SCHEME>::W;
+  if (tile.t0 >= W && tile.t1 + W <= L.nr)
+    output_phase<SCHEME, STAGE, UB, VARIANT, true>(a, L, tile, s, tid, pre);
+  else
+    output_phase<SCHEME, STAGE, UB, VARIANT, false>(a, L, tile, s, tid, pre);
+}
+
+// Workgroups are dispatched to the 8 XCDs in turn (workgroup b to XCD b mod
+// 8), each XCD with its own L2.  Within every run of 8 * kXcdChunk
+// workgroups, XCD x takes kXcdChunk consecutive tiles, so neighbouring tiles
+// of a line, which stage each other's halo rows, mostly share an L2; runs
+// stay short so every XCD still gets an even mix of the group's tiles (HBM
+// reads of the ell-24 class: -10 %).
+inline constexpr int kXcdChunk = 8;
+__device__ __forceinline__ int xcd_tile_order(int b, int n) {
+  constexpr int span = 8 * kXcdChunk;
+  if constexpr (kXcdChunk <= 1) return b;
+  if (b >= n / span * span) return b;  // the last partial run keeps its order
+  const int w = b % span;
+  return b - w + (w % 8) * kXcdChunk + w / 8;
 }
 
 // One workgroup per tile.  (A loop over several tiles per workgroup keeps
-// more values live across tiles and measured slower at every count.)
+// more values live across tiles and measured slower at every count.)  The
+// LDS budget allows two workgroups per CU, so registers are sized for two.
 template <int SCHEME, int STAGE, int UB, int VARIANT>
-__global__ __launch_bounds__(kStageThreads, UB == 1 ? 3 : 2) void teuk_stage_kernel(StageArgs a) {
+__global__ __launch_bounds__(kStageThreads, 2) void teuk_stage_kernel(
+    StageArgs a) {
   extern __shared__ __align__(16) double smem[];
-  stage_tile<SCHEME, STAGE, UB, VARIANT>(a, a.tiles[blockIdx.x], smem);
+  stage_tile<SCHEME, STAGE, UB, VARIANT>(a, a.tiles[xcd_tile_order(blockIdx.x, gridDim.x)],
+                                         smem);
 }
 
 }  // namespace teuk
