@@ -359,6 +359,8 @@ def main() -> int:
         achieved = BYTES_PER_GRID_STEP_13 / 4.0 * grid / stage_s / 1e9
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                    "algorithmic_bytes": {"per_unit_and_step": BYTES_PER_GRID_STEP_13,
+                                          "unit": "grid point (13 fields)", "units": grid},
                     "kernel": "second-order stage graph (10 launches per RK stage)",
                     "avg_stage_ms": stage_s * 1e3,
                     "graph_replays": plan.graph_replays()}
@@ -371,6 +373,9 @@ def main() -> int:
         achieved = BYTES_PER_MODAL_STEP / 4.0 * modal / stage_s / 1e9
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                    "algorithmic_bytes": {"per_unit_and_step": BYTES_PER_MODAL_STEP,
+                                          "unit": "retained modal coefficient (P, Q, Psi)",
+                                          "units": modal},
                     "kernel": "teuk_stage_kernel (4 RK stages)",
                     "avg_stage_ms": stage_s * 1e3,
                     "launches_per_stage": (float(sum(1 for g in groups if g["tiles"] > 0))
