@@ -147,6 +147,34 @@ def test_tiling_does_not_change_results():
         assert H.rel_err(o, out[0]) <= 1e-14
 
 
+@pytest.mark.parametrize("ell", [24, 32])
+def test_gram_split_and_group_outputs_agree_and_match_oracle(ell):
+    """Gram bands apply G = A S as I + E with the correction E X (and N Psi)
+    on FP32 MFMA.  Tiles of at most five rows per thread take the
+    one-point-per-thread routine, larger ones the accumulator-position routine;
+    both must give the same bits, and at ell 32, where |E| ~ 2e-9, a missing
+    or misplaced correction would exceed the FP64 bar against the oracle."""
+    params = H.small_case(spin=0.99, compactification_length=1.0, nr=61,
+                          radial_discretization="d10-5", ellmax_first=ell, ellmax_second=ell,
+                          seeds=[(2, 2, 1e-3), (6, 2, 2e-4 - 3e-4j)])
+    out = {}
+    # 16-row tiles fit the split routine (n_ell 23 or 31: 11 or 8 rows per
+    # thread pass, five passes); one 61-row tile does not
+    for rows in (16, 61):
+        with _plan(params) as plan:
+            plan.set_tile_rows(rows)
+            assert {g["variant"] for g in plan.groups()} == {2}
+            plan.initialize()
+            plan.advance(4)
+            out[rows] = plan.export_state()
+    assert np.array_equal(out[16], out[61])
+    ref = H.oracle_initialized(params)
+    ref.advance(4, params["final_time"] / params["steps"])
+    err = H.rel_err(out[16], ref.get_state())
+    print(f"[gpu] Gram ell {ell} split/group outputs, 4 steps rel err {err:.3e}", flush=True)
+    assert err <= TOL
+
+
 @pytest.mark.parametrize("scheme,ell", [("d10-5", 16), ("d10-5", 24), ("d8-4", 16), ("d4-2", 24)])
 def test_specialised_stage_kernels_match_general_kernel_and_oracle(scheme, ell):
     """Kerr lines without dissipation in free-damped mode run stage kernels

@@ -28,7 +28,7 @@ struct LineDesc {
   int mat_off;
   int node_off;     // double offset of sin(theta_j)[ntheta] then cos(theta_j)[ntheta]
   int frag_off;     // double offset of the MFMA operand fragments of S and A
-  int gfrag_off;    // double offset of the MFMA operand fragments of G
+  int gfrag_off;    // double offset of the FP32 MFMA fragments of G - I and M - Lambda
   double mass, spin, length, h, inv_h, gamma, eps, dt;
 };
 
@@ -76,11 +76,18 @@ __host__ __device__ inline int mfma_s_fragments(int ntheta, int n_ell) {
 __host__ __device__ inline int mfma_fragment_doubles(int ntheta, int n_ell) {
   return 64 * mfma_theta_tiles(ntheta) * (mfma_ell_blocks(n_ell) + 4 * mfma_ell_tiles(n_ell));
 }
-// G = A S and M = G Lambda G as B operands of k = G X + M Psi, one fragment
-// per (output ell tile uo, contraction block kb) each:
-//   fragment (uo, kb), lane: G[16 uo + lr][4 kb + lg]; see gram_output_group.
+// The Gram correction operands of k = G X + M Psi split as G = I + E,
+// M = G Lambda G = Lambda + N: E, then N, as FP32 B operands of
+// v_mfma_f32_16x16x4_f32, one fragment per (output ell tile uo, contraction
+// block kb), lane: E[16 uo + lr][4 kb + lg]; gram_correction_floats floats
+// each, gram_fragment_doubles doubles together.  v_mfma_f32_16x16x4_f32 takes
+// its operands as the FP64 instruction does, but its accumulator layout
+// differs (measured on gfx950): element v of lane holds D[4 lg + v][lr].
+__host__ __device__ inline int gram_correction_floats(int n_ell) {
+  return 64 * mfma_ell_tiles(n_ell) * mfma_ell_blocks(n_ell);
+}
 __host__ __device__ inline int gram_fragment_doubles(int n_ell) {
-  return 2 * 64 * mfma_ell_tiles(n_ell) * mfma_ell_blocks(n_ell);
+  return gram_correction_floats(n_ell);  // two float matrices
 }
 // Widest band the MFMA routines hold (4 ell tiles of 16).
 inline constexpr int kMaxBandEll = 64;

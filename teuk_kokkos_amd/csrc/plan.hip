@@ -173,32 +173,41 @@ static void append_mfma_fragments(const BandTransform& bt, int nt, std::vector<d
           a[((t * 4 + v) * LT + u) * 64 + lane] = th < nt && l < nl ? bt.A[l * nt + th] : 0.0;
         }
     }
-  // G = A S for k = G X, then M = G Lambda G for the Laplacian term
-  // (gram_output_group): fragment (uo, kb), lane: G[16 uo + lr][4 kb + lg]
-  std::vector<double> lap(static_cast<size_t>(nl) * nl, 0.0);
-  for (int i = 0; i < nl; ++i)
-    for (int k = 0; k < nl; ++k) {
-      long double acc = 0.0L;
-      for (int j = 0; j < nl; ++j) {
-        const int ell = bt.ell_min + j;
-        acc += static_cast<long double>(bt.G[i * nl + j]) *
-               static_cast<long double>(-(ell + 2) * (ell - 1)) *
-               static_cast<long double>(bt.G[j * nl + k]);
-      }
-      lap[i * nl + k] = static_cast<double>(acc);
-    }
-  const size_t gfirst = out.size();
-  out.resize(gfirst + gram_fragment_doubles(nl), 0.0);
-  for (int which = 0; which < 2; ++which) {
-    const std::vector<double>& mat = which == 0 ? bt.G : lap;
-    double* g = out.data() + gfirst + which * (gram_fragment_doubles(nl) / 2);
+  // The Gram projection G = A S of k = G X and M = G Lambda G of the
+  // Laplacian term, split as G = I + E and M = Lambda + N: E and N (of the
+  // size of A S - I) as FP32 B operands, fragment (uo, kb), lane:
+  // E[16 uo + lr][4 kb + lg] (gram_correction_floats), both summed in long
+  // double from the double G before the final rounding.
+  const size_t efirst = out.size();
+  out.resize(efirst + gram_fragment_doubles(nl), 0.0);
+  float* e = reinterpret_cast<float*>(out.data() + efirst);
+  for (int which = 0; which < 2; ++which)
     for (int uo = 0; uo < LT; ++uo)
       for (int kb = 0; kb < K; ++kb)
         for (int lane = 0; lane < 64; ++lane) {
           const int i = 16 * uo + lane % 16, k = 4 * kb + lane / 16;
-          g[(uo * K + kb) * 64 + lane] = i < nl && k < nl ? mat[i * nl + k] : 0.0;
+          float v = 0.0f;
+          if (i < nl && k < nl) {
+            long double x;
+            if (which == 0) {
+              x = static_cast<long double>(bt.G[i * nl + k]) - (i == k ? 1.0L : 0.0L);
+            } else {
+              x = 0.0L;
+              for (int j = 0; j < nl; ++j) {
+                const int ell = bt.ell_min + j;
+                x += static_cast<long double>(bt.G[i * nl + j]) *
+                     static_cast<long double>(-(ell + 2) * (ell - 1)) *
+                     static_cast<long double>(bt.G[j * nl + k]);
+              }
+              if (i == k) {
+                const int ell = bt.ell_min + i;
+                x -= static_cast<long double>(-(ell + 2) * (ell - 1));
+              }
+            }
+            v = static_cast<float>(x);
+          }
+          e[which * 64 * LT * K + (uo * K + kb) * 64 + lane] = v;
         }
-  }
 }
 
 void Plan::build_lines() {

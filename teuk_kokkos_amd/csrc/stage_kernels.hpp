@@ -78,12 +78,31 @@ __device__ __forceinline__ mfma_acc mfma_f64(double a, double b, mfma_acc c) {
 // global memory (cache resident, shared by every tile of the band), one
 // theta tile ahead; sin and cos of the nodes from LDS.  UB = ell tiles of 16
 // held at once (K <= 4 UB blocks).
+// S and A fragments of theta tile t, lane `lane` (KB = 4 UB S blocks, 4 x UB
+// A blocks).
+template <int UB>
+__device__ __forceinline__ void kerr_load_tile(const double* Sf, const double* Af, int K, int LT,
+                                               int t, int lane, double (&sf)[4 * UB],
+                                               double (&af)[4][UB]) {
+  const double* s = Sf + t * K * 64 + lane;
+#pragma unroll
+  for (int q = 0; q < 4 * UB; ++q) sf[q] = q < K ? s[q * 64] : 0.0;
+  const double* a = Af + t * 4 * LT * 64 + lane;
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int u = 0; u < UB; ++u) af[v][u] = u < LT ? a[(v * LT + u) * 64] : 0.0;
+}
+
+// s0, a0: theta tile 0's fragments when `have0` (requested before X was
+// formed, so their L2 latency is already spent).
 template <int UB>
 __device__ __forceinline__ void kerr_psi_group(int nl, int nt, double spin, const RowCoeffs* RC,
                                                cdouble* PT, int pts, const cdouble* psi, int us,
                                                int g0, int mrows, const double* Sf,
                                                const double* Af, const double* nodes,
-                                               int lane) {
+                                               int lane, const double (&s0)[4 * UB],
+                                               const double (&a0)[4][UB], bool have0) {
   constexpr int KB = 4 * UB;
   const int T = mfma_theta_tiles(nt), K = mfma_ell_blocks(nl), LT = mfma_ell_tiles(nl);
   const int lr = lane & 15, lg = lane >> 4;
@@ -103,16 +122,18 @@ __device__ __forceinline__ void kerr_psi_group(int nl, int nt, double spin, cons
   // the current one is processed.
   double s_cur[KB], s_nxt[KB], a_cur[4][UB], a_nxt[4][UB];
   auto load_tile = [&](int t, double (&sf)[KB], double (&af)[4][UB]) {
-    const double* s = Sf + t * K * 64 + lane;
+    kerr_load_tile<UB>(Sf, Af, K, LT, t, lane, sf, af);
+  };
+  if (have0) {
 #pragma unroll
-    for (int q = 0; q < KB; ++q) sf[q] = q < K ? s[q * 64] : 0.0;
-    const double* a = Af + t * 4 * LT * 64 + lane;
+    for (int q = 0; q < KB; ++q) s_cur[q] = s0[q];
 #pragma unroll
     for (int v = 0; v < 4; ++v)
 #pragma unroll
-      for (int u = 0; u < UB; ++u) af[v][u] = u < LT ? a[(v * LT + u) * 64] : 0.0;
-  };
-  load_tile(0, s_cur, a_cur);
+      for (int u = 0; u < UB; ++u) a_cur[v][u] = a0[v][u];
+  } else {
+    load_tile(0, s_cur, a_cur);
+  }
   for (int t = 0; t < T; ++t) {
     if (t + 1 < T) load_tile(t + 1, s_nxt, a_nxt);
     mfma_acc sxr = {0.0, 0.0, 0.0, 0.0}, sxi = sxr, spr = sxr, spi = sxr;
@@ -360,103 +381,114 @@ __device__ __forceinline__ void plain_output(const StageArgs& a, const TileCtx& 
   }
 }
 
-// G (and M) fragment blocks in flight ahead of the block in use (two or three
-// measured no faster).
-inline constexpr int kGramAhead = 1;
+typedef float mfma_accf __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ mfma_accf mfma_f32(float a, float b, mfma_accf c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
 
-// Output rows [g0, g0 + 16) of the tile (relative to t0) by one wave for
-// bands that need G = A S: k = G X (+ M Psi for P, M = G Lambda G: the
-// reference's S Lambda A acting on the nodal stage) with FP64 MFMA,
-//   k[row][ell] = sum_k X[row][k] G[ell][k]:
-//   A operand X      lane: row g0 + lr, k = 4 kb + lg (pre-projection terms
-//                    straight from the stencils),
-//   B operand G^T    fragment (uo, kb), lane: G[16 uo + lr][4 kb + lg],
-//   accumulator      element w: row g0 + lg + 4 w, ell 16 uo + lr,
-// so each RK load or store instruction covers 16 consecutive ell of four
-// rows.  UB = ell tiles of 16 accumulated at once.
-template <int SCHEME, int STAGE, int UB, bool NT, bool IN>
-__device__ __forceinline__ void gram_output_group(const StageArgs& a, const TileCtx& c,
-                                                  const double* Gf, int g0, int lane) {
+// Gram bands: the projection G = A S of the pre-projection terms X and the
+// Laplacian M Psi (M = G Lambda G) are applied split as G = I + E and
+// M = Lambda + N.  E and N are of the size of A S - I (6e-12 at ell 24,
+// 2e-9 at ell 32), so
+//   k_P = [X_P + Lambda Psi] + [E X_P + N Psi],  k_Q = [D psi_t + X_Q] + E X_Q,
+//   k_Psi = [psi_t + X_Psi] + E X_Psi  (X_Psi: dissipation only)
+// with the first bracket (the identity part, gram_identity) in FP64 at each
+// point and the correction on FP32 MFMA (operands, E, N and sums in FP32):
+// its relative error, about 1e-6, is 1e-17 of k, below FP64 rounding.
+// Both output routines below form these two parts with the same functions in
+// the same order, so they agree bit for bit.
+template <int SCHEME, int F, bool IN>
+__device__ __forceinline__ cdouble gram_identity(const TileCtx& c, int r, int l, cdouble x) {
   const LineDesc& L = *c.L;
   const TileDesc& tile = *c.tile;
-  const int nl = L.n_ell, us = 3 * nl, n = L.nr;
+  const int nl = L.n_ell;
+  if constexpr (F == 0) {
+    const double lam = -static_cast<double>((L.ell_min + l + 2) * (L.ell_min + l - 1));
+    return cadd(x, cscale(lam, c.U[(r - tile.lo) * 3 * nl + 2 * nl + l]));
+  } else if constexpr (F == 1) {
+    return cadd(apply_d<SCHEME, IN>(*c.T, c.PT + l, c.pts, tile.mlo, r, L.nr, L.inv_h), x);
+  } else {
+    return cadd(c.PT[(r - tile.mlo) * c.pts + l], x);
+  }
+}
+
+// One contraction block kb of the correction of field F into the FP32
+// accumulators (uo = output ell tile): E X (+ N Psi for P).  Fragment (uo,
+// kb), lane: E[16 uo + lr][4 kb + lg] (gram_correction_floats).
+template <int F, int UB>
+__device__ __forceinline__ void gram_correction_step(int LT, int K, int kb, int lane,
+                                                     const float* Ef, const float* Nf, float xr,
+                                                     float xi, float pr, float pi,
+                                                     mfma_accf (&cr)[UB], mfma_accf (&ci)[UB]) {
+#pragma unroll
+  for (int uo = 0; uo < UB; ++uo) {
+    if (uo >= LT) break;
+    const float e = Ef[(uo * K + kb) * 64 + lane];
+    cr[uo] = mfma_f32(xr, e, cr[uo]);
+    ci[uo] = mfma_f32(xi, e, ci[uo]);
+    if constexpr (F == 0) {
+      const float nn = Nf[(uo * K + kb) * 64 + lane];
+      cr[uo] = mfma_f32(pr, nn, cr[uo]);
+      ci[uo] = mfma_f32(pi, nn, ci[uo]);
+    }
+  }
+}
+
+// Output rows [g0, g0 + 16) of the tile (relative to t0) by one wave for
+// Gram bands (the general kernel, and tiles with more rows than the split
+// routine below holds).  Per field: the correction from the pre-projection
+// terms at the operand positions (lane: row g0 + lr, ell 4 kb + lg), then at
+// each accumulator position (row g0 + 4 lg + v, ell 16 uo + lr) the identity
+// part from the stencils there, the sum and the RK combination; every RK
+// load or store instruction covers 16 consecutive ell of four rows.
+template <int SCHEME, int STAGE, int UB, bool NT, bool IN>
+__device__ __forceinline__ void gram_output_group(const StageArgs& a, const TileCtx& c,
+                                                  const float* Ef, int g0, int lane) {
+  const LineDesc& L = *c.L;
+  const TileDesc& tile = *c.tile;
+  const int nl = L.n_ell, us = 3 * nl;
   const int LT = mfma_ell_tiles(nl), K = mfma_ell_blocks(nl);
   const int lr = lane & 15, lg = lane >> 4;
   const int orows = tile.t1 - tile.t0;
   // operand row of this lane
   const int r_in = tile.t0 + min(g0 + lr, orows - 1);
   const cdouble* u_in = c.U + (r_in - tile.lo) * us;
-  const double* Mf = Gf + gram_fragment_doubles(nl) / 2;
+  const float* Nf = Ef + gram_correction_floats(nl);
   const cdouble zero = cmake(0.0, 0.0);
-  // accumulator position (uo, w): output row and ell, clamped for addresses
-  auto out_row = [&](int w) { return tile.t0 + min(g0 + lg + 4 * w, orows - 1); };
-  auto out_ok = [&](int uo, int w) { return g0 + lg + 4 * w < orows && 16 * uo + lr < nl; };
-  auto out_index = [&](int F, int uo, int w) {
-    return L.base + static_cast<long long>(out_row(w)) * us + F * nl + min(16 * uo + lr, nl - 1);
+  // accumulator position (uo, v): output row and ell, clamped for addresses
+  auto out_row = [&](int v) { return tile.t0 + min(g0 + 4 * lg + v, orows - 1); };
+  auto out_ok = [&](int uo, int v) { return g0 + 4 * lg + v < orows && 16 * uo + lr < nl; };
+  auto out_index = [&](int F, int uo, int v) {
+    return L.base + static_cast<long long>(out_row(v)) * us + F * nl + min(16 * uo + lr, nl - 1);
   };
-
   constexpr int NR = UB <= 2 ? UB : 1;  // ell tiles of RK inputs requested early
 
   auto field = [&](auto field_tag) {
     constexpr int F = decltype(field_tag)::value;
     const bool need_ys = F == 0 && c.inplace;
-    // RK inputs of this field's accumulator positions (requesting all three
-    // fields' inputs up front exceeds the register budget: spills, slower)
     RkIn in[NR][4];
     if constexpr (UB <= 2) {
 #pragma unroll
       for (int uo = 0; uo < UB; ++uo)
 #pragma unroll
-        for (int w = 0; w < 4; ++w)
-          if (uo < LT) in[uo][w] = rk_load<STAGE>(a, out_index(F, uo, w), need_ys);
+        for (int v = 0; v < 4; ++v)
+          if (uo < LT) in[uo][v] = rk_load<STAGE>(a, out_index(F, uo, v), need_ys);
     }
-    mfma_acc ar[UB], ai[UB];
+    mfma_accf cr[UB], ci[UB];
 #pragma unroll
-    for (int uo = 0; uo < UB; ++uo) ar[uo] = ai[uo] = mfma_acc{0.0, 0.0, 0.0, 0.0};
+    for (int uo = 0; uo < UB; ++uo) cr[uo] = ci[uo] = mfma_accf{0.0f, 0.0f, 0.0f, 0.0f};
     // without dissipation the Psi pre-projection terms vanish
     if (F != 2 || L.dissipate) {
-      // The G (and M) fragments of blocks kb + 1 .. kb + kGramAhead (from L2)
-      // are in flight during block kb.
-      constexpr int GD = kGramAhead;
-      double gq[GD + 1][UB], mq[GD + 1][UB];
-      auto load_block = [&](int kb, double (&g)[UB], double (&m)[UB]) {
-#pragma unroll
-        for (int uo = 0; uo < UB; ++uo) {
-          g[uo] = uo < LT ? Gf[(uo * K + kb) * 64 + lane] : 0.0;
-          if constexpr (F == 0) m[uo] = uo < LT ? Mf[(uo * K + kb) * 64 + lane] : 0.0;
-        }
-      };
-#pragma unroll
-      for (int d = 0; d < GD; ++d)
-        if (d < K) load_block(d, gq[d], mq[d]);
-#pragma unroll
       for (int kb = 0; kb < K; ++kb) {
-        if (kb + GD < K) load_block(kb + GD, gq[GD], mq[GD]);
-        const double (&g_cur)[UB] = gq[0];
-        const double (&m_cur)[UB] = mq[0];
         const int l = 4 * kb + lg;
         cdouble x = zero, ps = zero;
         if (l < nl) {
           x = pre_projection<SCHEME, F, false, IN>(c, r_in, l, zero);
           if constexpr (F == 0) ps = u_in[2 * nl + l];
         }
-#pragma unroll
-        for (int uo = 0; uo < UB; ++uo) {
-          if (uo >= LT) break;
-          ar[uo] = mfma_f64(x.re, g_cur[uo], ar[uo]);
-          ai[uo] = mfma_f64(x.im, g_cur[uo], ai[uo]);
-          if constexpr (F == 0) {
-            ar[uo] = mfma_f64(ps.re, m_cur[uo], ar[uo]);
-            ai[uo] = mfma_f64(ps.im, m_cur[uo], ai[uo]);
-          }
-        }
-#pragma unroll
-        for (int d = 0; d < GD; ++d)
-#pragma unroll
-          for (int uo = 0; uo < UB; ++uo) {
-            gq[d][uo] = gq[d + 1][uo];
-            if constexpr (F == 0) mq[d][uo] = mq[d + 1][uo];
-          }
+        gram_correction_step<F, UB>(LT, K, kb, lane, Ef, Nf, static_cast<float>(x.re),
+                                    static_cast<float>(x.im), static_cast<float>(ps.re),
+                                    static_cast<float>(ps.im), cr, ci);
       }
     }
 #pragma unroll
@@ -465,24 +497,175 @@ __device__ __forceinline__ void gram_output_group(const StageArgs& a, const Tile
       const int slot = UB <= 2 ? uo : 0;
       if constexpr (UB > 2) {
 #pragma unroll
-        for (int w = 0; w < 4; ++w) in[0][w] = rk_load<STAGE>(a, out_index(F, uo, w), need_ys);
+        for (int v = 0; v < 4; ++v) in[0][v] = rk_load<STAGE>(a, out_index(F, uo, v), need_ys);
       }
 #pragma unroll
-      for (int w = 0; w < 4; ++w) {
-        if (!out_ok(uo, w)) continue;
-        const int r = out_row(w), l = 16 * uo + lr;
-        cdouble k = cmake(ar[uo][w], ai[uo][w]);
-        if constexpr (F == 1)
-          k = cadd(k, apply_d<SCHEME, IN>(*c.T, c.PT + l, c.pts, tile.mlo, r, n, L.inv_h));
-        if constexpr (F == 2) k = cadd(k, c.PT[(r - tile.mlo) * c.pts + l]);
-        const cdouble ys = need_ys ? in[slot][w].ys : c.U[(r - tile.lo) * us + F * nl + l];
-        rk_store<STAGE, NT>(a, out_index(F, uo, w), in[slot][w], ys, k, c.dt);
+      for (int v = 0; v < 4; ++v) {
+        if (!out_ok(uo, v)) continue;
+        const int r = out_row(v), l = 16 * uo + lr;
+        const cdouble x = pre_projection<SCHEME, F, false, IN>(c, r, l, zero);
+        const cdouble k = cadd(gram_identity<SCHEME, F, IN>(c, r, l, x),
+                               cmake(static_cast<double>(cr[uo][v]),
+                                     static_cast<double>(ci[uo][v])));
+        const cdouble ys = need_ys ? in[slot][v].ys : c.U[(r - tile.lo) * us + F * nl + l];
+        rk_store<STAGE, NT>(a, out_index(F, uo, v), in[slot][v], ys, k, c.dt);
       }
     }
   };
   field(std::integral_constant<int, 0>{});
   field(std::integral_constant<int, 1>{});
   field(std::integral_constant<int, 2>{});
+}
+
+// The same output for Gram bands without dissipation in free-damped mode
+// (the kStageKerrGram variant), one (row, ell) point per thread like
+// plain_output, for tiles of at most kGramPoints rows per thread:
+//   1. per point: the pre-projection terms, the identity parts of P and Q
+//      (kept in registers) and Psi's whole output (its RK inputs were
+//      requested while psi_t was formed);
+//   2. the FP32 operands X_P, X_Q written over the point's psi_t slot (a
+//      row's psi_t slot holds exactly n_ell complex doubles = 2 n_ell complex
+//      floats), once every thread has read the psi_t stencils;
+//   3. per wave and 16 rows the corrections on FP32 MFMA, written over their
+//      operands;
+//   4. per point: identity part + correction and the RK combination.
+// The stencils run once per point with every thread busy, where the
+// accumulator-position routine above runs them at the operand and at the
+// output positions.
+inline constexpr int kGramPoints = 5;
+template <int STAGE>
+__device__ __forceinline__ void gram_split_prefetch(const StageArgs& a, const LineDesc& L,
+                                                    const TileDesc& tile, int tid,
+                                                    RkIn (&in)[kGramPoints]) {
+  const int nl = L.n_ell;
+  const int lane_l = tid % nl, row_l = tid / nl, step_l = kStageThreads / nl;
+  if (row_l >= step_l) return;
+#pragma unroll
+  for (int j = 0; j < kGramPoints; ++j) {
+    const int r = tile.t0 + row_l + j * step_l;
+    if (r < tile.t1) in[j] = rk_load<STAGE>(a, L.base + static_cast<long long>(r) * 3 * nl + 2 * nl + lane_l);
+  }
+}
+
+__device__ __forceinline__ bool gram_split_fits(const LineDesc& L, const TileDesc& tile) {
+  return tile.t1 - tile.t0 <= kGramPoints * (kStageThreads / L.n_ell);
+}
+
+template <int UB>
+__device__ __forceinline__ void gram_split_correction(const TileCtx& c, const float* Ef, int g0,
+                                                      int lane) {
+  const LineDesc& L = *c.L;
+  const TileDesc& tile = *c.tile;
+  const int nl = L.n_ell, us = 3 * nl;
+  const int LT = mfma_ell_tiles(nl), K = mfma_ell_blocks(nl);
+  const int lr = lane & 15, lg = lane >> 4;
+  const int orows = tile.t1 - tile.t0;
+  const int r_in = tile.t0 + min(g0 + lr, orows - 1);
+  cdouble* urow = const_cast<cdouble*>(c.U) + (r_in - tile.lo) * us;
+  const float* x = reinterpret_cast<const float*>(urow);  // X_P | X_Q as complex floats
+  const float* Nf = Ef + gram_correction_floats(nl);
+  mfma_accf pr[UB], pi[UB], qr[UB], qi[UB];
+#pragma unroll
+  for (int uo = 0; uo < UB; ++uo) pr[uo] = pi[uo] = qr[uo] = qi[uo] = mfma_accf{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int kb = 0; kb < K; ++kb) {
+    const int l = 4 * kb + lg;
+    const bool ok = l < nl;
+    const int lc = ok ? l : 0;
+    const cdouble ps = urow[2 * nl + lc];
+    gram_correction_step<0, UB>(LT, K, kb, lane, Ef, Nf, ok ? x[2 * lc] : 0.0f,
+                                ok ? x[2 * lc + 1] : 0.0f,
+                                ok ? static_cast<float>(ps.re) : 0.0f,
+                                ok ? static_cast<float>(ps.im) : 0.0f, pr, pi);
+    gram_correction_step<1, UB>(LT, K, kb, lane, Ef, Nf, ok ? x[2 * nl + 2 * lc] : 0.0f,
+                                ok ? x[2 * nl + 2 * lc + 1] : 0.0f, 0.0f, 0.0f, qr, qi);
+  }
+  // element v: row g0 + 4 lg + v, ell 16 uo + lr.  This wave alone reads
+  // and writes these rows, and its operand reads were issued first.
+#pragma unroll
+  for (int uo = 0; uo < UB; ++uo)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = g0 + 4 * lg + v, l = 16 * uo + lr;
+      if (uo < LT && row < orows && l < nl) {
+        float* o = reinterpret_cast<float*>(const_cast<cdouble*>(c.U) +
+                                            (tile.t0 + row - tile.lo) * us);
+        o[2 * l] = pr[uo][v];
+        o[2 * l + 1] = pi[uo][v];
+        o[2 * nl + 2 * l] = qr[uo][v];
+        o[2 * nl + 2 * l + 1] = qi[uo][v];
+      }
+    }
+}
+
+template <int SCHEME, int STAGE, int UB, bool IN>
+__device__ __forceinline__ void gram_split_output(const StageArgs& a, const TileCtx& c,
+                                                  const float* Ef, int tid,
+                                                  RkIn (&psi_in)[kGramPoints]) {
+  constexpr int NP = kGramPoints;
+  const LineDesc& L = *c.L;
+  const TileDesc& tile = *c.tile;
+  const int nl = L.n_ell, us = 3 * nl;
+  const int lane_l = tid % nl, row_l = tid / nl, step_l = kStageThreads / nl;
+  const bool on = row_l < step_l;
+  const cdouble zero = cmake(0.0, 0.0);
+  cdouble kP[NP], kQ[NP];
+  float xf[NP][4];
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const int r = tile.t0 + row_l + j * step_l;
+    if (on && r < tile.t1) {
+      const cdouble xp = pre_projection<SCHEME, 0, false, IN>(c, r, lane_l, zero);
+      const cdouble xq = pre_projection<SCHEME, 1, false, IN>(c, r, lane_l, zero);
+      const cdouble xs = pre_projection<SCHEME, 2, false, IN>(c, r, lane_l, zero);
+      kP[j] = gram_identity<SCHEME, 0, IN>(c, r, lane_l, xp);
+      kQ[j] = gram_identity<SCHEME, 1, IN>(c, r, lane_l, xq);
+      const cdouble ks = cadd(gram_identity<SCHEME, 2, IN>(c, r, lane_l, xs), cmake(0.0, 0.0));
+      xf[j][0] = static_cast<float>(xp.re);
+      xf[j][1] = static_cast<float>(xp.im);
+      xf[j][2] = static_cast<float>(xq.re);
+      xf[j][3] = static_cast<float>(xq.im);
+      const long long g = L.base + static_cast<long long>(r) * us + lane_l;
+      rk_store<STAGE>(a, g + 2 * nl, psi_in[j], c.U[(r - tile.lo) * us + 2 * nl + lane_l], ks,
+                      c.dt);
+    }
+  }
+  __syncthreads();  // every psi_t stencil has been read
+  RkIn inP[NP], inQ[NP];
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const int r = tile.t0 + row_l + j * step_l;
+    if (on && r < tile.t1) {
+      float* o = reinterpret_cast<float*>(const_cast<cdouble*>(c.U) + (r - tile.lo) * us);
+      o[2 * lane_l] = xf[j][0];
+      o[2 * lane_l + 1] = xf[j][1];
+      o[2 * nl + 2 * lane_l] = xf[j][2];
+      o[2 * nl + 2 * lane_l + 1] = xf[j][3];
+      const long long g = L.base + static_cast<long long>(r) * us + lane_l;
+      inP[j] = rk_load<STAGE>(a, g, c.inplace);
+      inQ[j] = rk_load<STAGE>(a, g + nl);
+    }
+  }
+  __syncthreads();
+  const int wave = tid >> 6, lane = tid & 63;
+  const int orows = tile.t1 - tile.t0;
+  for (int g0 = 16 * wave; g0 < orows; g0 += 16 * (kStageThreads / 64))
+    gram_split_correction<UB>(c, Ef, g0, lane);
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const int r = tile.t0 + row_l + j * step_l;
+    if (on && r < tile.t1) {
+      const float* o = reinterpret_cast<const float*>(c.U + (r - tile.lo) * us);
+      const cdouble kp = cadd(kP[j], cmake(static_cast<double>(o[2 * lane_l]),
+                                           static_cast<double>(o[2 * lane_l + 1])));
+      const cdouble kq = cadd(kQ[j], cmake(static_cast<double>(o[2 * nl + 2 * lane_l]),
+                                           static_cast<double>(o[2 * nl + 2 * lane_l + 1])));
+      const long long g = L.base + static_cast<long long>(r) * us + lane_l;
+      rk_store<STAGE>(a, g, inP[j], c.inplace ? inP[j].ys : c.U[(r - tile.lo) * us + lane_l], kp,
+                      c.dt);
+      rk_store<STAGE>(a, g + nl, inQ[j], c.U[(r - tile.lo) * us + nl + lane_l], kq, c.dt);
+    }
+  }
 }
 
 // LDS pointers of one pass of the stage kernels.  A pass covers the rows of
@@ -525,6 +708,11 @@ __device__ __forceinline__ void psi_t_phase(const StageArgs& a, const LineDesc& 
   const RowCoeffs* RC = s.RC;
   const cdouble* U = s.U;
   const bool constrained = CONSTRAINED && L.constrained;
+  // theta tile 0's S and A fragments, in flight while X is formed
+  const double* Sf = a.mats + L.frag_off;
+  const double* Af = Sf + 64 * mfma_s_fragments(nt, nl);
+  double s0[4 * UB], a0[4][UB];
+  if (L.kerr) kerr_load_tile<UB>(Sf, Af, mfma_ell_blocks(nl), mfma_ell_tiles(nl), 0, lane, s0, a0);
   if (constrained) {
     if (on_l)
       for (int j = tile.mlo + row_l; j < tile.mhi; j += step_l)
@@ -551,12 +739,10 @@ __device__ __forceinline__ void psi_t_phase(const StageArgs& a, const LineDesc& 
     }
   __syncthreads();
   if (L.kerr) {
-    const double* Sf = a.mats + L.frag_off;
-    const double* Af = Sf + 64 * mfma_s_fragments(nt, nl);
     const cdouble* psi = U + (tile.mlo - tile.lo) * us + 2 * nl;
     for (int g0 = 16 * wave; g0 < mrows; g0 += 16 * (kStageThreads / 64))
       kerr_psi_group<UB>(nl, nt, L.spin, RC + (tile.mlo - s.rc_lo), s.PT, s.pts, psi, us, g0,
-                         mrows, Sf, Af, s.NODES, lane);
+                         mrows, Sf, Af, s.NODES, lane, s0, a0, g0 == 16 * wave);
     __syncthreads();
   }
 }
@@ -566,7 +752,8 @@ __device__ __forceinline__ void psi_t_phase(const StageArgs& a, const LineDesc& 
 template <int SCHEME, int STAGE, int UB, int VARIANT, bool IN>
 __device__ __forceinline__ void output_phase(const StageArgs& a, const LineDesc& L,
                                              const TileDesc& tile, const StageLds& s, int tid,
-                                             RkIn (&pre)[kPlainDepth<UB>][3]) {
+                                             RkIn (&pre)[kPlainDepth<UB>][3],
+                                             RkIn (&psi_in)[kGramPoints]) {
   const int wave = tid >> 6, lane = tid & 63;
   const int orows = tile.t1 - tile.t0;
   TileCtx c{&L,     &tile, s.T,       s.RC,      s.rc_lo, s.U, s.PT, s.DP, s.pts, s.inplace,
@@ -575,9 +762,15 @@ __device__ __forceinline__ void output_phase(const StageArgs& a, const LineDesc&
     plain_output<SCHEME, STAGE, kPlainDepth<UB>, IN>(a, c, tid, pre);
     return;
   }
-  const double* Gf = a.mats + L.gfrag_off;
+  const float* Ef = reinterpret_cast<const float*>(a.mats + L.gfrag_off);
+  if constexpr (VARIANT == kStageKerrGram) {
+    if (gram_split_fits(L, tile)) {
+      gram_split_output<SCHEME, STAGE, UB, IN>(a, c, Ef, tid, psi_in);
+      return;
+    }
+  }
   for (int g0 = 16 * wave; g0 < orows; g0 += 16 * (kStageThreads / 64))
-    gram_output_group<SCHEME, STAGE, UB, VARIANT == kStageKerrGram, IN>(a, c, Gf, g0, lane);
+    gram_output_group<SCHEME, STAGE, UB, VARIANT == kStageKerrGram, IN>(a, c, Ef, g0, lane);
 }
 
 // One radial tile of one line; UB = ell tiles of 16 the MFMA routines hold
@@ -626,8 +819,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const TileDesc ti
   // (Requested before the staging loads they delay the staging, which the
   // rest of the tile waits for: slower.)
   RkIn pre[kPlainDepth<UB>][3];
+  RkIn psi_in[kGramPoints];
   stage_copy(s.U, a.yin + L.base + static_cast<long long>(tile.lo) * us, rows * us, tid);
   if (!L.gram) plain_prefetch<STAGE>(a, L, tile, s.inplace, tid, pre);
+  if constexpr (VARIANT == kStageKerrGram)
+    if (gram_split_fits(L, tile)) gram_split_prefetch<STAGE>(a, L, tile, tid, psi_in);
   for (int j = tid; j < static_cast<int>(sizeof(SchemeTable) / 8); j += kStageThreads)
     smem[j] = reinterpret_cast<const double*>(a.table)[j];
   if (L.kerr)
@@ -639,9 +835,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const TileDesc ti
   psi_t_phase<SCHEME, UB>(a, L, tile, s, tid);
   constexpr int W = SchemeShape<SCHEME>::W;
   if (tile.t0 >= W && tile.t1 + W <= L.nr)
-    output_phase<SCHEME, STAGE, UB, VARIANT, true>(a, L, tile, s, tid, pre);
+    output_phase<SCHEME, STAGE, UB, VARIANT, true>(a, L, tile, s, tid, pre, psi_in);
   else
-    output_phase<SCHEME, STAGE, UB, VARIANT, false>(a, L, tile, s, tid, pre);
+    output_phase<SCHEME, STAGE, UB, VARIANT, false>(a, L, tile, s, tid, pre, psi_in);
 }
 
 // Workgroups are dispatched to the 8 XCDs in turn (workgroup b to XCD b mod
