@@ -134,13 +134,20 @@ __device__ __forceinline__ void kerr_psi_group(int nl, int nt, double spin, cons
   } else {
     load_tile(0, s_cur, a_cur);
   }
+  // Wait here for every load issued so far (tile 0's fragments among them).
+  // Otherwise the compiler's wait at the loop head, which must cover them on
+  // the first pass, also covers the next tile's fragments on every later
+  // pass, and their cache latency is paid in full once per theta tile.
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0); expcnt and lgkmcnt open
   for (int t = 0; t < T; ++t) {
     if (t + 1 < T) load_tile(t + 1, s_nxt, a_nxt);
     mfma_acc sxr = {0.0, 0.0, 0.0, 0.0}, sxi = sxr, spr = sxr, spi = sxr;
 #pragma unroll
     for (int q = 0; q < KB; ++q) {
       if (q >= K) break;
-      // B operands X, Psi of (row jj, ell 4 q + lg) from LDS
+      // B operands X, Psi of (row jj, ell 4 q + lg) from LDS.  (Reading
+      // block q + 1's while block q's products run, and keeping block 0's
+      // for every theta tile, measured slower: more registers.)
       const int l = 4 * q + lg;
       const bool ok = row_ok && l < nl;
       const int lc = ok ? l : 0;
@@ -547,13 +554,47 @@ __device__ __forceinline__ void gram_split_prefetch(const StageArgs& a, const Li
   }
 }
 
+// The split output serves the Gram variant up to two ell tiles (its E and N
+// fragments live in registers); wider bands take gram_output_group.
+template <int VARIANT, int UB>
+constexpr bool split_gram() {
+  return VARIANT == kStageKerrGram && UB <= 2;
+}
+
 __device__ __forceinline__ bool gram_split_fits(const LineDesc& L, const TileDesc& tile) {
   return tile.t1 - tile.t0 <= kGramPoints * (kStageThreads / L.n_ell);
 }
 
+// This lane's E and N fragments of every (output ell tile, contraction
+// block): requested once per tile, ahead of the correction, so it finds them
+// in registers instead of waiting on a cache load per block.
 template <int UB>
-__device__ __forceinline__ void gram_split_correction(const TileCtx& c, const float* Ef, int g0,
-                                                      int lane) {
+struct GramFragments {
+  float e[UB][4 * UB], n[UB][4 * UB];
+};
+template <int UB>
+__device__ __forceinline__ void gram_load_fragments(const float* Ef, int nl, int lane,
+                                                    GramFragments<UB>& f) {
+  const int LT = mfma_ell_tiles(nl), K = mfma_ell_blocks(nl);
+  const float* Nf = Ef + gram_correction_floats(nl);
+#pragma unroll
+  for (int uo = 0; uo < UB; ++uo)
+#pragma unroll
+    for (int kb = 0; kb < 4 * UB; ++kb) {
+      const bool ok = uo < LT && kb < K;
+      f.e[uo][kb] = ok ? Ef[(uo * K + kb) * 64 + lane] : 0.0f;
+      f.n[uo][kb] = ok ? Nf[(uo * K + kb) * 64 + lane] : 0.0f;
+    }
+}
+
+// Corrections E X_P + N Psi and E X_Q of rows [g0, g0 + 16) on FP32 MFMA,
+// written over their operands.  Every accumulator takes its terms in the
+// order of gram_correction_step (per block: E X, then N Psi), so the result
+// equals gram_output_group's bit for bit.  The LDS operands of block kb + 1
+// are read while block kb's products run.
+template <int UB>
+__device__ __forceinline__ void gram_split_correction(const TileCtx& c, const GramFragments<UB>& f,
+                                                      int g0, int lane) {
   const LineDesc& L = *c.L;
   const TileDesc& tile = *c.tile;
   const int nl = L.n_ell, us = 3 * nl;
@@ -563,21 +604,44 @@ __device__ __forceinline__ void gram_split_correction(const TileCtx& c, const fl
   const int r_in = tile.t0 + min(g0 + lr, orows - 1);
   cdouble* urow = const_cast<cdouble*>(c.U) + (r_in - tile.lo) * us;
   const float* x = reinterpret_cast<const float*>(urow);  // X_P | X_Q as complex floats
-  const float* Nf = Ef + gram_correction_floats(nl);
   mfma_accf pr[UB], pi[UB], qr[UB], qi[UB];
 #pragma unroll
   for (int uo = 0; uo < UB; ++uo) pr[uo] = pi[uo] = qr[uo] = qi[uo] = mfma_accf{0.0f, 0.0f, 0.0f, 0.0f};
-  for (int kb = 0; kb < K; ++kb) {
+  // operands of block kb at (row r_in, ell 4 kb + lg): X_P, Psi, X_Q
+  auto operands = [&](int kb, float (&o)[6]) {
     const int l = 4 * kb + lg;
     const bool ok = l < nl;
     const int lc = ok ? l : 0;
     const cdouble ps = urow[2 * nl + lc];
-    gram_correction_step<0, UB>(LT, K, kb, lane, Ef, Nf, ok ? x[2 * lc] : 0.0f,
-                                ok ? x[2 * lc + 1] : 0.0f,
-                                ok ? static_cast<float>(ps.re) : 0.0f,
-                                ok ? static_cast<float>(ps.im) : 0.0f, pr, pi);
-    gram_correction_step<1, UB>(LT, K, kb, lane, Ef, Nf, ok ? x[2 * nl + 2 * lc] : 0.0f,
-                                ok ? x[2 * nl + 2 * lc + 1] : 0.0f, 0.0f, 0.0f, qr, qi);
+    o[0] = ok ? x[2 * lc] : 0.0f;
+    o[1] = ok ? x[2 * lc + 1] : 0.0f;
+    o[2] = ok ? static_cast<float>(ps.re) : 0.0f;
+    o[3] = ok ? static_cast<float>(ps.im) : 0.0f;
+    o[4] = ok ? x[2 * nl + 2 * lc] : 0.0f;
+    o[5] = ok ? x[2 * nl + 2 * lc + 1] : 0.0f;
+  };
+  float cur[6], nxt[6];
+  operands(0, cur);
+#pragma unroll
+  for (int kb = 0; kb < 4 * UB; ++kb) {
+    if (kb >= K) break;
+    if (kb + 1 < K) operands(kb + 1, nxt);
+#pragma unroll
+    for (int uo = 0; uo < UB; ++uo) {
+      if (uo >= LT) break;
+      pr[uo] = mfma_f32(cur[0], f.e[uo][kb], pr[uo]);
+      pi[uo] = mfma_f32(cur[1], f.e[uo][kb], pi[uo]);
+      pr[uo] = mfma_f32(cur[2], f.n[uo][kb], pr[uo]);
+      pi[uo] = mfma_f32(cur[3], f.n[uo][kb], pi[uo]);
+    }
+#pragma unroll
+    for (int uo = 0; uo < UB; ++uo) {
+      if (uo >= LT) break;
+      qr[uo] = mfma_f32(cur[4], f.e[uo][kb], qr[uo]);
+      qi[uo] = mfma_f32(cur[5], f.e[uo][kb], qi[uo]);
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) cur[i] = nxt[i];
   }
   // element v: row g0 + 4 lg + v, ell 16 uo + lr.  This wave alone reads
   // and writes these rows, and its operand reads were issued first.
@@ -629,6 +693,9 @@ __device__ __forceinline__ void gram_split_output(const StageArgs& a, const Tile
                       c.dt);
     }
   }
+  // E and N fragments for the correction, in flight across the barriers
+  GramFragments<UB> frag;
+  gram_load_fragments<UB>(Ef, nl, tid & 63, frag);
   __syncthreads();  // every psi_t stencil has been read
   RkIn inP[NP], inQ[NP];
 #pragma unroll
@@ -649,7 +716,7 @@ __device__ __forceinline__ void gram_split_output(const StageArgs& a, const Tile
   const int wave = tid >> 6, lane = tid & 63;
   const int orows = tile.t1 - tile.t0;
   for (int g0 = 16 * wave; g0 < orows; g0 += 16 * (kStageThreads / 64))
-    gram_split_correction<UB>(c, Ef, g0, lane);
+    gram_split_correction<UB>(c, frag, g0, lane);
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < NP; ++j) {
@@ -763,7 +830,7 @@ __device__ __forceinline__ void output_phase(const StageArgs& a, const LineDesc&
     return;
   }
   const float* Ef = reinterpret_cast<const float*>(a.mats + L.gfrag_off);
-  if constexpr (VARIANT == kStageKerrGram) {
+  if constexpr (split_gram<VARIANT, UB>()) {
     if (gram_split_fits(L, tile)) {
       gram_split_output<SCHEME, STAGE, UB, IN>(a, c, Ef, tid, psi_in);
       return;
@@ -818,16 +885,33 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, const TileDesc ti
   //    bands) are requested once the staging loads are issued.
   // (Requested before the staging loads they delay the staging, which the
   // rest of the tile waits for: slower.)
+  // The SBP table and the sines and cosines of the nodes, a few doubles per
+  // thread, are requested ahead of the staging loads and written to LDS after
+  // them: their latency hides behind the staging instead of following it, and
+  // no wait for them also waits for the RK inputs requested below.
+  constexpr int kTab = static_cast<int>(sizeof(SchemeTable) / 8);
+  constexpr int kTabPer = (kTab + kStageThreads - 1) / kStageThreads;
+  const double* tab = reinterpret_cast<const double*>(a.table);
+  double tabv[kTabPer];
+#pragma unroll
+  for (int k = 0; k < kTabPer; ++k) tabv[k] = tab[min(tid + k * kStageThreads, kTab - 1)];
+  const int nn = L.kerr ? 2 * nt : 0;
+  double nodev = 0.0;
+  if (tid < nn) nodev = a.nodes[L.node_off + tid];
   RkIn pre[kPlainDepth<UB>][3];
   RkIn psi_in[kGramPoints];
   stage_copy(s.U, a.yin + L.base + static_cast<long long>(tile.lo) * us, rows * us, tid);
+  // (written before the RK inputs are requested: the compiler's wait for
+  // these values would otherwise also cover those loads)
+#pragma unroll
+  for (int k = 0; k < kTabPer; ++k)
+    if (tid + k * kStageThreads < kTab) smem[tid + k * kStageThreads] = tabv[k];
+  if (tid < nn) s.NODES[tid] = nodev;
   if (!L.gram) plain_prefetch<STAGE>(a, L, tile, s.inplace, tid, pre);
-  if constexpr (VARIANT == kStageKerrGram)
+  if constexpr (split_gram<VARIANT, UB>())
     if (gram_split_fits(L, tile)) gram_split_prefetch<STAGE>(a, L, tile, tid, psi_in);
-  for (int j = tid; j < static_cast<int>(sizeof(SchemeTable) / 8); j += kStageThreads)
-    smem[j] = reinterpret_cast<const double*>(a.table)[j];
-  if (L.kerr)
-    for (int j = tid; j < 2 * nt; j += kStageThreads) s.NODES[j] = a.nodes[L.node_off + j];
+  for (int j = tid + kStageThreads; j < nn; j += kStageThreads)
+    s.NODES[j] = a.nodes[L.node_off + j];
   row_coefficients(L, s.RC, tile.mlo, tile.mlo, tile.mhi, tid);
   __syncthreads();
 
