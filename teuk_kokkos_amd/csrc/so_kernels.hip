@@ -267,7 +267,9 @@ __device__ __forceinline__ cdouble row_project(const double* ops, int an_off, in
 // 756-766 and, for KIND 2, the outer operator and coordinate forcing of
 // source_spatial.hpp:357-425, second_order.hpp:152-182.
 template <int SCHEME, int KIND>
-__global__ __launch_bounds__(kSoThreads) void so_linear_kernel(SoArgs a) {
+// Four waves per SIMD (at most 128 registers): measured 5 % faster per
+// second-order step than the three the unconstrained allocation allows.
+__global__ __launch_bounds__(kSoThreads, 4) void so_linear_kernel(SoArgs a) {
   extern __shared__ __align__(16) double smem[];
   const SoTile tile = a.tiles[blockIdx.x];
   const SoInstance& I = a.inst[tile.inst];
@@ -489,7 +491,9 @@ __global__ __launch_bounds__(kSoThreads) void so_drad_kernel(SoArgs a, int which
 // operators run on FP64 MFMA over the workgroup's rows (apply_rows) and
 // leave their results in these rows.
 template <int KIND, int NT>
-__global__ __launch_bounds__(NT) void so_recon_kernel(SoArgs a) {
+// The 256-thread form at three waves per SIMD (at most 168 registers, a few
+// spilled): measured 5 % faster per second-order step than at two.
+__global__ __launch_bounds__(NT, NT == 256 ? 3 : 1) void so_recon_kernel(SoArgs a) {
   constexpr int CH = 8;  // operator loads in flight per row_op chunk
   extern __shared__ __align__(16) double smem[];
   const int inst = a.first_inst + blockIdx.y;
@@ -843,6 +847,8 @@ __global__ __launch_bounds__(NT) void so_recon_kernel(SoArgs a) {
 // targets, point block 8 (x / (8 targets)) + x % 8).  Workgroups x and x + 8
 // go to the same XCD, so the targets of one point block, which read the same
 // parent values, meet in one L2 at about the same time.
+// (Two waves per SIMD spill about 108 registers here and measured 17 %
+// slower per second-order step; the kernel keeps one.)
 __global__ __launch_bounds__(kSoThreads) void so_source_kernel(SoArgs a, int max_targets) {
   const SoInstance& I = a.inst[a.first_inst + blockIdx.y];
   // The instance record is read through a pointer the compiler cannot prove

@@ -546,11 +546,14 @@ __device__ __forceinline__ void gram_split_prefetch(const StageArgs& a, const Li
                                                     RkIn (&in)[kGramPoints]) {
   const int nl = L.n_ell;
   const int lane_l = tid % nl, row_l = tid / nl, step_l = kStageThreads / nl;
-  if (row_l >= step_l) return;
+  // Unconditional loads (rows past the tile clamped to its last row, their
+  // values unused): with every load issued on every path, the compiler's
+  // waits can count exactly which loads a use needs, instead of waiting for
+  // every load in flight.
 #pragma unroll
   for (int j = 0; j < kGramPoints; ++j) {
-    const int r = tile.t0 + row_l + j * step_l;
-    if (r < tile.t1) in[j] = rk_load<STAGE>(a, L.base + static_cast<long long>(r) * 3 * nl + 2 * nl + lane_l);
+    const int r = min(tile.t0 + row_l + j * step_l, tile.t1 - 1);
+    in[j] = rk_load<STAGE>(a, L.base + static_cast<long long>(r) * 3 * nl + 2 * nl + lane_l);
   }
 }
 
@@ -577,13 +580,17 @@ __device__ __forceinline__ void gram_load_fragments(const float* Ef, int nl, int
                                                     GramFragments<UB>& f) {
   const int LT = mfma_ell_tiles(nl), K = mfma_ell_blocks(nl);
   const float* Nf = Ef + gram_correction_floats(nl);
+  // unconditional loads (clamped to the last fragment), as in
+  // gram_split_prefetch, so the correction waits for these alone
 #pragma unroll
   for (int uo = 0; uo < UB; ++uo)
 #pragma unroll
     for (int kb = 0; kb < 4 * UB; ++kb) {
       const bool ok = uo < LT && kb < K;
-      f.e[uo][kb] = ok ? Ef[(uo * K + kb) * 64 + lane] : 0.0f;
-      f.n[uo][kb] = ok ? Nf[(uo * K + kb) * 64 + lane] : 0.0f;
+      const int fi = min(uo * K + kb, LT * K - 1) * 64 + lane;
+      const float e = Ef[fi], n = Nf[fi];
+      f.e[uo][kb] = ok ? e : 0.0f;
+      f.n[uo][kb] = ok ? n : 0.0f;
     }
 }
 
@@ -707,10 +714,11 @@ __device__ __forceinline__ void gram_split_output(const StageArgs& a, const Tile
       o[2 * lane_l + 1] = xf[j][1];
       o[2 * nl + 2 * lane_l] = xf[j][2];
       o[2 * nl + 2 * lane_l + 1] = xf[j][3];
-      const long long g = L.base + static_cast<long long>(r) * us + lane_l;
-      inP[j] = rk_load<STAGE>(a, g, c.inplace);
-      inQ[j] = rk_load<STAGE>(a, g + nl);
     }
+    // unconditional loads (clamped rows), as in gram_split_prefetch
+    const long long g = L.base + static_cast<long long>(min(r, tile.t1 - 1)) * us + lane_l;
+    inP[j] = rk_load<STAGE>(a, g, c.inplace);
+    inQ[j] = rk_load<STAGE>(a, g + nl);
   }
   __syncthreads();
   const int wave = tid >> 6, lane = tid & 63;
