@@ -179,9 +179,10 @@ def test_gram_split_and_group_outputs_agree_and_match_oracle(ell):
 def test_specialised_stage_kernels_match_general_kernel_and_oracle(scheme, ell):
     """Kerr lines without dissipation in free-damped mode run stage kernels
     with the line flags fixed at compile time (G = I for ell <= 16, the Gram
-    projection at ell 24).  They must reproduce the general kernel bit for
-    bit (the same arithmetic in the same order) and the oracle to the FP64
-    bar."""
+    projection at ell 24).  At ell <= 16 they must reproduce the general
+    kernel bit for bit (the same arithmetic in the same order); at ell 24
+    the specialised kernel's BF16 Gram correction agrees with the general
+    kernel's FP32 one to 1e-13.  Both meet the oracle's FP64 bar."""
     params = H.small_case(spin=0.8, nr=229, radial_discretization=scheme, ellmax_first=ell,
                           ellmax_second=ell, seeds=[(2, 2, 1e-3), (5, 2, 3e-4j)])
     out = {}
@@ -193,7 +194,15 @@ def test_specialised_stage_kernels_match_general_kernel_and_oracle(scheme, ell):
             plan.initialize()
             plan.advance(3)
             out[special] = plan.export_state()
-    assert np.array_equal(out[True], out[False])
+    if ell <= 16:
+        assert np.array_equal(out[True], out[False])
+    else:
+        # the Kerr-Gram kernel takes the ell-24 band's Gram correction on
+        # BF16 operands, the general kernel on FP32 ones: the correction is
+        # 6e-12 of k and BF16 keeps it to 4e-3, so the two agree to 1e-13
+        diff = H.rel_err(out[True], out[False])
+        print(f"[gpu] specialised (BF16 correction) vs general (FP32) {diff:.3e}", flush=True)
+        assert diff <= 1e-13
     ref = H.oracle_initialized(params)
     ref.advance(3, params["final_time"] / params["steps"])
     err = H.rel_err(out[True], ref.get_state())

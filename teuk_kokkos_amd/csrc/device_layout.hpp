@@ -22,7 +22,7 @@ struct LineDesc {
   int nr, n_ell, ntheta, ell_min;
   int m, constrained, kerr, dissipate;
   int gram;         // apply G = A S (its deviation from I exceeds roundoff)
-  int gram_pad;
+  int gram_bf16;    // the correction of the Gram projection on BF16 MFMA (below)
   // double offset of S (ntheta x n_ell), then A (n_ell x ntheta),
   // G = A S (n_ell x n_ell) and B = G^-1 A (n_ell x ntheta)
   int mat_off;
@@ -88,6 +88,17 @@ __host__ __device__ inline int gram_correction_floats(int n_ell) {
 }
 __host__ __device__ inline int gram_fragment_doubles(int n_ell) {
   return gram_correction_floats(n_ell);  // two float matrices
+}
+// The same two matrices as BF16 operands of v_mfma_f32_16x16x32_bf16 for
+// bands of at most 32 ell values (one contraction block covers the band),
+// stored after the FP32 fragments: fragment (which = E or N, uo), lane holds
+// the eight values E[16 uo + lr][8 lg + j], j = 0..7 (lane layout measured on
+// gfx950: A operand A[lr][8 lg + j], B operand B[8 lg + j][lr], accumulator
+// element v D[4 lg + v][lr]).  Lines whose E and N are small enough take
+// them (LineDesc::gram_bf16).
+inline constexpr int kGramBf16MaxEll = 32;
+__host__ __device__ inline int gram_bf16_doubles(int n_ell) {
+  return n_ell <= kGramBf16MaxEll ? 2 * mfma_ell_tiles(n_ell) * 64 * 2 : 0;
 }
 // Widest band the MFMA routines hold (4 ell tiles of 16).
 inline constexpr int kMaxBandEll = 64;

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <map>
 #include <stdexcept>
@@ -153,7 +154,22 @@ Plan::~Plan() {
 
 // S and A of one band in the operand order of the Kerr MFMA kernels (the
 // fragment layout is described with mfma_fragment_doubles).
-static void append_mfma_fragments(const BandTransform& bt, int nt, std::vector<double>& out) {
+// Round a float to BF16, to nearest with ties to even (finite values).
+static uint16_t bf16_bits(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, sizeof u);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return static_cast<uint16_t>(u >> 16);
+}
+
+// Largest |E| and |N| / max|Lambda| for which the Gram correction runs on
+// BF16 operands.  BF16 keeps 8 significant bits, so the correction's own
+// relative error is about 4e-3; below these sizes that is under 1e-13 of the
+// terms it corrects (ell 24 bands: |E| about 6e-12).
+constexpr double kGramBf16Limit = 2e-11;
+
+// Returns whether the band's Gram correction may take the BF16 fragments.
+static bool append_mfma_fragments(const BandTransform& bt, int nt, std::vector<double>& out) {
   const int nl = bt.modes();
   const int T = mfma_theta_tiles(nt), K = mfma_ell_blocks(nl), LT = mfma_ell_tiles(nl);
   const size_t first = out.size();
@@ -179,8 +195,13 @@ static void append_mfma_fragments(const BandTransform& bt, int nt, std::vector<d
   // E[16 uo + lr][4 kb + lg] (gram_correction_floats), both summed in long
   // double from the double G before the final rounding.
   const size_t efirst = out.size();
-  out.resize(efirst + gram_fragment_doubles(nl), 0.0);
+  out.resize(efirst + gram_fragment_doubles(nl) + gram_bf16_doubles(nl), 0.0);
   float* e = reinterpret_cast<float*>(out.data() + efirst);
+  long double emax = 0.0L, nmax = 0.0L, lmax = 0.0L;
+  for (int j = 0; j < nl; ++j) {
+    const int ell = bt.ell_min + j;
+    lmax = std::max(lmax, static_cast<long double>((ell + 2) * (ell - 1)));
+  }
   for (int which = 0; which < 2; ++which)
     for (int uo = 0; uo < LT; ++uo)
       for (int kb = 0; kb < K; ++kb)
@@ -205,14 +226,32 @@ static void append_mfma_fragments(const BandTransform& bt, int nt, std::vector<d
               }
             }
             v = static_cast<float>(x);
+            (which == 0 ? emax : nmax) = std::max(which == 0 ? emax : nmax, fabsl(x));
           }
           e[which * 64 * LT * K + (uo * K + kb) * 64 + lane] = v;
         }
+  if (nl > kGramBf16MaxEll) return false;
+  // BF16 fragments: (which, uo), lane, j -> E[16 uo + lr][8 lg + j], the
+  // FP32 value rounded once more
+  uint16_t* b = reinterpret_cast<uint16_t*>(out.data() + efirst + gram_fragment_doubles(nl));
+  for (int which = 0; which < 2; ++which)
+    for (int uo = 0; uo < LT; ++uo)
+      for (int lane = 0; lane < 64; ++lane)
+        for (int j = 0; j < 8; ++j) {
+          const int i = 16 * uo + lane % 16, k = 8 * (lane / 16) + j;
+          float v = 0.0f;
+          if (i < nl && k < nl) {
+            // the FP32 fragment holding E[i][k]: (uo, kb = k / 4), lane i % 16 + 16 (k % 4)
+            v = e[which * 64 * LT * K + (uo * K + k / 4) * 64 + (i % 16) + 16 * (k % 4)];
+          }
+          b[((which * LT + uo) * 64 + lane) * 8 + j] = bf16_bits(v);
+        }
+  return emax <= kGramBf16Limit && nmax <= kGramBf16Limit * lmax;
 }
 
 void Plan::build_lines() {
   // (m, E, ntheta) -> offsets of the dense matrices and of the MFMA fragments
-  std::map<std::tuple<int, int, int>, std::pair<int, int>> mat_index;
+  std::map<std::tuple<int, int, int>, std::tuple<int, int, int>> mat_index;
   long long base = 0;
   for (int i = 0; i < instance_count(); ++i) {
     Instance& in = inst_[i];
@@ -225,7 +264,7 @@ void Plan::build_lines() {
       const int m = in.registry.modes()[mi];
       if (!in.registry.is_parent(m)) continue;
       const auto key = std::make_tuple(m, p.ell_max_first, in.T);
-      int off, frag_off;
+      int off, frag_off, bf16_ok;
       auto it = mat_index.find(key);
       if (it == mat_index.end()) {
         const BandTransform bt = make_band_transform(-2, m, p.ell_max_first, in.T);
@@ -238,11 +277,10 @@ void Plan::build_lines() {
         mats_.insert(mats_.end(), B.begin(), B.end());
         mats_.resize((mats_.size() + 1) / 2 * 2, 0.0);  // fragments start 16-byte aligned
         frag_off = static_cast<int>(mats_.size());
-        append_mfma_fragments(bt, in.T, mats_);
-        mat_index[key] = {off, frag_off};
+        bf16_ok = append_mfma_fragments(bt, in.T, mats_) ? 1 : 0;
+        mat_index[key] = {off, frag_off, bf16_ok};
       } else {
-        off = it->second.first;
-        frag_off = it->second.second;
+        std::tie(off, frag_off, bf16_ok) = it->second;
       }
       const int n_ell = p.ell_max_first - minimum_ell(-2, m) + 1;
       if (n_ell > kMaxBandEll)
@@ -259,6 +297,7 @@ void Plan::build_lines() {
       L.kerr = p.spin != 0.0 ? 1 : 0;
       L.dissipate = p.dissipation > 0.0 ? 1 : 0;
       L.gram = band_gram_needed(-2, m, p.ell_max_first, in.T) ? 1 : 0;
+      L.gram_bf16 = L.gram && bf16_ok ? 1 : 0;
       L.mat_off = off;
       L.frag_off = frag_off;
       L.gfrag_off = frag_off + mfma_fragment_doubles(in.T, n_ell);

@@ -441,6 +441,45 @@ __device__ __forceinline__ void gram_correction_step(int LT, int K, int kb, int 
   }
 }
 
+// BF16 form of the correction for lines with LineDesc::gram_bf16 (E and N
+// below 2e-11 of the terms they correct): one v_mfma_f32_16x16x32_bf16 per
+// (output ell tile, matrix, real or imaginary part) covers the whole band
+// (n_ell <= 32).  Operand lane (lr, lg): row lr, ell 8 lg + j, j = 0..7; the
+// FP32 operand values rounded once more to BF16.  The correction's own
+// relative error, about 4e-3, is below 1e-13 of k.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ mfma_accf mfma_bf16(bf16x8 a, bf16x8 b, mfma_accf c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+template <int UB>
+struct GramFragmentsB {
+  bf16x8 e[UB], n[UB];
+};
+template <int UB>
+__device__ __forceinline__ void gram_load_fragments_bf16(const double* mats, const LineDesc& L,
+                                                         int lane, GramFragmentsB<UB>& f) {
+  const int LT = mfma_ell_tiles(L.n_ell);
+  const bf16x8* p =
+      reinterpret_cast<const bf16x8*>(mats + L.gfrag_off + gram_fragment_doubles(L.n_ell));
+#pragma unroll
+  for (int uo = 0; uo < UB; ++uo) {
+    // unconditional loads (clamped to the last tile): the correction then
+    // waits for these alone
+    const int u = min(uo, LT - 1);
+    const bf16x8 e = p[u * 64 + lane], n = p[(LT + u) * 64 + lane];
+    const bf16x8 z = {};
+    f.e[uo] = uo < LT ? e : z;
+    f.n[uo] = uo < LT ? n : z;
+  }
+}
+// Eight FP32 values to one BF16 operand.
+__device__ __forceinline__ bf16x8 to_bf16x8(const float (&v)[8]) {
+  bf16x8 r;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) r[j] = static_cast<__bf16>(v[j]);
+  return r;
+}
+
 // Output rows [g0, g0 + 16) of the tile (relative to t0) by one wave for
 // Gram bands (the general kernel, and tiles with more rows than the split
 // routine below holds).  Per field: the correction from the pre-projection
@@ -448,7 +487,7 @@ __device__ __forceinline__ void gram_correction_step(int LT, int K, int kb, int 
 // each accumulator position (row g0 + 4 lg + v, ell 16 uo + lr) the identity
 // part from the stencils there, the sum and the RK combination; every RK
 // load or store instruction covers 16 consecutive ell of four rows.
-template <int SCHEME, int STAGE, int UB, bool NT, bool IN>
+template <int SCHEME, int STAGE, int UB, bool NT, bool IN, bool BF>
 __device__ __forceinline__ void gram_output_group(const StageArgs& a, const TileCtx& c,
                                                   const float* Ef, int g0, int lane) {
   const LineDesc& L = *c.L;
@@ -469,7 +508,6 @@ __device__ __forceinline__ void gram_output_group(const StageArgs& a, const Tile
     return L.base + static_cast<long long>(out_row(v)) * us + F * nl + min(16 * uo + lr, nl - 1);
   };
   constexpr int NR = UB <= 2 ? UB : 1;  // ell tiles of RK inputs requested early
-
   auto field = [&](auto field_tag) {
     constexpr int F = decltype(field_tag)::value;
     const bool need_ys = F == 0 && c.inplace;
@@ -485,7 +523,63 @@ __device__ __forceinline__ void gram_output_group(const StageArgs& a, const Tile
 #pragma unroll
     for (int uo = 0; uo < UB; ++uo) cr[uo] = ci[uo] = mfma_accf{0.0f, 0.0f, 0.0f, 0.0f};
     // without dissipation the Psi pre-projection terms vanish
-    if (F != 2 || L.dissipate) {
+    if constexpr (BF) {
+      if (F != 2 || L.dissipate) {
+        // fragments per field (kept across all three they spill)
+        GramFragmentsB<UB> fb;
+        gram_load_fragments_bf16<UB>(a.mats, L, lane, fb);
+        // the operands X at (r_in, ell 8 lg + j), one stencil evaluation at a
+        // time, then E X; then Psi and N Psi (gram_split_correction's order)
+        // two passes of four operands (a loop, not unrolled: eight stencil
+        // evaluations inline spill)
+        bf16x8 bxr = {}, bxi = {};
+#pragma unroll 1
+        for (int h = 0; h < 2; ++h) {
+          float xr[4], xi[4];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const int l = 8 * lg + 4 * h + j;
+            const cdouble x =
+                l < nl ? pre_projection<SCHEME, F, false, IN>(c, r_in, l, zero) : zero;
+            xr[j] = static_cast<float>(x.re);
+            xi[j] = static_cast<float>(x.im);
+          }
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            if (h == 0) {
+              bxr[j] = static_cast<__bf16>(xr[j]);
+              bxi[j] = static_cast<__bf16>(xi[j]);
+            } else {
+              bxr[4 + j] = static_cast<__bf16>(xr[j]);
+              bxi[4 + j] = static_cast<__bf16>(xi[j]);
+            }
+          }
+        }
+#pragma unroll
+        for (int uo = 0; uo < UB; ++uo) {
+          if (uo >= LT) break;
+          cr[uo] = mfma_bf16(bxr, fb.e[uo], cr[uo]);
+          ci[uo] = mfma_bf16(bxi, fb.e[uo], ci[uo]);
+        }
+        if constexpr (F == 0) {
+          float sr[8], si[8];
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            const int l = 8 * lg + j;
+            const cdouble ps = u_in[2 * nl + (l < nl ? l : 0)];
+            sr[j] = l < nl ? static_cast<float>(ps.re) : 0.0f;
+            si[j] = l < nl ? static_cast<float>(ps.im) : 0.0f;
+          }
+          const bf16x8 bsr = to_bf16x8(sr), bsi = to_bf16x8(si);
+#pragma unroll
+          for (int uo = 0; uo < UB; ++uo) {
+            if (uo >= LT) break;
+            cr[uo] = mfma_bf16(bsr, fb.n[uo], cr[uo]);
+            ci[uo] = mfma_bf16(bsi, fb.n[uo], ci[uo]);
+          }
+        }
+      }
+    } else if (F != 2 || L.dissipate) {
       for (int kb = 0; kb < K; ++kb) {
         const int l = 4 * kb + lg;
         cdouble x = zero, ps = zero;
@@ -668,7 +762,90 @@ __device__ __forceinline__ void gram_split_correction(const TileCtx& c, const Gr
     }
 }
 
-template <int SCHEME, int STAGE, int UB, bool IN>
+// The same corrections from the BF16 fragments (gram_output_group's BF16
+// form, term for term).
+template <int UB>
+__device__ __forceinline__ void gram_split_correction(const TileCtx& c,
+                                                      const GramFragmentsB<UB>& f, int g0,
+                                                      int lane) {
+  const LineDesc& L = *c.L;
+  const TileDesc& tile = *c.tile;
+  const int nl = L.n_ell, us = 3 * nl;
+  const int LT = mfma_ell_tiles(nl);
+  const int lr = lane & 15, lg = lane >> 4;
+  const int orows = tile.t1 - tile.t0;
+  const int r_in = tile.t0 + min(g0 + lr, orows - 1);
+  cdouble* urow = const_cast<cdouble*>(c.U) + (r_in - tile.lo) * us;
+  const float* x = reinterpret_cast<const float*>(urow);  // X_P | X_Q as complex floats
+  // one operand at a time (all six gathered at once spill): X_P with E,
+  // Psi with N, X_Q with E, each accumulator taking its terms in
+  // gram_output_group's order
+  auto gather = [&](int which) {
+    float v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int l = 8 * lg + j;
+      const bool ok = l < nl;
+      const int lc = ok ? l : 0;
+      float t;
+      if (which < 2) t = x[2 * lc + which];
+      else if (which < 4) t = static_cast<float>(which == 2 ? urow[2 * nl + lc].re
+                                                            : urow[2 * nl + lc].im);
+      else t = x[2 * nl + 2 * lc + (which - 4)];
+      v[j] = ok ? t : 0.0f;
+    }
+    const bf16x8 r = to_bf16x8(v);
+    __builtin_amdgcn_sched_barrier(0);
+    return r;
+  };
+  mfma_accf pr[UB], pi[UB], qr[UB], qi[UB];
+#pragma unroll
+  for (int uo = 0; uo < UB; ++uo) pr[uo] = pi[uo] = qr[uo] = qi[uo] = mfma_accf{0.0f, 0.0f, 0.0f, 0.0f};
+  {
+    const bf16x8 xr = gather(0), xi = gather(1);
+#pragma unroll
+    for (int uo = 0; uo < UB; ++uo) {
+      if (uo >= LT) break;
+      pr[uo] = mfma_bf16(xr, f.e[uo], pr[uo]);
+      pi[uo] = mfma_bf16(xi, f.e[uo], pi[uo]);
+    }
+  }
+  {
+    const bf16x8 sr = gather(2), si = gather(3);
+#pragma unroll
+    for (int uo = 0; uo < UB; ++uo) {
+      if (uo >= LT) break;
+      pr[uo] = mfma_bf16(sr, f.n[uo], pr[uo]);
+      pi[uo] = mfma_bf16(si, f.n[uo], pi[uo]);
+    }
+  }
+  {
+    const bf16x8 xr = gather(4), xi = gather(5);
+#pragma unroll
+    for (int uo = 0; uo < UB; ++uo) {
+      if (uo >= LT) break;
+      qr[uo] = mfma_bf16(xr, f.e[uo], qr[uo]);
+      qi[uo] = mfma_bf16(xi, f.e[uo], qi[uo]);
+    }
+  }
+  // element v: row g0 + 4 lg + v, ell 16 uo + lr (as the FP32 form)
+#pragma unroll
+  for (int uo = 0; uo < UB; ++uo)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = g0 + 4 * lg + v, l = 16 * uo + lr;
+      if (uo < LT && row < orows && l < nl) {
+        float* w = reinterpret_cast<float*>(const_cast<cdouble*>(c.U) +
+                                            (tile.t0 + row - tile.lo) * us);
+        w[2 * l] = pr[uo][v];
+        w[2 * l + 1] = pi[uo][v];
+        w[2 * nl + 2 * l] = qr[uo][v];
+        w[2 * nl + 2 * l + 1] = qi[uo][v];
+      }
+    }
+}
+
+template <int SCHEME, int STAGE, int UB, bool IN, bool BF>
 __device__ __forceinline__ void gram_split_output(const StageArgs& a, const TileCtx& c,
                                                   const float* Ef, int tid,
                                                   RkIn (&psi_in)[kGramPoints]) {
@@ -701,8 +878,11 @@ __device__ __forceinline__ void gram_split_output(const StageArgs& a, const Tile
     }
   }
   // E and N fragments for the correction, in flight across the barriers
-  GramFragments<UB> frag;
-  gram_load_fragments<UB>(Ef, nl, tid & 63, frag);
+  std::conditional_t<BF, GramFragmentsB<UB>, GramFragments<UB>> frag;
+  if constexpr (BF)
+    gram_load_fragments_bf16<UB>(a.mats, L, tid & 63, frag);
+  else
+    gram_load_fragments<UB>(Ef, nl, tid & 63, frag);
   __syncthreads();  // every psi_t stencil has been read
   RkIn inP[NP], inQ[NP];
 #pragma unroll
@@ -838,14 +1018,30 @@ __device__ __forceinline__ void output_phase(const StageArgs& a, const LineDesc&
     return;
   }
   const float* Ef = reinterpret_cast<const float*>(a.mats + L.gfrag_off);
+  // The BF16 correction serves the Kerr-Gram variant's bands of at most two
+  // ell tiles; the general kernel keeps the FP32 form for every line (its
+  // routines with BF16 inlined spill).
+  constexpr bool kBf = UB <= 2 && VARIANT == kStageKerrGram;
+  const bool bf = kBf && L.gram_bf16;
   if constexpr (split_gram<VARIANT, UB>()) {
     if (gram_split_fits(L, tile)) {
-      gram_split_output<SCHEME, STAGE, UB, IN>(a, c, Ef, tid, psi_in);
+      if (bf)
+        gram_split_output<SCHEME, STAGE, UB, IN, true>(a, c, Ef, tid, psi_in);
+      else
+        gram_split_output<SCHEME, STAGE, UB, IN, false>(a, c, Ef, tid, psi_in);
+      return;
+    }
+  }
+  constexpr bool NT = VARIANT == kStageKerrGram;
+  if constexpr (kBf) {
+    if (bf) {
+      for (int g0 = 16 * wave; g0 < orows; g0 += 16 * (kStageThreads / 64))
+        gram_output_group<SCHEME, STAGE, UB, NT, IN, true>(a, c, Ef, g0, lane);
       return;
     }
   }
   for (int g0 = 16 * wave; g0 < orows; g0 += 16 * (kStageThreads / 64))
-    gram_output_group<SCHEME, STAGE, UB, VARIANT == kStageKerrGram, IN>(a, c, Ef, g0, lane);
+    gram_output_group<SCHEME, STAGE, UB, NT, IN, false>(a, c, Ef, g0, lane);
 }
 
 // One radial tile of one line; UB = ell tiles of 16 the MFMA routines hold
