@@ -178,13 +178,17 @@ __device__ __forceinline__ void kerr_psi_group(int nl, int nt, double spin, cons
       nim[v] = tok ? num_im * rct : 0.0;
     }
 #pragma unroll
-    for (int v = 0; v < 4; ++v)
+    for (int v = 0; v < 4; ++v) {
+      // theta block 16 t + 4 v .. + 3: past the last node it is all
+      // padding (zero operands), so its products are skipped
+      if (16 * t + 4 * v >= nt) break;
 #pragma unroll
       for (int u = 0; u < UB; ++u)
         if (u < LT) {
           ar[u] = mfma_f64(nre[v], a_cur[v][u], ar[u]);
           ai[u] = mfma_f64(nim[v], a_cur[v][u], ai[u]);
         }
+    }
 #pragma unroll
     for (int q = 0; q < KB; ++q) s_cur[q] = s_nxt[q];
 #pragma unroll
