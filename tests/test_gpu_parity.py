@@ -338,6 +338,33 @@ def test_explicit_steps_are_timed_in_graph_mode():
     assert steps == 3 and ms > 0.0 and replays == 2
 
 
+def test_multi_step_graph_replays_match_eager_steps():
+    """teuk_advance replays a graph of eight RK4 steps (each execution
+    group's stage chain running through all eight between one fork and one
+    join) while eight or more steps remain, then one-step graphs: 19 steps
+    are two eight-step and three one-step replays.  The state equals the
+    eager steps' bit for bit, and step_timing counts every step."""
+    cases = _two_group_cases()
+    texts = [W.config_text(**dict(c)) for c in cases]
+    with T.Plan(texts) as plan:
+        plan.initialize()
+        plan.set_graph(True)
+        plan.set_kernel_timing(True)
+        plan.advance(19)
+        ms, steps = plan.step_timing()
+        plan.set_kernel_timing(False)
+        replays = plan.graph_replays()
+        t, done = plan.progress()
+        graph_state = [plan.export_state(i) for i in range(len(cases))]
+    assert steps == 19 and replays == 19 and ms > 0.0 and done == 19
+    with T.Plan(texts) as plan:
+        plan.initialize()
+        plan.advance(19)
+        assert plan.progress() == (t, done)
+        for i, g in enumerate(graph_state):
+            assert np.array_equal(plan.export_state(i), g)
+
+
 def test_graph_replays_are_timed_and_match_eager_steps():
     """With timing on, each graph replay is bracketed by HIP events on the
     plan's stream (no per-launch events inside the graph), and the replayed

@@ -129,6 +129,7 @@ Plan::~Plan() {
   if (stream_) hipStreamSynchronize(stream_);
   if (own_stream_ && own_stream_ != stream_) hipStreamSynchronize(own_stream_);
   if (graph_exec_) hipGraphExecDestroy(graph_exec_);
+  if (graph_multi_) hipGraphExecDestroy(graph_multi_);
   for (hipEvent_t e : events_) hipEventDestroy(e);
   for (hipEvent_t e : joins_) hipEventDestroy(e);
   if (fork_) hipEventDestroy(fork_);
@@ -844,13 +845,15 @@ hipEvent_t Plan::timing_event() {
   return events_[k];
 }
 
-// One RK4 step of every instance.  Execution groups touch disjoint lines, so
-// each runs its four stage launches on a stream of its own (forked from and
-// joined back into the plan's stream): their tails and their different
-// bottlenecks (MFMA-heavy wide bands, memory-bound narrow ones) overlap.
-// With `capturing` the launches go into a hipGraph and are not timed;
-// otherwise, with kernel timing on, HIP events bracket every launch.
-void Plan::launch_step(double dt_override, bool capturing) {
+// `steps` RK4 steps of every instance.  Execution groups touch disjoint
+// lines, so each runs its stage launches on a stream of its own (forked from
+// and joined back into the plan's stream): their tails and their different
+// bottlenecks (MFMA-heavy wide bands, memory-bound narrow ones) overlap, and
+// over several steps a group that finishes its step early goes on with the
+// next instead of waiting at a join for the other.  With `capturing` the
+// launches go into a hipGraph and are not timed; otherwise, with kernel
+// timing on, HIP events bracket every launch.
+void Plan::launch_step(double dt_override, bool capturing, int steps) {
   const bool timed = timing_ && !capturing;
   const int ng = static_cast<int>(groups_.size());
   if (side_.size() + 1 < groups_.size()) throw std::logic_error("group streams missing");
@@ -871,7 +874,8 @@ void Plan::launch_step(double dt_override, bool capturing) {
       a.dt_override = dt_override;
       const cdouble* in_of[5] = {nullptr, d_y_, d_ya_, d_yb_, d_ya_};
       cdouble* out_of[5] = {nullptr, d_ya_, d_yb_, d_ya_, d_y_};
-      for (int s = 1; s <= 4; ++s) {
+      for (int k = 0; k < 4 * steps; ++k) {
+        const int s = 1 + k % 4;
         a.yin = in_of[s];
         a.yout = out_of[s];
         hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -901,11 +905,11 @@ hipEvent_t Plan::step_begin_event() {
   return e;
 }
 
-void Plan::step_end_event(hipEvent_t begin) {
+void Plan::step_end_event(hipEvent_t begin, int steps) {
   if (!begin) return;
   hipEvent_t e = timing_event();
   check(hipEventRecord(e, stream_), "event");
-  step_events_.push_back({begin, e});
+  step_events_.push_back({begin, e, steps});
 }
 
 void Plan::capture_graph() {
@@ -928,6 +932,22 @@ void Plan::capture_graph() {
   const hipError_t e = hipGraphInstantiate(&graph_exec_, graph, nullptr, nullptr, 0);
   hipGraphDestroy(graph);
   check(e, "instantiate graph");
+  if (graph_multi_) {
+    hipGraphExecDestroy(graph_multi_);
+    graph_multi_ = nullptr;
+  }
+  if (so_mode_) return;
+  check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture");
+  try {
+    launch_step(0.0, true, kGraphSteps);
+  } catch (...) {
+    (void)hipStreamEndCapture(stream_, &graph);
+    throw;
+  }
+  check(hipStreamEndCapture(stream_, &graph), "end capture");
+  const hipError_t e2 = hipGraphInstantiate(&graph_multi_, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  check(e2, "instantiate graph");
 }
 
 void Plan::record_time_step(double dt_override) {
@@ -998,14 +1018,18 @@ void Plan::advance(int steps) {
   }
   if (use_graph_) {
     if (!graph_exec_) capture_graph();  // made by set_graph; rebuilt after a failed capture
-    for (int s = 0; s < steps; ++s) {
-      // with timing on, HIP events around each replay (recorded outside the
-      // graph) give the GPU time of the step
+    for (int s = 0; s < steps;) {
+      // kGraphSteps steps per replay while that many remain, then one; with
+      // timing on, HIP events around each replay (recorded outside the
+      // graph) give the GPU time of its steps
+      const bool multi = graph_multi_ && steps - s >= kGraphSteps;
+      const int n = multi ? kGraphSteps : 1;
       hipEvent_t e0 = step_begin_event();
-      check(hipGraphLaunch(graph_exec_, stream_), "graph launch");
-      step_end_event(e0);
-      ++graph_replays_;
-      record_time_step(0.0);
+      check(hipGraphLaunch(multi ? graph_multi_ : graph_exec_, stream_), "graph launch");
+      step_end_event(e0, n);
+      graph_replays_ += n;
+      for (int k = 0; k < n; ++k) record_time_step(0.0);
+      s += n;
     }
     return;
   }
@@ -1142,6 +1166,10 @@ void Plan::set_graph(bool enable) {
     hipGraphExecDestroy(graph_exec_);
     graph_exec_ = nullptr;
   }
+  if (graph_multi_) {
+    hipGraphExecDestroy(graph_multi_);
+    graph_multi_ = nullptr;
+  }
   if (use_graph_) capture_graph();
 }
 
@@ -1178,9 +1206,13 @@ void Plan::kernel_timing(double* total_ms, long long* launches) {
 void Plan::step_timing(double* total_ms, long long* steps) {
   check(hipStreamSynchronize(stream_), "timing sync");
   double ms = 0.0;
-  for (const auto& se : step_events_) ms += elapsed_ms(se.first, se.second);
+  long long n = 0;
+  for (const auto& se : step_events_) {
+    ms += elapsed_ms(se.first, se.second);
+    n += se.steps;
+  }
   *total_ms = ms;
-  *steps = static_cast<long long>(step_events_.size());
+  *steps = n;
 }
 
 void Plan::group_info(int g, int* scheme, int* ub, int* tiles, int* lds_bytes,

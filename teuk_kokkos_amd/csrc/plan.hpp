@@ -172,10 +172,10 @@ class Plan {
   void build_lines();
   void build_tiles();
   void upload_tables();
-  void launch_step(double dt_override, bool capturing);
+  void launch_step(double dt_override, bool capturing, int steps = 1);
   void capture_graph();
   hipEvent_t step_begin_event();
-  void step_end_event(hipEvent_t begin);
+  void step_end_event(hipEvent_t begin, int steps = 1);
   void allocate_staging();
   void allocate_diagnostics();
   void record_time_step(double dt_override);
@@ -230,6 +230,10 @@ class Plan {
 
   bool use_graph_ = false;
   hipGraphExec_t graph_exec_ = nullptr;
+  // kGraphSteps RK4 steps in one graph (linear plans): the groups' stage
+  // chains run through all of them between one fork and one join
+  static constexpr int kGraphSteps = 8;
+  hipGraphExec_t graph_multi_ = nullptr;
   long long graph_replays_ = 0;
   bool timing_ = false;
   hipEvent_t timing_event();
@@ -240,7 +244,11 @@ class Plan {
     int group;
   };
   std::vector<LaunchEvents> launch_events_;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> step_events_;
+  struct StepEvents {
+    hipEvent_t first, second;
+    int steps = 1;  // RK4 steps between the two events
+  };
+  std::vector<StepEvents> step_events_;
   std::vector<hipStream_t> side_;  // streams of execution groups 1..n-1
   std::vector<hipEvent_t> joins_;
   hipEvent_t fork_ = nullptr;
