@@ -231,7 +231,10 @@ int teuk_source_pairs(const teuk_plan* plan, int instance, double* out, int capa
  * the reference's SpatialPipelineTiming, spatial_pipeline.hpp:27-38)
  * ------------------------------------------------------------------- */
 /* Graph mode (default off): one RK4 step of every instance is captured as a
- * hipGraph now, and teuk_advance replays it.  Second-order plans replay it
+ * hipGraph now, and teuk_advance replays it; linear plans also capture eight
+ * steps as one graph (each execution group's stage chain joined once per
+ * eight steps) and replay that while eight or more steps remain, with the
+ * same results bit for bit.  Second-order plans replay the one-step graph
  * for every step in which no instance splits at its source start and no
  * latch decision needs a constraint evaluation (the Unrestricted and
  * Disabled policies decide on the host); other steps launch directly, with
@@ -253,8 +256,10 @@ int teuk_allocation_count(unsigned long long* count);
 int teuk_plan_set_lds_budget(teuk_plan* plan, int bytes);
 int teuk_plan_set_tile_rows(teuk_plan* plan, int rows);
 /* Stage kernels specialised per line variant (default 1); 0 runs every line
- * through the general kernel, which gives the same results bit for bit.
- * Re-tiles the plan. */
+ * through the general kernel, which gives the same results bit for bit, save
+ * the Gram correction of bands whose E = A S - I is below 2e-11 (ell 24):
+ * BF16 MFMA in the specialised kernel, FP32 in the general one (they agree
+ * to 1e-13).  Re-tiles the plan. */
 int teuk_plan_set_specialized(teuk_plan* plan, int enable);
 int teuk_plan_tiles(const teuk_plan* plan, int* tiles, int* max_lds_bytes);
 /* HIP-event timing of every stage-kernel launch on the plan's stream
