@@ -335,10 +335,11 @@ __device__ __forceinline__ cdouble pre_projection(const TileCtx& c, int r, int l
 // tile starts (plain_prefetch, before the staging copy, so they arrive while
 // the tile is staged and psi_t is formed) and each consumed slot is refilled
 // with the row kPlainDepth steps ahead.
-// Rows in flight per thread: four for narrow bands (measured best of 2, 4
-// and 6); wide bands (UB > 1) keep two, within their register budget.
+// Rows in flight per thread: three for narrow bands (measured best of 2 to
+// 6 with the present staging order; four was best before it); wide bands
+// (UB > 1) keep two, within their register budget.
 template <int UB>
-inline constexpr int kPlainDepth = UB == 1 ? 4 : 2;
+inline constexpr int kPlainDepth = UB == 1 ? 3 : 2;
 template <int STAGE>
 __device__ __forceinline__ void plain_fetch(const StageArgs& a, const LineDesc& L, bool inplace,
                                             int r, int lane_l, RkIn (&v)[3]) {
