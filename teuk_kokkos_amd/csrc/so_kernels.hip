@@ -492,7 +492,8 @@ __global__ __launch_bounds__(kSoThreads) void so_drad_kernel(SoArgs a, int which
 // leave their results in these rows.
 template <int KIND, int NT>
 // The 256-thread form at three waves per SIMD (at most 168 registers, a few
-// spilled): measured 5 % faster per second-order step than at two.
+// spilled): measured 5 % faster per second-order step than at two (at four,
+// with about 100 spilled, 7 % slower).
 __global__ __launch_bounds__(NT, NT == 256 ? 3 : 1) void so_recon_kernel(SoArgs a) {
   constexpr int CH = 8;  // operator loads in flight per row_op chunk
   extern __shared__ __align__(16) double smem[];
@@ -977,7 +978,10 @@ __global__ __launch_bounds__(kSoThreads) void so_source_kernel(SoArgs a, int max
 // block: the projection of target ts and coordinator (D pair or T pair)
 // takes rows (radial row, component of the pair, real/imag), eth'_3 T rows
 // (radial row, real/imag); the blocks go round the four waves.
-__global__ __launch_bounds__(kSoThreads) void so_source_projection_kernel(SoArgs a) {
+// Six waves per SIMD (at most 80 registers, a few spilled; the LDS of a
+// workgroup allows six per CU): measured 2 % faster per second-order step than
+// the three its unconstrained 96 registers gave.
+__global__ __launch_bounds__(kSoThreads, 6) void so_source_projection_kernel(SoArgs a) {
   static_assert(kProjRows == 4, "a projection block is 4 rows x 2 components x 2 parts");
   extern __shared__ __align__(16) double smem[];
   const SoInstance& I = a.inst[a.first_inst + blockIdx.y];
