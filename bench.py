@@ -140,6 +140,7 @@ def measure_traffic(workload_name: str, second_order: bool, algorithmic_stage_by
             cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run", "--",
                    sys.executable, os.path.abspath(__file__), "--workload", workload_name,
                    "--steps", "1", "--warmup", "0", "--no-cpu", "--no-traffic", "--eager",
+                   "--no-compare",
                    "--calibrate-copy-mib", str(copy_bytes >> 20)]
             try:
                 # the profiler may exit non-zero after a complete run; the
@@ -260,6 +261,11 @@ def main() -> int:
                     help="launch every stage kernel directly instead of replaying the step as "
                          "a captured hipGraph (the default for first-order plans)")
     ap.add_argument("--graph", action="store_true", help=argparse.SUPPRESS)  # the default
+    ap.add_argument("--fp64-gram", action="store_true",
+                    help="time the FP32 Gram correction (as accurate as the FP64 product) "
+                         "instead of the mixed-precision BF16 one on the ell-24 bands")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the timed run with the FP32 Gram correction after the headline")
     ap.add_argument("--lds-budget", type=int, default=0)
     ap.add_argument("--tile-rows", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -331,6 +337,11 @@ def main() -> int:
     # a constraint evaluation for the latch)
     use_graph = not args.eager
     plan.set_graph(use_graph)
+    # mixed precision (first-order plans): the ell-24 bands' Gram correction
+    # on BF16 MFMA; the FP32 form is timed after the headline run for the
+    # mixed-vs-FP64 ratio
+    mixed = plan.modal_points > 0 and not args.fp64_gram
+    plan.set_mixed_gram(mixed)
     plan.advance(args.warmup)
     plan.sync()
 
@@ -350,6 +361,18 @@ def main() -> int:
     grid = plan.grid_points
     modal = plan.modal_points
     value = grid * args.steps / seconds
+    fp64_gram = None
+    if mixed and not args.no_compare:
+        # the same steps with the FP32 Gram correction (graph replays as above)
+        plan.set_mixed_gram(False)
+        plan.advance(max(1, args.warmup))
+        plan.sync()
+        t1 = time.perf_counter()
+        plan.advance(args.steps)
+        plan.sync()
+        s64 = time.perf_counter() - t1
+        fp64_gram = {"ms_per_step": s64 / args.steps * 1e3, "value": grid * args.steps / s64,
+                     "mixed_over_fp64": s64 / seconds}
     roofline = None
     second_order = modal == 0
     if timed_steps and second_order:
@@ -391,7 +414,9 @@ def main() -> int:
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": desc, "instances": len(cases), "grid_points_per_step": grid,
-                   "modal_points_per_step": modal, "precision": "fp64",
+                   "modal_points_per_step": modal,
+                   "precision": ("fp64 state; ell-24 Gram correction on bf16 MFMA (mixed)"
+                                 if mixed else "fp64"),
                    "graph": bool(use_graph),
                    "tuning": {"source": tuning_source,
                               "shapes": {f"{g['scheme']}/{g['ell_tiles']}/{g['variant']}":
@@ -399,6 +424,7 @@ def main() -> int:
                                                             "tiles", "lds_bytes")}
                                          for g in groups}}},
         "modal_updates_per_s": modal * args.steps / seconds,
+        "fp64_gram": fp64_gram,
         "roofline": roofline,
         "cpu_baseline": None,
     }

@@ -256,11 +256,17 @@ int teuk_allocation_count(unsigned long long* count);
 int teuk_plan_set_lds_budget(teuk_plan* plan, int bytes);
 int teuk_plan_set_tile_rows(teuk_plan* plan, int rows);
 /* Stage kernels specialised per line variant (default 1); 0 runs every line
- * through the general kernel, which gives the same results bit for bit, save
- * the Gram correction of bands whose E = A S - I is below 2e-11 (ell 24):
- * BF16 MFMA in the specialised kernel, FP32 in the general one (they agree
- * to 1e-13).  Re-tiles the plan. */
+ * through the general kernel, which gives the same results bit for bit (with
+ * teuk_plan_set_mixed_gram on, the general kernel keeps the FP32 Gram
+ * correction: the two then agree to 1e-13).  Re-tiles the plan. */
 int teuk_plan_set_specialized(teuk_plan* plan, int enable);
+/* Mixed precision (default 0, no reference counterpart): 1 takes the Gram
+ * correction E X + N Psi of bands whose E = A S - I and N are below 2e-11
+ * (ell 24) on BF16 MFMA in the specialised Kerr-Gram kernel, where 0 keeps
+ * it on FP32 MFMA, as accurate as the FP64 product.  The correction is
+ * 6e-12 of the RHS there; BF16 holds it to 4e-3 of itself.  Changes no
+ * tiling and allocates nothing; takes effect from the next launch or replay. */
+int teuk_plan_set_mixed_gram(teuk_plan* plan, int enable);
 int teuk_plan_tiles(const teuk_plan* plan, int* tiles, int* max_lds_bytes);
 /* HIP-event timing of every stage-kernel launch on the plan's stream
  * (eager mode only); reset by enabling. */

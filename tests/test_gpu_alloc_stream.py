@@ -56,6 +56,10 @@ def _exercise(plan, graph, tmp_path):
     plan.write_checkpoint(str(tmp_path / f"ck_{graph}"))
     plan.load_checkpoint(str(tmp_path / f"ck_{graph}"))
     plan.advance(2)
+    if plan.modal_points > 0:
+        plan.set_mixed_gram(True)
+        plan.advance(9)
+        plan.set_mixed_gram(False)
     plan.sync()
     after = T.allocation_count()
     d_in.free()

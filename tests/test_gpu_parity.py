@@ -147,13 +147,16 @@ def test_tiling_does_not_change_results():
         assert H.rel_err(o, out[0]) <= 1e-14
 
 
+@pytest.mark.parametrize("mixed", [False, True])
 @pytest.mark.parametrize("ell", [24, 32])
-def test_gram_split_and_group_outputs_agree_and_match_oracle(ell):
+def test_gram_split_and_group_outputs_agree_and_match_oracle(ell, mixed):
     """Gram bands apply G = A S as I + E with the correction E X (and N Psi)
     on FP32 MFMA.  Tiles of at most five rows per thread take the
     one-point-per-thread routine, larger ones the accumulator-position routine;
     both must give the same bits, and at ell 32, where |E| ~ 2e-9, a missing
-    or misplaced correction would exceed the FP64 bar against the oracle."""
+    or misplaced correction would exceed the FP64 bar against the oracle.
+    With mixed precision on, the ell-24 band's correction runs on BF16 in
+    both routines (ell 32 keeps FP32), again to the same bits."""
     params = H.small_case(spin=0.99, compactification_length=1.0, nr=61,
                           radial_discretization="d10-5", ellmax_first=ell, ellmax_second=ell,
                           seeds=[(2, 2, 1e-3), (6, 2, 2e-4 - 3e-4j)])
@@ -162,6 +165,7 @@ def test_gram_split_and_group_outputs_agree_and_match_oracle(ell):
     # thread pass, five passes); one 61-row tile does not
     for rows in (16, 61):
         with _plan(params) as plan:
+            plan.set_mixed_gram(mixed)
             plan.set_tile_rows(rows)
             assert {g["variant"] for g in plan.groups()} == {2}
             plan.initialize()
@@ -171,7 +175,8 @@ def test_gram_split_and_group_outputs_agree_and_match_oracle(ell):
     ref = H.oracle_initialized(params)
     ref.advance(4, params["final_time"] / params["steps"])
     err = H.rel_err(out[16], ref.get_state())
-    print(f"[gpu] Gram ell {ell} split/group outputs, 4 steps rel err {err:.3e}", flush=True)
+    print(f"[gpu] Gram ell {ell} (mixed {mixed}) split/group outputs, 4 steps rel err {err:.3e}",
+          flush=True)
     assert err <= TOL
 
 
@@ -179,10 +184,10 @@ def test_gram_split_and_group_outputs_agree_and_match_oracle(ell):
 def test_specialised_stage_kernels_match_general_kernel_and_oracle(scheme, ell):
     """Kerr lines without dissipation in free-damped mode run stage kernels
     with the line flags fixed at compile time (G = I for ell <= 16, the Gram
-    projection at ell 24).  At ell <= 16 they must reproduce the general
-    kernel bit for bit (the same arithmetic in the same order); at ell 24
-    the specialised kernel's BF16 Gram correction agrees with the general
-    kernel's FP32 one to 1e-13.  Both meet the oracle's FP64 bar."""
+    projection at ell 24).  They must reproduce the general kernel bit for
+    bit (the same arithmetic in the same order) and the oracle to the FP64
+    bar; at ell 24 the mixed-precision Gram correction (BF16) agrees with
+    the FP32 one to 1e-13."""
     params = H.small_case(spin=0.8, nr=229, radial_discretization=scheme, ellmax_first=ell,
                           ellmax_second=ell, seeds=[(2, 2, 1e-3), (5, 2, 3e-4j)])
     out = {}
@@ -194,15 +199,19 @@ def test_specialised_stage_kernels_match_general_kernel_and_oracle(scheme, ell):
             plan.initialize()
             plan.advance(3)
             out[special] = plan.export_state()
-    if ell <= 16:
-        assert np.array_equal(out[True], out[False])
-    else:
-        # the Kerr-Gram kernel takes the ell-24 band's Gram correction on
-        # BF16 operands, the general kernel on FP32 ones: the correction is
-        # 6e-12 of k and BF16 keeps it to 4e-3, so the two agree to 1e-13
-        diff = H.rel_err(out[True], out[False])
-        print(f"[gpu] specialised (BF16 correction) vs general (FP32) {diff:.3e}", flush=True)
-        assert diff <= 1e-13
+    assert np.array_equal(out[True], out[False])
+    if ell > 16:
+        # mixed precision: the Kerr-Gram kernel takes the ell-24 band's Gram
+        # correction on BF16 operands (the general kernel keeps FP32): the
+        # correction is 6e-12 of k and BF16 holds it to 4e-3, so 1e-13
+        with _plan(params) as plan:
+            plan.set_mixed_gram(True)
+            plan.initialize()
+            plan.advance(3)
+            mixed = plan.export_state()
+        diff = H.rel_err(mixed, out[False])
+        print(f"[gpu] mixed (BF16 correction) vs FP32 correction {diff:.3e}", flush=True)
+        assert 0.0 < diff <= 1e-13
     ref = H.oracle_initialized(params)
     ref.advance(3, params["final_time"] / params["steps"])
     err = H.rel_err(out[True], ref.get_state())

@@ -90,6 +90,7 @@ SIGNATURES = {
     "teuk_plan_group_info": (_I, [_P, _I, _IP, _IP, _IP, _IP, _DP]),
     "teuk_plan_group_variant": (_I, [_P, _I, _IP]),
     "teuk_plan_set_specialized": (_I, [_P, _I]),
+    "teuk_plan_set_mixed_gram": (_I, [_P, _I]),
     "teuk_plan_set_group_shape": (_I, [_P, _I, _I, _I, _I, _I]),
     "teuk_plan_group_shape": (_I, [_P, _I, _IP, _IP]),
     "teuk_plan_set_stream": (_I, [_P, _P]),
@@ -430,6 +431,11 @@ class Plan:
         """Stage kernels specialised per line variant (the default), or every
         line through the general kernel."""
         self._check(lib().teuk_plan_set_specialized(self._h, int(enable)))
+
+    def set_mixed_gram(self, enable: bool):
+        """The Gram correction of the ell-24 bands on BF16 MFMA (mixed
+        precision) instead of FP32 (as accurate as the FP64 product)."""
+        self._check(lib().teuk_plan_set_mixed_gram(self._h, int(enable)))
 
     def tiles(self):
         n = ctypes.c_int()

@@ -397,6 +397,11 @@ int teuk_plan_set_specialized(teuk_plan* plan, int enable) {
   return guarded(plan ? plan->error : dummy, [&] { plan_of(plan).set_specialized(enable != 0); });
 }
 
+int teuk_plan_set_mixed_gram(teuk_plan* plan, int enable) {
+  std::string dummy;
+  return guarded(plan ? plan->error : dummy, [&] { plan_of(plan).set_mixed_gram(enable != 0); });
+}
+
 int teuk_plan_tiles(const teuk_plan* plan, int* tiles, int* max_lds_bytes) {
   auto* p = const_cast<teuk_plan*>(plan);
   std::string dummy;

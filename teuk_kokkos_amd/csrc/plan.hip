@@ -298,7 +298,8 @@ void Plan::build_lines() {
       L.kerr = p.spin != 0.0 ? 1 : 0;
       L.dissipate = p.dissipation > 0.0 ? 1 : 0;
       L.gram = band_gram_needed(-2, m, p.ell_max_first, in.T) ? 1 : 0;
-      L.gram_bf16 = L.gram && bf16_ok ? 1 : 0;
+      L.gram_bf16 = L.gram && bf16_ok && mixed_gram_ ? 1 : 0;
+      line_bf16_ok_.push_back(L.gram && bf16_ok ? 1 : 0);
       L.mat_off = off;
       L.frag_off = frag_off;
       L.gfrag_off = frag_off + mfma_fragment_doubles(in.T, n_ell);
@@ -1248,6 +1249,18 @@ void Plan::set_tile_rows(int rows) {
   build_tiles();
   upload_tables();
   set_graph(use_graph_);
+}
+
+// Mixed-precision Gram correction on or off: the line flags change in place
+// (no re-tiling, no allocation); graphs read them at every replay.
+void Plan::set_mixed_gram(bool enable) {
+  if (so_mode_) return;
+  mixed_gram_ = enable;
+  for (size_t i = 0; i < lines_.size(); ++i)
+    lines_[i].gram_bf16 = enable && line_bf16_ok_[i] ? 1 : 0;
+  check(hipStreamSynchronize(stream_), "mixed Gram sync");
+  check(hipMemcpy(d_lines_, lines_.data(), lines_.size() * sizeof(LineDesc), hipMemcpyHostToDevice),
+        "upload lines");
 }
 
 void Plan::set_specialized(bool enable) {

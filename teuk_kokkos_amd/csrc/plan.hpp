@@ -113,6 +113,7 @@ class Plan {
   // Compile-time line variants for the stage kernels (default on); off runs
   // every line through the general kernel.  Re-tiles.
   void set_specialized(bool enable);
+  void set_mixed_gram(bool enable);
   int group_variant(int g) const;
   // Tile shape of one execution group: LDS budget and tile rows (0 = the
   // plan's setting).
@@ -199,6 +200,10 @@ class Plan {
   int lds_budget_ = 80 * 1024;  // two workgroups per CU (160 KiB LDS)
   int tile_rows_ = 0;
   bool specialized_ = true;
+  // The Gram correction of eligible bands on BF16 MFMA (LineDesc::gram_bf16);
+  // line_bf16_ok_: the band's E and N qualify
+  bool mixed_gram_ = false;
+  std::vector<char> line_bf16_ok_;
 
   hipStream_t stream_ = nullptr;      // where work is enqueued (the caller's, or own_stream_)
   hipStream_t own_stream_ = nullptr;
