@@ -170,10 +170,13 @@ __device__ __forceinline__ void kerr_psi_group(int nl, int nt, double spin, cons
       const double pr = spr[v], pi = spi[v];
       const double num_re = sxr[v] - (gre * pr - gi * pi);
       const double num_im = sxi[v] - (gre * pi + gi * pr);
-      // one correctly rounded reciprocal and two products instead of two
-      // divisions (1.3 % of the sweep's stage time; the result differs from
-      // the quotient by about an ulp)
-      const double rct = 1.0 / ct;
+      // one reciprocal and two products instead of two divisions (1.3 % of
+      // the sweep's stage time), the reciprocal from the hardware estimate
+      // and two Newton steps (C_T is positive and normal on the grid; the
+      // result differs from the quotient by about an ulp)
+      double rct = __builtin_amdgcn_rcp(ct);
+      rct = __builtin_fma(__builtin_fma(-ct, rct, 1.0), rct, rct);
+      rct = __builtin_fma(__builtin_fma(-ct, rct, 1.0), rct, rct);
       nre[v] = tok ? num_re * rct : 0.0;
       nim[v] = tok ? num_im * rct : 0.0;
     }
