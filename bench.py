@@ -362,7 +362,9 @@ def main() -> int:
     modal = plan.modal_points
     value = grid * args.steps / seconds
     fp64_gram = None
-    if mixed and not args.no_compare:
+    # (not under a profiler: its trace summary takes the last steps' dispatches
+    # as the headline run's)
+    if mixed and not args.no_compare and not _under_profiler():
         # the same steps with the FP32 Gram correction (graph replays as above)
         plan.set_mixed_gram(False)
         plan.advance(max(1, args.warmup))
