@@ -221,3 +221,40 @@ def test_sweep_512_batch_2000_steps_matches_oracle():
         assert np.isfinite(got[k]).all()
         # measured 4.0e-14, 5.1e-14, 1.8e-13 and 8.8e-14
         assert err <= 1e-12
+
+
+def test_mixed_gram_correction_stays_within_the_mixed_bar_over_2000_steps():
+    """Mixed precision (teuk_plan_set_mixed_gram): the nr 257 ell-24 lines
+    among the sweep's first 16 ell-24 instances, 2000 graph-replayed steps
+    with the Gram correction on BF16 and on FP32.  The scri waveforms of the
+    first instance's dominant modes (ell 2..4) agree to 1e-9 relative L2
+    (measured 2.1e-15 at worst) and all 8 states to 1e-10 (measured 5.8e-14
+    to 1.8e-11)."""
+    allc = W.sweep(512)
+    idx = [k for k, c in enumerate(allc) if c["ellmax_first"] == 24][:16]
+    cases = [allc[k] for k in idx if allc[k]["nr"] == 257]
+    out = {}
+    for mixed in (True, False):
+        with T.Plan(W.texts(cases)) as plan:
+            plan.set_mixed_gram(mixed)
+            plan.set_graph(True)
+            plan.initialize()
+            wf = {}
+            for _ in range(100):
+                plan.advance(20)
+                for r in plan.sample_waveform(0):
+                    if r.boundary == 0 and r.ell <= 4:
+                        wf.setdefault((r.m, r.ell), []).append(complex(r.rescaled_re,
+                                                                       r.rescaled_im))
+            out[mixed] = ([plan.export_state(i) for i in range(len(cases))], wf)
+    states = [H.rel_err(a, b) for a, b in zip(out[True][0], out[False][0])]
+    waves = {}
+    for key, a in out[True][1].items():
+        a, b = np.array(a), np.array(out[False][1][key])
+        waves[key] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    print(f"[gpu] mixed vs FP32 Gram correction, {len(cases)} instances, 2000 steps: state "
+          f"{min(states):.2e} .. {max(states):.2e}; scri ell <= 4 waveforms (L2) "
+          f"{max(waves.values()):.2e}", flush=True)
+    assert len(cases) >= 4 and waves
+    assert max(states) <= 1e-10
+    assert max(waves.values()) <= 1e-9
